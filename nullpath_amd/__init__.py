@@ -1,0 +1,113 @@
+"""nullpath_amd -- MI355X (gfx950) null-geodesic hot path.
+
+Host-side Python surface over libnullpath.so (C ABI in include/nullpath.h):
+the Kerr renderer (`render_tile`, `render_accumulate`).  Device buffers are
+torch tensors on a HIP device; torch is used only for memory and streams.
+"""
+from __future__ import annotations
+
+import math
+
+from ._lib import (COUNTER_FIELDS, NullPathError, RenderParams, TraceCounters, check, lib)
+
+__all__ = ["RenderParams", "TraceCounters", "NullPathError", "default_params", "kerr_disk_params",
+           "render_tile", "render_accumulate", "isco_radius", "lib"]
+
+PI_F = 3.14159265358979323846
+
+
+def _f32(x: float) -> float:
+    import struct
+    return struct.unpack("f", struct.pack("f", x))[0]
+
+
+def default_params(**kw) -> RenderParams:
+    """The reference renderer's defaults (render.cu:15-46, mode=balanced)."""
+    p = RenderParams()
+    p.width, p.height, p.fov_y_deg = 1280, 720, 50.0
+    p.theta_cam = _f32(_f32(90.0 * _f32(PI_F)) / 180.0)
+    p.phi_cam, p.mass, p.spin, p.r_cam_Rs = 0.0, 10.0, 0.0, 50.0
+    p.integrator, p.camera, p.bg_mode = 0, 0, 0
+    p.bg_sigma_rad = _f32(_f32(1.0 * _f32(PI_F)) / 180.0)
+    p.bg_peak, p.disk_on, p.disk_gain, p.disk_max_orders = 3.0, 1, 1.0, 1
+    p.check_inv, p.max_steps, p.precision, p.emiss_m_units = 0, 10000, 0, 0
+    p.rtol, p.atol = 1e-10, 1e-12
+    p.emiss_q, p.seed = 2.0, 0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    mass = float(p.mass)
+    if "disk_rin" not in kw:
+        p.disk_rin = 3.0 * 2.0 * mass
+    if "disk_rout" not in kw:
+        p.disk_rout = 20.0 * 2.0 * mass
+    return p
+
+
+def deg2rad_f32(d: float) -> float:
+    """render.cu:483 in float arithmetic."""
+    return _f32(_f32(_f32(d) * _f32(PI_F)) / 180.0)
+
+
+def isco_radius(a: float) -> float:
+    """Prograde ISCO in units of M (Bardeen, Press & Teukolsky 1972)."""
+    z1 = 1.0 + (1.0 - a * a) ** (1.0 / 3.0) * ((1.0 + a) ** (1.0 / 3.0) + (1.0 - a) ** (1.0 / 3.0))
+    z2 = math.sqrt(3.0 * a * a + z1 * z1)
+    return 3.0 + z2 - math.sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2))
+
+
+def kerr_disk_params(width=1920, height=1080, spin=0.9, theta_deg=80.0, fov=55.0,
+                     rtol=1e-8, max_orders=3, q=3.0, gain=1.0, max_steps=200000,
+                     adaptive=True, **kw) -> RenderParams:
+    """Kerr thin-disk still: pinhole at r = 1000 M (M = 1), disk from the
+    ISCO to 20 M with emissivity (r/M)^-q and g^3 redshift."""
+    p = default_params(width=width, height=height, fov_y_deg=fov, mass=1.0, spin=spin,
+                       r_cam_Rs=500.0, theta_cam=deg2rad_f32(theta_deg),
+                       integrator=1 if adaptive else 0, camera=1 if adaptive else 0,
+                       rtol=rtol, atol=1e-12, disk_max_orders=max_orders, emiss_q=q,
+                       emiss_m_units=1, disk_gain=gain, max_steps=max_steps,
+                       disk_rin=isco_radius(spin), disk_rout=20.0, **kw)
+    return p
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return int(stream)
+
+
+def render_tile(p: RenderParams, out, y0=0, rows=None, spp=1, sample_offset=0, inv=None,
+                counters=None, records=None, stream=None) -> None:
+    """Launch the renderer on rows [y0, y0+rows) into `out` (torch float32
+    device tensor of rows*width*3 elements)."""
+    rows = p.height - y0 if rows is None else rows
+    if out.numel() < rows * p.width * 3 or out.dtype.itemsize != 4 or not out.is_contiguous():
+        raise NullPathError("out must be a contiguous float32 device tensor of rows*width*3")
+    code = lib().np_render_tile(p, y0, rows, spp, sample_offset, out.data_ptr(),
+                                None if inv is None else inv.data_ptr(),
+                                None if counters is None else counters.data_ptr(),
+                                None if records is None else records.data_ptr(),
+                                _stream_ptr(stream))
+    check(code, "np_render_tile")
+
+
+def counters_dict(t) -> dict:
+    """Decode a uint64 counter tensor (8 elements) into a dict."""
+    vals = t.cpu().tolist()
+    return {k: int(v) & ((1 << 64) - 1) for k, v in zip(COUNTER_FIELDS, vals)}
+
+
+def render_accumulate(p: RenderParams, accum, spp_done=0, samples_per_pass=1, passes=1,
+                      tile_h=128, inv_max=None) -> None:
+    """Progressive passes into a host numpy float32 accumulation buffer."""
+    import ctypes as C
+
+    import numpy as np
+    assert accum.dtype == np.float32 and accum.flags.c_contiguous
+    invp = None
+    if inv_max is not None:
+        assert inv_max.dtype == np.float32
+        invp = inv_max.ctypes.data
+    code = lib().np_render_accumulate(p, spp_done, samples_per_pass, passes, tile_h,
+                                      accum.ctypes.data_as(C.c_void_p), invp)
+    check(code, "np_render_accumulate")

@@ -1,0 +1,87 @@
+"""ctypes binding of libnullpath.so (include/nullpath.h).
+
+The library is loaded from this package folder.  There is no fallback: if
+libnullpath.so is missing or cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libnullpath.so")
+
+_lib: C.CDLL | None = None
+
+
+class NullPathError(RuntimeError):
+    pass
+
+
+class RenderParams(C.Structure):
+    """np_render_params (include/np_render.h)."""
+
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("fov_y_deg", C.c_float),
+        ("theta_cam", C.c_float), ("phi_cam", C.c_float), ("mass", C.c_float),
+        ("spin", C.c_float), ("r_cam_Rs", C.c_float), ("integrator", C.c_int32),
+        ("camera", C.c_int32), ("bg_mode", C.c_int32), ("bg_sigma_rad", C.c_float),
+        ("bg_peak", C.c_float), ("disk_on", C.c_int32), ("disk_gain", C.c_float),
+        ("disk_max_orders", C.c_int32), ("check_inv", C.c_int32),
+        ("max_steps", C.c_int32), ("precision", C.c_int32), ("emiss_m_units", C.c_int32),
+        ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
+        ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
+    ]
+
+    def copy(self) -> "RenderParams":
+        q = RenderParams()
+        C.pointer(q)[0] = self
+        return q
+
+
+class TraceCounters(C.Structure):
+    """np_trace_counters (include/nullpath.h)."""
+
+    _fields_ = [(n, C.c_uint64) for n in (
+        "rays", "steps", "attempts", "rhs_evals", "captured", "escaped", "stepcap", "disk_hits")]
+
+
+COUNTER_FIELDS = [f[0] for f in TraceCounters._fields_]
+
+# np_ray_record as a numpy dtype (32 bytes)
+def ray_record_dtype():
+    import numpy as np
+    return np.dtype([("cls", np.int32), ("orders", np.int32), ("steps", np.int32),
+                     ("attempts", np.int32), ("min_r", np.float64), ("r_end", np.float64)])
+
+
+def lib() -> C.CDLL:
+    """Load libnullpath.so (raises NullPathError when it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NullPathError(
+                f"{LIB_PATH} is missing: build it with `python -m nullpath_amd.build`")
+        try:
+            L = C.CDLL(LIB_PATH)
+        except OSError as e:  # pragma: no cover - depends on the machine
+            raise NullPathError(f"cannot load {LIB_PATH}: {e}") from e
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L: C.CDLL) -> None:
+    i32, vp = C.c_int, C.c_void_p
+    L.np_status_string.argtypes = [i32]
+    L.np_status_string.restype = C.c_char_p
+    L.np_render_tile.argtypes = [C.POINTER(RenderParams), i32, i32, i32, i32, vp, vp, vp, vp, vp]
+    L.np_render_tile.restype = i32
+    L.np_render_accumulate.argtypes = [C.POINTER(RenderParams), i32, i32, i32, i32, vp, vp]
+    L.np_render_accumulate.restype = i32
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        msg = lib().np_status_string(code).decode()
+        raise NullPathError(f"{what} failed: {msg} ({code})")

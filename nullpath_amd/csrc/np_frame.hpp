@@ -1,0 +1,39 @@
+// np_frame.hpp -- per-frame constants of a Kerr render, computed once on the
+// host in FP64 and handed to the kernels by value (they are staged into LDS
+// or kept in SGPRs).  Everything that depends only on the camera position
+// -- the observer tetrad, metric blocks at the camera, the pinhole scale --
+// is folded here so the per-ray camera set-up is a handful of multiplies.
+#pragma once
+
+#include <stdint.h>
+#include "np_render.h"
+
+#define NP_PI_F_HOST 3.14159265358979323846f
+
+namespace np {
+
+struct FrameConst {
+    // geometry (absolute units)
+    double M, a, r_plus, r_cam, th_cam, ph_cam, Rs_d, r_far, rin, rout;
+    double rtol, atol;
+    // observer tetrad: p^mu components as functions of the local direction n
+    //   p^t   = ct0 + n_phi * ct_ph
+    //   p^r   = n_r * cr
+    //   p^th  = n_th * cth
+    //   p^phi = cph0 + n_phi * cph_ph
+    double ct0, ct_ph, cr, cth, cph0, cph_ph;
+    // covariant metric at the camera, for lowering p^mu
+    double g_tt, g_tphi, g_phiphi, g_rr, g_thth;
+    // float pinhole camera (render.cu:137-139)
+    float aspect, tan_half, Rs, mass, disk_gain, emiss_q, bg_sigma_rad, bg_peak;
+    uint32_t salt;        // jitter salt (0 = reference hash keys)
+    int32_t width, height;
+    int32_t zamo;         // 1 when the ZAMO tetrad is used
+    int32_t bg_mode, disk_on, disk_max_orders, check_inv, max_steps, emiss_m_units;
+};
+
+// Returns 0 on success, NP_EINVAL when the parameters are outside what the
+// reference accepts (render.cu:431-465).
+int make_frame(const np_render_params* p, FrameConst* f);
+
+}  // namespace np

@@ -1,0 +1,134 @@
+/*
+ * np_oracle.h -- CPU oracle for the null-geodesic hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the
+ * reference algorithms, used by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg as the checker.  The product library
+ * (libnullpath.so) never links, loads or calls it.
+ *
+ * Each function cites the reference file:line it restates.  How each one is
+ * pinned (golden vectors, closed forms, invariants) is listed in the test
+ * files that exercise it.
+ */
+#ifndef NP_ORACLE_H
+#define NP_ORACLE_H
+
+#include <stdint.h>
+#include "../include/np_render.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Kerr metric blocks, FP64 (include/bh_kerr.h:575-672) ---- */
+/* out = {g_tt, g_tphi, g_phiphi, g_rr, g_thth} */
+void orc_kerr_blocks_cov_d(double M, double a, double r, double th, double out[5]);
+/* out = {g^tt, g^tphi, g^phiphi, g^rr, g^thth} */
+void orc_kerr_blocks_contra_d(double M, double a, double r, double th, double out[5]);
+/* out = {d_r g^tt, d_r g^tphi, d_r g^phiphi, d_r g^rr, d_r g^thth,
+ *        d_th g^tt, ..., d_th g^thth} */
+void orc_kerr_blocks_contra_derivs_d(double M, double a, double r, double th, double out[10]);
+
+/* State vectors are {t, r, theta, phi, p_t, p_r, p_theta, p_phi}
+ * (GeodesicStateD member order, bh_kerr.h:553-556). */
+void orc_kerr_rhs_d(const double s[8], double M, double a, double d[8]);
+double orc_kerr_hamiltonian_d(const double s[8], double M, double a);
+/* integrateGeodesicKerrD: in-place fixed step, returns continue flag. */
+int orc_kerr_rk4_fixed_d(double s[8], double M, double a, double h);
+/* rk4_step_d: out-of-place step, returns ok flag. */
+int orc_kerr_rk4_step_d(const double s[8], double M, double a, double h, double out[8]);
+/* rk4_adaptive_step_d: returns growth factor (>0 accepted, 0 rejected);
+ * on acceptance s is replaced, on rejection *h is shrunk. */
+double orc_kerr_adaptive_step_d(double s[8], double M, double a, double* h,
+                                double rtol, double atol);
+
+/* Cameras (bh_kerr.h:781-826, 880-884, 910-967) and Doppler (833-862). */
+void orc_init_static_d(double M, double a, double r, double th, double ph,
+                       double n_r, double n_th, double n_ph, double s[8]);
+void orc_init_zamo_d(double M, double a, double r, double th, double ph,
+                     double n_r, double n_th, double n_ph, double s[8]);
+int orc_static_timelike_d(double M, double a, double r, double th);
+double orc_doppler_d(const double s[8], double M, double a);
+
+/* Compensated summation (bh_common.h:451-493). */
+double orc_rk4_sum_d(double k1, double k2, double k3, double k4);
+
+/* ---- Per-pixel renderer (render.cu:124-363, FP64 path) ---- */
+typedef struct orc_ray_info {
+    int32_t cls;          /* np_ray_class */
+    int32_t orders;       /* disk crossings accumulated */
+    int64_t steps;        /* accepted steps */
+    int64_t attempts;     /* adaptive attempts (== steps in fixed mode) */
+    int64_t rhs_evals;    /* Hamilton right-hand-side evaluations, reference count */
+    double  min_r;
+    float   inv[3];       /* |H|, dE/E, dL/L at crossings/ends (render.cu:233-244,327-345) */
+    float   rgb[3];       /* this sample's colour */
+} orc_ray_info;
+
+/* Camera-space unit direction of sample `sample_id` of pixel (x, y)
+ * (render.cu:137-159).  n = {n_r, n_theta, n_phi}; cam = {cam_x, cam_y}. */
+void orc_camera_dir(const np_render_params* p, int x, int y, uint32_t sample_id,
+                    float n[3], float cam[2]);
+uint32_t orc_wanghash(uint32_t a);
+
+/* Trace one sample; returns its colour in info->rgb. */
+void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
+                    orc_ray_info* info);
+/* Sum of `spp` samples starting at sample_offset for each listed pixel, in
+ * the reference's order (render.cu:141-363).  rgb is npix*3 floats.
+ * info (optional) gets npix*spp entries, pixel-major.  nthreads <= 0 uses
+ * every core. */
+void orc_render_pixels(const np_render_params* p, int npix, const int32_t* xs,
+                       const int32_t* ys, int sample_offset, int spp, float* rgb,
+                       orc_ray_info* info, int nthreads);
+
+/* ---- Schwarzschild (include/bh_schwarzschild.h, schwarzschild_blackhole.cu) ---- */
+/* Reference FP32 precompute kernel body (schwarzschild_blackhole.cu:48-139). */
+void orc_schw_precompute_f32(float mass, float r_start, const float* b, int n,
+                             float* defl, float* rmin, int32_t* status, int nthreads);
+int orc_schw_rmin_analytic(float Rs, float b, float* rmin_out);
+/* FP64 equatorial sweep: RK4 of the Schwarzschild Hamiltonian restated in
+ * FP64, step law of schwarzschild_blackhole.cu:84-85, launched from r0 with
+ * E=1, L=b; captured when r < r_capture, escaped when r > r0 with p_r > 0,
+ * photon-sphere class when |b - 3 sqrt(3) M| < ps_band.  Outputs class,
+ * unwrapped total dphi and step count. */
+void orc_schw_sweep_d(double M, double r0, double r_capture, double ps_band,
+                      int64_t max_steps, const double* b, int n, int32_t* cls,
+                      double* dphi, int64_t* steps, int nthreads);
+
+/* ---- Carlson symmetric integrals (geokerr_original.f:3444-3489, 3608-3762) ---- */
+double orc_rf(double x, double y, double z);
+double orc_rc(double x, double y);
+double orc_rd(double x, double y, double z);
+double orc_rj(double x, double y, double z, double p);
+
+/* ---- GEOKERR GEOMU and its numerics (geokerr_original.f:771-1186, 3137-3441) ---- */
+typedef struct orc_geomu_out {
+    double muf;       /* final mu = cos(theta) */
+    double iu;        /* I_u integral between u0 and uf */
+    double u1, u2, u3, u4;  /* real roots of U(u) where the case has them */
+    double u0_used, uf_used;  /* after the reference's clamps/warning edits */
+    double i1mu, i3mu;
+    double h1;        /* Eq. (21) root, ncase 6 */
+    int32_t tpm;      /* mu turning points */
+    int32_t ncase;    /* Table 1 case 1..8 (0 = error path) */
+    int32_t tpr;      /* radial turning points (as modified) */
+    int32_t warned;   /* 1 when the reference would print a WARNING and modify u0 */
+} orc_geomu_out;
+
+void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
+               double su, double sm, orc_geomu_out* o);
+void orc_sncndn(double uu, double emmc, double* sn, double* cn, double* dn);
+double orc_asech(double x);
+void orc_geokerr_camera(double alpha, double beta, double mu0, double a_in, double* L,
+                        double* Q2, double* a_out, double* su, double* sm);
+/* Batch of GEOMU solves from screen coordinates (config: u0 fixed, tpr = 0). */
+void orc_geomu_batch(const double* alpha, const double* beta, const double* a, const double* mu0,
+                     const double* uf, double u0, int n, double* muf, int32_t* tpm, int32_t* ncase,
+                     double* iu, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NP_ORACLE_H */

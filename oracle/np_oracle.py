@@ -1,0 +1,277 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker of the HIP library,
+never as a product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class RenderParams(C.Structure):
+    """Mirror of np_render_params (include/np_render.h)."""
+
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("fov_y_deg", C.c_float),
+        ("theta_cam", C.c_float), ("phi_cam", C.c_float), ("mass", C.c_float),
+        ("spin", C.c_float), ("r_cam_Rs", C.c_float), ("integrator", C.c_int32),
+        ("camera", C.c_int32), ("bg_mode", C.c_int32), ("bg_sigma_rad", C.c_float),
+        ("bg_peak", C.c_float), ("disk_on", C.c_int32), ("disk_gain", C.c_float),
+        ("disk_max_orders", C.c_int32), ("check_inv", C.c_int32),
+        ("max_steps", C.c_int32), ("precision", C.c_int32), ("emiss_m_units", C.c_int32),
+        ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
+        ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
+    ]
+
+
+def default_params(**kw) -> RenderParams:
+    """Reference renderer defaults (render.cu:15-46, mode=balanced)."""
+    p = RenderParams()
+    p.width, p.height, p.fov_y_deg = 1280, 720, 50.0
+    p.theta_cam = np.float32(90.0) * np.float32(np.pi) / np.float32(180.0)
+    p.phi_cam, p.mass, p.spin, p.r_cam_Rs = 0.0, 10.0, 0.0, 50.0
+    p.integrator, p.camera, p.bg_mode = 0, 0, 0
+    p.bg_sigma_rad = np.float32(1.0) * np.float32(np.pi) / np.float32(180.0)
+    p.bg_peak, p.disk_on, p.disk_gain, p.disk_max_orders = 3.0, 1, 1.0, 1
+    p.check_inv, p.max_steps, p.precision, p.emiss_m_units = 0, 10000, 0, 0
+    p.rtol, p.atol = 1e-10, 1e-12
+    p.disk_rin, p.disk_rout = 6.0 * p.mass, 40.0 * p.mass
+    p.emiss_q, p.seed = 2.0, 0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    if "mass" in kw and "disk_rin" not in kw:
+        p.disk_rin = 6.0 * float(np.float32(p.mass))
+    if "mass" in kw and "disk_rout" not in kw:
+        p.disk_rout = 40.0 * float(np.float32(p.mass))
+    return p
+
+
+class RayInfo(C.Structure):
+    _fields_ = [
+        ("cls", C.c_int32), ("orders", C.c_int32), ("steps", C.c_int64),
+        ("attempts", C.c_int64), ("rhs_evals", C.c_int64), ("min_r", C.c_double),
+        ("inv", C.c_float * 3), ("rgb", C.c_float * 3),
+    ]
+
+
+RAYINFO_DTYPE = np.dtype([
+    ("cls", np.int32), ("orders", np.int32), ("steps", np.int64), ("attempts", np.int64),
+    ("rhs_evals", np.int64), ("min_r", np.float64), ("inv", np.float32, 3),
+    ("rgb", np.float32, 3),
+], align=True)
+
+
+def build() -> None:
+    """Compile liboracle.so from the C sources in this folder."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L: C.CDLL) -> None:
+    d, i32, pd = C.c_double, C.c_int, C.POINTER(C.c_double)
+    for name in ("orc_kerr_blocks_cov_d", "orc_kerr_blocks_contra_d",
+                 "orc_kerr_blocks_contra_derivs_d"):
+        f = getattr(L, name)
+        f.argtypes = [d, d, d, d, pd]
+        f.restype = None
+    L.orc_kerr_rhs_d.argtypes = [pd, d, d, pd]
+    L.orc_kerr_hamiltonian_d.argtypes = [pd, d, d]
+    L.orc_kerr_hamiltonian_d.restype = d
+    L.orc_kerr_rk4_fixed_d.argtypes = [pd, d, d, d]
+    L.orc_kerr_rk4_fixed_d.restype = i32
+    L.orc_kerr_rk4_step_d.argtypes = [pd, d, d, d, pd]
+    L.orc_kerr_rk4_step_d.restype = i32
+    L.orc_kerr_adaptive_step_d.argtypes = [pd, d, d, pd, d, d]
+    L.orc_kerr_adaptive_step_d.restype = d
+    for name in ("orc_init_static_d", "orc_init_zamo_d"):
+        f = getattr(L, name)
+        f.argtypes = [d, d, d, d, d, d, d, d, pd]
+        f.restype = None
+    L.orc_static_timelike_d.argtypes = [d, d, d, d]
+    L.orc_static_timelike_d.restype = i32
+    L.orc_doppler_d.argtypes = [pd, d, d]
+    L.orc_doppler_d.restype = d
+    L.orc_rk4_sum_d.argtypes = [d, d, d, d]
+    L.orc_rk4_sum_d.restype = d
+    L.orc_wanghash.argtypes = [C.c_uint32]
+    L.orc_wanghash.restype = C.c_uint32
+    L.orc_camera_dir.argtypes = [C.POINTER(RenderParams), i32, i32, C.c_uint32,
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_render_ray.argtypes = [C.POINTER(RenderParams), i32, i32, C.c_uint32,
+                                 C.POINTER(RayInfo)]
+    L.orc_render_pixels.argtypes = [C.POINTER(RenderParams), i32, C.c_void_p, C.c_void_p,
+                                    i32, i32, C.c_void_p, C.c_void_p, i32]
+    L.orc_schw_precompute_f32.argtypes = [C.c_float, C.c_float, C.c_void_p, i32, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, i32]
+    L.orc_schw_rmin_analytic.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float)]
+    L.orc_schw_rmin_analytic.restype = i32
+    L.orc_schw_sweep_d.argtypes = [d, d, d, d, C.c_int64, C.c_void_p, i32, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, i32]
+    for name, n in (("orc_rf", 3), ("orc_rc", 2), ("orc_rd", 3), ("orc_rj", 4)):
+        f = getattr(L, name)
+        f.argtypes = [d] * n
+        f.restype = d
+
+
+def _arr(x, dtype=np.float64):
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---- thin numpy-facing helpers ----
+def blocks_cov(M, a, r, th):
+    out = np.zeros(5)
+    lib().orc_kerr_blocks_cov_d(M, a, r, th, _ptr(out))
+    return out
+
+
+def blocks_contra(M, a, r, th):
+    out = np.zeros(5)
+    lib().orc_kerr_blocks_contra_d(M, a, r, th, _ptr(out))
+    return out
+
+
+def blocks_contra_derivs(M, a, r, th):
+    out = np.zeros(10)
+    lib().orc_kerr_blocks_contra_derivs_d(M, a, r, th, _ptr(out))
+    return out
+
+
+def rhs(s, M, a):
+    s = _arr(s)
+    d = np.zeros(8)
+    lib().orc_kerr_rhs_d(_ptr(s), M, a, _ptr(d))
+    return d
+
+
+def hamiltonian(s, M, a):
+    s = _arr(s)
+    return lib().orc_kerr_hamiltonian_d(_ptr(s), M, a)
+
+
+def rk4_fixed(s, M, a, h):
+    s = _arr(s).copy()
+    ok = lib().orc_kerr_rk4_fixed_d(_ptr(s), M, a, h)
+    return s, bool(ok)
+
+
+def rk4_step(s, M, a, h):
+    s = _arr(s)
+    out = np.zeros(8)
+    ok = lib().orc_kerr_rk4_step_d(_ptr(s), M, a, h, _ptr(out))
+    return out, bool(ok)
+
+
+def adaptive_step(s, M, a, h, rtol=1e-10, atol=1e-12):
+    s = _arr(s).copy()
+    hh = np.array([h], dtype=np.float64)
+    fac = lib().orc_kerr_adaptive_step_d(_ptr(s), M, a, _ptr(hh), rtol, atol)
+    return s, float(hh[0]), fac
+
+
+def init_static(M, a, r, th, ph, n):
+    s = np.zeros(8)
+    lib().orc_init_static_d(M, a, r, th, ph, float(n[0]), float(n[1]), float(n[2]), _ptr(s))
+    return s
+
+
+def init_zamo(M, a, r, th, ph, n):
+    s = np.zeros(8)
+    lib().orc_init_zamo_d(M, a, r, th, ph, float(n[0]), float(n[1]), float(n[2]), _ptr(s))
+    return s
+
+
+def doppler(s, M, a):
+    s = _arr(s)
+    return lib().orc_doppler_d(_ptr(s), M, a)
+
+
+def camera_dir(p: RenderParams, x: int, y: int, sample_id: int):
+    n = (C.c_float * 3)()
+    cam = (C.c_float * 2)()
+    lib().orc_camera_dir(C.byref(p), x, y, sample_id, n, cam)
+    return np.array(n[:], dtype=np.float32), np.array(cam[:], dtype=np.float32)
+
+
+def render_pixels(p: RenderParams, xs, ys, sample_offset=0, spp=1, nthreads=0, with_info=True):
+    """Per-pixel colour sums (npix, 3) and per-ray info (npix*spp) records."""
+    xs = _arr(xs, np.int32)
+    ys = _arr(ys, np.int32)
+    n = xs.size
+    rgb = np.zeros((n, 3), dtype=np.float32)
+    info = np.zeros(n * spp, dtype=RAYINFO_DTYPE) if with_info else None
+    assert RAYINFO_DTYPE.itemsize == C.sizeof(RayInfo)
+    lib().orc_render_pixels(C.byref(p), n, xs.ctypes.data, ys.ctypes.data, sample_offset, spp,
+                            rgb.ctypes.data, info.ctypes.data if with_info else None, nthreads)
+    return rgb, info
+
+
+def render_image(p: RenderParams, spp=1, sample_offset=0, nthreads=0):
+    ys, xs = np.mgrid[0:p.height, 0:p.width]
+    rgb, info = render_pixels(p, xs.ravel(), ys.ravel(), sample_offset, spp, nthreads)
+    return rgb.reshape(p.height, p.width, 3), info
+
+
+def schw_precompute_f32(mass, r_start, b, nthreads=0):
+    b = _arr(b, np.float32)
+    n = b.size
+    defl = np.zeros(n, np.float32)
+    rmin = np.zeros(n, np.float32)
+    st = np.zeros(n, np.int32)
+    lib().orc_schw_precompute_f32(mass, r_start, b.ctypes.data, n, defl.ctypes.data,
+                                  rmin.ctypes.data, st.ctypes.data, nthreads)
+    return defl, rmin, st
+
+
+def schw_rmin_analytic(Rs, b):
+    out = C.c_float(0.0)
+    ok = lib().orc_schw_rmin_analytic(Rs, b, C.byref(out))
+    return bool(ok), out.value
+
+
+def schw_sweep(M, r0, r_capture, ps_band, max_steps, b, nthreads=0):
+    b = _arr(b)
+    n = b.size
+    cls = np.zeros(n, np.int32)
+    dphi = np.zeros(n, np.float64)
+    steps = np.zeros(n, np.int64)
+    lib().orc_schw_sweep_d(M, r0, r_capture, ps_band, max_steps, b.ctypes.data, n,
+                           cls.ctypes.data, dphi.ctypes.data, steps.ctypes.data, nthreads)
+    return cls, dphi, steps
+
+
+def rf(x, y, z):
+    return lib().orc_rf(x, y, z)
+
+
+def rc(x, y):
+    return lib().orc_rc(x, y)
+
+
+def rd(x, y, z):
+    return lib().orc_rd(x, y, z)
+
+
+def rj(x, y, z, p):
+    return lib().orc_rj(x, y, z, p)

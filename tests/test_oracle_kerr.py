@@ -1,0 +1,246 @@
+"""Pins for the FP64 Kerr oracle (oracle/kerr_oracle.c) that do not restate it.
+
+* metric blocks: the textbook Boyer-Lindquist metric in a different
+  algebraic form, and the contravariant blocks as its matrix inverse;
+* analytic derivatives: the reference's own finite-difference self-test
+  (tests/test_kerr_derivs.cu) tightened to FP64 tolerances;
+* right-hand side: Hamilton's equations against finite-difference gradients
+  of H;
+* RK4: fourth-order convergence; conservation of H, E, L_z and Carter's Q;
+* cameras: H = 0 at launch, unit local energy, zero L_z of a ZAMO photon
+  with n_phi = 0;
+* Doppler factor: closed-form Keplerian redshift (Bardeen, Press &
+  Teukolsky 1972);
+* classification: capture iff the radial potential R(r) has no turning
+  point between the horizon and the camera.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import nullpath_amd as npa
+
+
+def bl_metric_textbook(M, a, r, th):
+    s, c = math.sin(th), math.cos(th)
+    Sig = r * r + a * a * c * c
+    Del = r * r - 2 * M * r + a * a
+    g_tt = -(1 - 2 * M * r / Sig)
+    g_tph = -2 * M * a * r * s * s / Sig
+    g_phph = (r * r + a * a + 2 * M * a * a * r * s * s / Sig) * s * s
+    return g_tt, g_tph, g_phph, Sig / Del, Sig
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9, 0.998])
+def test_cov_blocks_match_textbook_form(oracle, a):
+    rng = np.random.default_rng(1)
+    rp = 1 + math.sqrt(1 - a * a)
+    for _ in range(200):
+        r = rp * 1.01 + rng.random() * 50
+        th = 0.05 + 3.04 * rng.random()
+        got = oracle.blocks_cov(1.0, a, r, th)
+        ref = bl_metric_textbook(1.0, a, r, th)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9])
+def test_contra_blocks_are_inverse(oracle, a):
+    rng = np.random.default_rng(2)
+    rp = 1 + math.sqrt(1 - a * a)
+    for _ in range(200):
+        r = rp * 1.01 + rng.random() * 100
+        th = 0.05 + 3.04 * rng.random()
+        g = oracle.blocks_cov(1.0, a, r, th)
+        G = oracle.blocks_contra(1.0, a, r, th)
+        low = np.array([[g[0], g[1]], [g[1], g[2]]])
+        up = np.array([[G[0], G[1]], [G[1], G[2]]])
+        np.testing.assert_allclose(up @ low, np.eye(2), atol=1e-10)
+        assert abs(G[3] * g[3] - 1) < 1e-12 and abs(G[4] * g[4] - 1) < 1e-12
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9])
+def test_contra_derivs_vs_five_point_fd(oracle, a):
+    """tests/test_kerr_derivs.cu sampling (r in [r+ + 1e-3, 50], theta in
+    [0.05 pi, 0.95 pi], 5-point stencil), tolerance 5e-3 tightened to 1e-6."""
+    rng = np.random.default_rng(12345 ^ int(round(1000 * a)))
+    rp = 1 + math.sqrt(max(0.0, 1 - a * a))
+    rmin, rmax = rp + 1e-3, 50.0
+    worst = 0.0
+    for _ in range(300):
+        r = rmin * (rmax / rmin) ** rng.random()
+        th = (0.05 + 0.90 * rng.random()) * math.pi
+        # step shrunk near the horizon so the stencil stays well inside r > r+
+        hr = min(max(1e-4 * max(1.0, r), 1e-6), 2e-2 * (r - rp))
+        hth = 1e-4
+        an = oracle.blocks_contra_derivs(1.0, a, r, th)
+
+        def f(rr, tt):
+            return oracle.blocks_contra(1.0, a, rr, tt)
+
+        fd_r = (-f(r + 2 * hr, th) + 8 * f(r + hr, th) - 8 * f(r - hr, th) + f(r - 2 * hr, th)) / (12 * hr)
+        fd_t = (-f(r, th + 2 * hth) + 8 * f(r, th + hth) - 8 * f(r, th - hth) + f(r, th - 2 * hth)) / (12 * hth)
+        fd = np.concatenate([fd_r, fd_t])
+        # mixed criterion as in the reference test (absolute near zero):
+        # FD round-off of a derivative that is exactly zero is ~1e-11
+        scale = np.maximum(np.abs(an), np.abs(fd))
+        worst = max(worst, float(np.max(np.abs(an - fd) - (1e-8 + 1e-6 * scale))))
+    assert worst <= 0.0, worst
+
+
+def _H(oracle, s, M, a):
+    return oracle.hamiltonian(s, M, a)
+
+
+@pytest.mark.parametrize("a", [0.0, 0.7, 0.95])
+def test_rhs_is_hamiltons_equations(oracle, a):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        s = np.array([0.0, 3 + 20 * rng.random(), 0.2 + 2.7 * rng.random(), 0.3,
+                      -1.0, rng.normal(), rng.normal() * 3, rng.normal() * 3])
+        d = oracle.rhs(s, 1.0, a)
+        grad = np.zeros(8)
+        for i in range(8):
+            h = 1e-6 * max(1.0, abs(s[i]))
+            sp, sm = s.copy(), s.copy()
+            sp[i] += h
+            sm[i] -= h
+            grad[i] = (_H(oracle, sp, 1.0, a) - _H(oracle, sm, 1.0, a)) / (2 * h)
+        # x' = dH/dp, p' = -dH/dx (t and phi are cyclic)
+        np.testing.assert_allclose(d[1], grad[5], rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(d[2], grad[6], rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(d[3], grad[7], rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(d[0], grad[4], rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(d[5], -grad[1], rtol=1e-6, atol=1e-9)
+        np.testing.assert_allclose(d[6], -grad[2], rtol=1e-6, atol=1e-9)
+        assert d[4] == 0.0 and d[7] == 0.0
+
+
+def _photon(oracle, a, r=30.0, th=1.2, n=(-0.8, 0.36, 0.48)):
+    n = np.array(n) / np.linalg.norm(n)
+    return oracle.init_zamo(1.0, a, r, th, 0.0, n)
+
+
+def test_rk4_fourth_order(oracle):
+    a = 0.8
+    s0 = _photon(oracle, a, r=6.0, n=(-0.5, 0.5, 0.7))
+    lam = 4.0
+
+    def run(h):
+        s = s0.copy()
+        for _ in range(int(round(lam / h))):
+            s, _ = oracle.rk4_fixed(s, 1.0, a, h)
+        return s
+
+    ref = run(0.003125)
+    e1 = np.abs(run(0.2) - ref)[1:4].max()
+    e2 = np.abs(run(0.1) - ref)[1:4].max()
+    e3 = np.abs(run(0.05) - ref)[1:4].max()
+    assert 14.0 < e1 / e2 < 18.0 and 14.0 < e2 / e3 < 18.0, (e1 / e2, e2 / e3)
+
+
+def carter_Q(s, a):
+    E, L = -s[4], s[7]
+    c2 = math.cos(s[2]) ** 2
+    return s[6] ** 2 + c2 * (L * L / (1 - c2) - a * a * E * E)
+
+
+@pytest.mark.parametrize("a", [0.3, 0.9])
+def test_adaptive_conserves_invariants(oracle, a):
+    s = _photon(oracle, a)
+    Q0 = carter_Q(s, a)
+    E0, L0 = s[4], s[7]
+    h = 0.05
+    for _ in range(1500):
+        s2, h2, fac = oracle.adaptive_step(s, 1.0, a, h, 1e-10, 1e-12)
+        if fac > 0:
+            s, h = s2, min(max(h2 * fac, 1e-6), 0.2)
+        else:
+            h = max(h2, 1e-6)
+        if s[1] > 80 or s[1] < 2.2:
+            break
+    assert s[4] == E0 and s[7] == L0
+    assert abs(_H(oracle, s, 1.0, a)) < 1e-9
+    assert abs(carter_Q(s, a) - Q0) < 1e-8 * max(1.0, abs(Q0))
+
+
+@pytest.mark.parametrize("cam", ["static", "zamo"])
+@pytest.mark.parametrize("a", [0.0, 0.6, 0.95])
+def test_camera_rays_are_null_with_unit_local_energy(oracle, cam, a):
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        r, th = 20 + 30 * rng.random(), 0.3 + 2.5 * rng.random()
+        init = oracle.init_zamo if cam == "zamo" else oracle.init_static
+        s = init(1.0, a, r, th, 0.1, n)
+        assert abs(_H(oracle, s, 1.0, a)) < 1e-11
+    # ZAMO photons with n_phi = 0 carry no angular momentum
+    s = oracle.init_zamo(1.0, 0.9, 25.0, 1.0, 0.0, np.array([-0.6, 0.8, 0.0]))
+    assert abs(s[7]) < 1e-12
+
+
+def keplerian_redshift(a, r, b):
+    """g = 1 / (u^t (1 - Omega b)) for a prograde circular equatorial orbit (M=1)."""
+    ut = (r ** 1.5 + a) / (r ** 0.75 * math.sqrt(r ** 1.5 - 3 * math.sqrt(r) + 2 * a))
+    Om = 1.0 / (r ** 1.5 + a)
+    return 1.0 / (ut * (1 - Om * b))
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9])
+def test_doppler_closed_form(oracle, a):
+    for r in [npa.isco_radius(a) * 1.01, 8.0, 15.0, 40.0]:
+        for b in [-6.0, -1.0, 0.0, 2.5, 7.0]:
+            if 1.0 - b / (r ** 1.5 + a) <= 0.05:
+                continue  # photon cannot leave this orbit toward infinity with this b
+            s = np.array([0, r, math.pi / 2, 0, -1.0, 0.1, 0.2, b])
+            np.testing.assert_allclose(oracle.doppler(s, 1.0, a), keplerian_redshift(a, r, b), rtol=1e-12)
+
+
+def test_isco_known_values():
+    assert abs(npa.isco_radius(0.0) - 6.0) < 1e-14
+    assert abs(npa.isco_radius(1.0) - 1.0) < 1e-12
+    assert abs(npa.isco_radius(0.9) - 2.3208830) < 1e-6
+    assert abs(npa.isco_radius(0.5) - 4.2330025) < 1e-6
+
+
+def _radial_turning_point(E, L, Q, a, r_lo, r_hi):
+    """True when R(r) = ((r^2+a^2)E - aL)^2 - Delta (Q + (L - aE)^2) has a
+    root in (r_lo, r_hi)."""
+    K = Q + (L - a * E) ** 2
+    # R(r) = E^2 r^4 + (a^2E^2 - L^2 - Q) r^2 ... expand exactly
+    c4 = E * E
+    c3 = 0.0
+    c2 = 2 * a * a * E * E - 2 * a * L * E - K
+    c1 = 2 * K
+    c0 = (a * a * E - a * L) ** 2 - a * a * K
+    roots = np.roots([c4, c3, c2, c1, c0])
+    real = roots[np.abs(roots.imag) < 1e-9].real
+    return bool(np.any((real > r_lo) & (real < r_hi)))
+
+
+def test_classification_matches_radial_potential(oracle):
+    """Rays of a small frame: captured <=> no radial turning point, except
+    rays whose turning point sits within 2% of the photon orbit radius."""
+    p = oracle.default_params(width=48, height=30, mass=1.0, r_cam_Rs=25.0, spin=0.9,
+                              theta_cam=np.float32(np.deg2rad(75.0)), fov_y_deg=30.0,
+                              integrator=1, camera=1, rtol=1e-9, max_steps=200000)
+    ys, xs = np.mgrid[0:p.height, 0:p.width]
+    _, info = oracle.render_pixels(p, xs.ravel(), ys.ravel(), 0, 1)
+    M, a = 1.0, 0.9
+    r_cam = 50.0
+    rp = 1 + math.sqrt(1 - a * a)
+    n_checked = n_bad = 0
+    for i, (x, y) in enumerate(zip(xs.ravel(), ys.ravel())):
+        n, _ = oracle.camera_dir(p, int(x), int(y), 0)
+        s = oracle.init_zamo(M, a, r_cam, float(np.float32(p.theta_cam)), 0.0, n.astype(np.float64))
+        E, L = -s[4], s[7]
+        Q = carter_Q(s, a)
+        tp = _radial_turning_point(E, L, Q, a, rp, r_cam)
+        if abs(info["min_r"][i] - 3.0) < 0.5 and info["cls"][i] != 1:
+            continue
+        n_checked += 1
+        captured = info["cls"][i] == 1
+        n_bad += captured == tp
+    assert n_checked > 1000
+    assert n_bad == 0, n_bad
