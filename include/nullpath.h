@@ -82,6 +82,92 @@ int np_render_tile(const np_render_params* p, int y0, int rows, int spp, int sam
 int np_render_accumulate(const np_render_params* p, int spp_done, int samples_per_pass,
                          int passes, int tile_h, float* h_accum, float* inv_max);
 
+/* ---------------------------------------------------------------------
+ * GEOKERR semi-analytic solver (geokerr_port/reference/geokerr_original.f)
+ * ------------------------------------------------------------------- */
+
+enum np_carlson_kind { NP_CARLSON_RF = 0, NP_CARLSON_RC = 1, NP_CARLSON_RD = 2, NP_CARLSON_RJ = 3 };
+
+/*
+ * Batched Carlson symmetric integrals in FP64: out[i] = R_kind(x[i], y[i]
+ * [, z[i] [, p[i]]]).  RC ignores z and p, RF and RD ignore p; RJ with
+ * p <= 0 returns the Cauchy principal value.  Replaces the device functions
+ * carlson_rf/rc/rd/rj and elliptic_batch_kernel of
+ * geokerr_port/cuda/carlson_elliptic.cu:77-351 with the algorithm of
+ * geokerr_original.f:3444-3489, 3608-3762 (the CUDA file's RJ is wrong).
+ */
+int np_carlson_batch(int kind, const double* d_x, const double* d_y, const double* d_z,
+                     const double* d_p, int64_t n, double* d_out, void* stream);
+
+/*
+ * GEOMU for n rays (geokerr_original.f:771-1186, FIRSTPT = .true.):
+ * final mu = cos(theta) at inverse radius uf[i] for a photon that starts at
+ * (u0[i], mu0[i]) with constants of motion L[i], Q2[i], spin a = spin[i],
+ * initial signs su[i] = du/dlambda, sm[i] = dmu/dlambda, and tpr[i] radial
+ * turning points between u0 and uf (NULL = all zero).  Outputs: muf, and
+ * (each optional) tpm = mu turning points, ncase = Table 1 case, iu = the
+ * Mino-time integral I_u.  All arrays are device arrays of n elements.
+ */
+int np_geomu_batch(const double* d_u0, const double* d_uf, const double* d_mu0,
+                   const double* d_spin, const double* d_L, const double* d_Q2,
+                   const double* d_su, const double* d_sm, const int32_t* d_tpr, int64_t n,
+                   double* d_muf, int32_t* d_tpm, int32_t* d_ncase, double* d_iu, void* stream);
+
+/*
+ * GEOMU from screen coordinates: per ray (alpha, beta) at infinity, spin and
+ * mu0 = cos(theta_obs) give L = -alpha sqrt(1 - mu0^2),
+ * Q2 = beta^2 - (a^2 - alpha^2) mu0^2, su = 1, sm = +1 for beta >= 0
+ * (INITIALIZE_CAMERA_GEOKERR, geokerr_original.f:238-320, inputs below 1e-5
+ * zeroed), then GEOMU from the common start u0 to uf[i] with tpr = 0.
+ */
+int np_geomu_camera_batch(const double* d_alpha, const double* d_beta, const double* d_spin,
+                          const double* d_mu0, const double* d_uf, double u0, int64_t n,
+                          double* d_muf, int32_t* d_tpm, int32_t* d_ncase, double* d_iu,
+                          void* stream);
+
+/* ---------------------------------------------------------------------
+ * Schwarzschild equatorial sweeps (schwarzschild_blackhole.cu)
+ * ------------------------------------------------------------------- */
+
+typedef struct GeodesicLookupTable GeodesicLookupTable;   /* bh_types.h */
+
+/*
+ * The reference's FP32 lookup-table kernel, per impact parameter b[i]:
+ * photon from r_start with E = 1, L = b, fixed-step RK4 with
+ * h = 0.01 clamp(r / (Rs/2), 1, 100), at most 10000 steps; outputs the
+ * unwrapped deflection, closest approach (analytic for escaped rays) and
+ * status 0 escaped / 1 captured / 2 photon sphere / 3 inconclusive.
+ * Replaces the precomputeGeodesicsKernel launch of
+ * schwarzschild_blackhole.cu:44-139 / 211-217.  Device arrays of n.
+ */
+int np_schw_precompute(float mass, float r_start, const float* d_b, int64_t n, float* d_defl,
+                       float* d_rmin, int32_t* d_status, void* stream);
+
+/*
+ * Host driver of precomputeGeodesics (schwarzschild_blackhole.cu:142-266):
+ * log-spaced b on [0.3, 20] b_crit (env BH_B_MIN_RS / BH_B_MAX_RS in Rs),
+ * fills *table with new[] arrays (release with np_free_lookup_table or
+ * delete[]).  The C++ entry point GeodesicLookupTable*
+ * precomputeGeodesics(float, int) is declared in bh_schwarzschild.h.
+ */
+int np_precompute_geodesics(float mass, int num_rays, GeodesicLookupTable* table);
+void np_free_lookup_table(GeodesicLookupTable* table);
+
+/*
+ * FP64 plain-method sweep: the Schwarzschild Hamiltonian right-hand side and
+ * RK4 of bh_schwarzschild.h:124-295 in FP64, step law
+ * h = 0.01 clamp(r / M, 1, 100) (schwarzschild_blackhole.cu:84-85), photon
+ * launched from r0 with E = 1, L = b[i], p_r < 0 from H = 0.  A ray is
+ * captured when r < r_capture (or the integrator stops at the horizon),
+ * escaped when r > r0 with p_r > 0, class 2 when |b - 3 sqrt(3) M| <
+ * ps_band, class 3 when max_steps run out.  Outputs per ray: class, total
+ * unwrapped delta-phi, steps (optional); d_total_steps (optional, one u64)
+ * accumulates the step count.
+ */
+int np_schw_sweep(double M, double r0, double r_capture, double ps_band, int64_t max_steps,
+                  const double* d_b, int64_t n, int32_t* d_cls, double* d_dphi, int64_t* d_steps,
+                  uint64_t* d_total_steps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,107 @@ def counters_dict(t) -> dict:
     return {k: int(v) & ((1 << 64) - 1) for k, v in zip(COUNTER_FIELDS, vals)}
 
 
+CARLSON_KINDS = {"rf": 0, "rc": 1, "rd": 2, "rj": 3}
+
+
+def _dptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def carlson_batch(kind: str, x, y, z=None, p=None, out=None, stream=None):
+    """Batched Carlson integral on float64 device tensors (np_carlson_batch)."""
+    import torch
+    out = torch.empty_like(x) if out is None else out
+    code = lib().np_carlson_batch(CARLSON_KINDS[kind], _dptr(x), _dptr(y), _dptr(z), _dptr(p),
+                                  x.numel(), out.data_ptr(), _stream_ptr(stream))
+    check(code, "np_carlson_batch")
+    return out
+
+
+def geomu_camera_batch(alpha, beta, spin, mu0, uf, u0, stream=None):
+    """GEOMU from screen coordinates (np_geomu_camera_batch); float64 device
+    tensors in, (muf, tpm, ncase, iu) device tensors out."""
+    import torch
+    n = alpha.numel()
+    muf = torch.empty(n, dtype=torch.float64, device=alpha.device)
+    iu = torch.empty_like(muf)
+    tpm = torch.empty(n, dtype=torch.int32, device=alpha.device)
+    ncase = torch.empty_like(tpm)
+    code = lib().np_geomu_camera_batch(alpha.data_ptr(), beta.data_ptr(), spin.data_ptr(),
+                                       mu0.data_ptr(), uf.data_ptr(), float(u0), n, muf.data_ptr(),
+                                       tpm.data_ptr(), ncase.data_ptr(), iu.data_ptr(),
+                                       _stream_ptr(stream))
+    check(code, "np_geomu_camera_batch")
+    return muf, tpm, ncase, iu
+
+
+def geomu_batch(u0, uf, mu0, spin, L, Q2, su, sm, tpr=None, stream=None):
+    """GEOMU on constants of motion (np_geomu_batch)."""
+    import torch
+    n = u0.numel()
+    muf = torch.empty(n, dtype=torch.float64, device=u0.device)
+    iu = torch.empty_like(muf)
+    tpm = torch.empty(n, dtype=torch.int32, device=u0.device)
+    ncase = torch.empty_like(tpm)
+    code = lib().np_geomu_batch(u0.data_ptr(), uf.data_ptr(), mu0.data_ptr(), spin.data_ptr(),
+                                L.data_ptr(), Q2.data_ptr(), su.data_ptr(), sm.data_ptr(),
+                                _dptr(tpr), n, muf.data_ptr(), tpm.data_ptr(), ncase.data_ptr(),
+                                iu.data_ptr(), _stream_ptr(stream))
+    check(code, "np_geomu_batch")
+    return muf, tpm, ncase, iu
+
+
+def schw_sweep(b, M=10.0, r0=None, r_capture=None, ps_band=None, max_steps=2_000_000,
+               stream=None, total_steps=None):
+    """FP64 plain-method Schwarzschild sweep (np_schw_sweep) over a float64
+    device tensor of impact parameters; returns (cls, dphi, steps)."""
+    import torch
+    r0 = 1000.0 * M if r0 is None else r0
+    r_capture = 2.01 * M if r_capture is None else r_capture
+    ps_band = 1e-3 * M if ps_band is None else ps_band
+    n = b.numel()
+    cls = torch.empty(n, dtype=torch.int32, device=b.device)
+    dphi = torch.empty(n, dtype=torch.float64, device=b.device)
+    steps = torch.empty(n, dtype=torch.int64, device=b.device)
+    code = lib().np_schw_sweep(M, r0, r_capture, ps_band, max_steps, b.data_ptr(), n,
+                               cls.data_ptr(), dphi.data_ptr(), steps.data_ptr(),
+                               _dptr(total_steps), _stream_ptr(stream))
+    check(code, "np_schw_sweep")
+    return cls, dphi, steps
+
+
+def schw_precompute(b, mass=10.0, r_start=1000.0, stream=None):
+    """The reference's FP32 lookup-table kernel (np_schw_precompute)."""
+    import torch
+    n = b.numel()
+    defl = torch.empty(n, dtype=torch.float32, device=b.device)
+    rmin = torch.empty_like(defl)
+    st = torch.empty(n, dtype=torch.int32, device=b.device)
+    code = lib().np_schw_precompute(mass, r_start, b.data_ptr(), n, defl.data_ptr(),
+                                    rmin.data_ptr(), st.data_ptr(), _stream_ptr(stream))
+    check(code, "np_schw_precompute")
+    return defl, rmin, st
+
+
+def precompute_geodesics(mass=10.0, num_rays=20000):
+    """precomputeGeodesics through its C twin; returns numpy arrays."""
+    import ctypes as C
+
+    import numpy as np
+
+    from ._lib import LookupTable
+    t = LookupTable()
+    check(lib().np_precompute_geodesics(mass, num_rays, C.byref(t)), "np_precompute_geodesics")
+    try:
+        n = t.num_entries
+        out = {k: np.ctypeslib.as_array(getattr(t, k), shape=(n,)).copy() for k in (
+            "impact_parameters", "deflection_angles", "bending_angles", "closest_approaches",
+            "ray_status")}
+    finally:
+        lib().np_free_lookup_table(C.byref(t))
+    return out
+
+
 def render_accumulate(p: RenderParams, accum, spp_done=0, samples_per_pass=1, passes=1,
                       tile_h=128, inv_max=None) -> None:
     """Progressive passes into a host numpy float32 accumulation buffer."""

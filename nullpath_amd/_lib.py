@@ -79,6 +79,33 @@ def _declare(L: C.CDLL) -> None:
     L.np_render_tile.restype = i32
     L.np_render_accumulate.argtypes = [C.POINTER(RenderParams), i32, i32, i32, i32, vp, vp]
     L.np_render_accumulate.restype = i32
+    i64, d = C.c_int64, C.c_double
+    L.np_carlson_batch.argtypes = [i32, vp, vp, vp, vp, i64, vp, vp]
+    L.np_carlson_batch.restype = i32
+    L.np_geomu_batch.argtypes = [vp] * 9 + [i64] + [vp] * 5
+    L.np_geomu_batch.restype = i32
+    L.np_geomu_camera_batch.argtypes = [vp] * 5 + [d, i64] + [vp] * 5
+    L.np_geomu_camera_batch.restype = i32
+    f = C.c_float
+    L.np_schw_precompute.argtypes = [f, f, vp, i64, vp, vp, vp, vp]
+    L.np_schw_precompute.restype = i32
+    L.np_schw_sweep.argtypes = [d, d, d, d, i64, vp, i64, vp, vp, vp, vp, vp]
+    L.np_schw_sweep.restype = i32
+    L.np_precompute_geodesics.argtypes = [f, i32, C.POINTER(LookupTable)]
+    L.np_precompute_geodesics.restype = i32
+    L.np_free_lookup_table.argtypes = [C.POINTER(LookupTable)]
+    L.np_free_lookup_table.restype = None
+
+
+class LookupTable(C.Structure):
+    """GeodesicLookupTable (include/bh_types.h)."""
+
+    _fields_ = [("impact_parameters", C.POINTER(C.c_float)),
+                ("deflection_angles", C.POINTER(C.c_float)),
+                ("bending_angles", C.POINTER(C.c_float)),
+                ("closest_approaches", C.POINTER(C.c_float)),
+                ("ray_status", C.POINTER(C.c_int)),
+                ("num_entries", C.c_int), ("mass", C.c_float)]
 
 
 def check(code: int, what: str) -> None:

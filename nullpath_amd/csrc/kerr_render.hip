@@ -1,4 +1,6 @@
-// kerr_render.hip -- gfx950 Kerr null-geodesic renderer, FP64 path.
+// kerr_render.hip -- gfx950 Kerr null-geodesic renderer: the FP64 path
+// (reference formulas) and the FP32-compensated path, behind one persistent
+// work-queue kernel.
 //
 // One ray per lane, persistent grid, wave64 work queue:
 //   * work item = one pixel; the lane traces that pixel's spp samples one
@@ -151,16 +153,76 @@ __device__ __forceinline__ Y5 rk4_from_k1(const FrameConst& F, const Y5& y, cons
     return o;
 }
 
+// Step doubling of rk4_adaptive_step_d (bh_kerr.h:1058-1070): one RK4
+// step of size h and two of size h/2 from y, given k1 = f(y).  The ten
+// remaining right-hand sides run in a uniform, non-unrolled stage loop so
+// the kernel holds one copy of the RHS code and one stage's temporaries;
+// every product, compensated sum and update is the one rk4_step_d makes.
+__device__ __forceinline__ void step_doubling_f64(const FrameConst& F, const Y5& y, const Deriv& k1,
+                                                  const double pt, const double pph, const double h,
+                                                  Y5& full, Y5& half1, Y5& half2) {
+    const double h2 = 0.5 * h;
+    Klein<double> sr, sth, sph, spr, spth;
+    sr.add(k1.r); sth.add(k1.th); sph.add(k1.ph); spr.add(k1.pr); spth.add(k1.pth);
+    Deriv kp = k1;
+    Y5 base = y;
+#pragma clang loop unroll(disable)
+    for (int j = 0; j < 10; ++j) {
+        // stage coefficients: full step (j 0-2), first half step (3-5),
+        // second half step (6-9; stage 6 is its k1)
+        const double c = (j <= 1) ? 0.5 * h : (j == 2) ? h : (j == 5 || j == 9) ? h2 : 0.5 * h2;
+        Y5 in = base;
+        if (j != 6) {
+            in.r = base.r + c * kp.r;
+            in.th = base.th + c * kp.th;
+            in.pr = base.pr + c * kp.pr;
+            in.pth = base.pth + c * kp.pth;
+        }
+        const Deriv k = kerr_rhs(F.M, F.a, F.r_plus, in.r, in.th, pt, in.pr, in.pth, pph);
+        if (j == 6) {
+            sr = Klein<double>(); sth = Klein<double>(); sph = Klein<double>();
+            spr = Klein<double>(); spth = Klein<double>();
+            sr.add(k.r); sth.add(k.th); sph.add(k.ph); spr.add(k.pr); spth.add(k.pth);
+        } else {
+            const double w = (j == 2 || j == 5 || j == 9) ? 1.0 : 2.0;
+            sr.add(w * k.r); sth.add(w * k.th); sph.add(w * k.ph); spr.add(w * k.pr); spth.add(w * k.pth);
+        }
+        kp = k;
+        if (j == 2 || j == 5 || j == 9) {
+            const double h6 = (j == 2 ? h : h2) / 6.0;
+            const Y5& b = (j == 9) ? half1 : y;
+            Y5 o;
+            o.r = b.r + h6 * sr.result();
+            o.th = b.th + h6 * sth.result();
+            o.ph = b.ph + h6 * sph.result();
+            o.pr = b.pr + h6 * spr.result();
+            o.pth = b.pth + h6 * spth.result();
+            if (j == 2) {
+                full = o;
+                sr = Klein<double>(); sth = Klein<double>(); sph = Klein<double>();
+                spr = Klein<double>(); spth = Klein<double>();
+                sr.add(k1.r); sth.add(k1.th); sph.add(k1.ph); spr.add(k1.pr); spth.add(k1.pth);
+                kp = k1;
+            } else if (j == 5) {
+                half1 = o;
+                base = o;
+            } else {
+                half2 = o;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ double err1(double x, double y, double scale) {
     return fabs(x - y) / fmax(scale, 1e-30);
 }
 
-// doppler_g_general_d (bh_kerr.h:833-862) at the disk plane.
+// doppler_g_general_d (bh_kerr.h:833-862) at the disk plane.  sin and
+// cos of pi/2 come from the host (F.s_eq, F.c_eq); (r/M)^1.5 is x*sqrt(x).
 __device__ __forceinline__ double doppler_g(const FrameConst& F, double r, double pt, double pph) {
     const double M = F.M, a = F.a;
     // kerr_blocks_cov_d at theta = pi/2 (bh_kerr.h:575-592)
-    double s, c;
-    sincos(NP_PI_D * 0.5, &s, &c);
+    const double s = F.s_eq, c = F.c_eq;
     const double st2 = s * s, r2 = r * r, a2 = a * a;
     const double Sigma = r2 + a2 * c * c;
     const double Delta = r2 - 2.0 * M * r + a2;
@@ -169,8 +231,8 @@ __device__ __forceinline__ double doppler_g(const FrameConst& F, double r, doubl
     const double g_tphi = -2.0 * M * a * r * st2 / fmax(1e-12, Sigma);
     const double g_phiphi = Aq * st2 / fmax(1e-12, Sigma);
     const double M_safe = fmax(1e-12, M);
-    const double r_over_M = r / M_safe;
-    const double Omega = 1.0 / (a / M_safe + pow(fmax(1.0, r_over_M), 1.5)) / M_safe;
+    const double x = fmax(1.0, r / M_safe);
+    const double Omega = 1.0 / (a / M_safe + x * sqrt(x)) / M_safe;
     Klein<double> ds;
     ds.add(g_tt);
     ds.add(2.0 * g_tphi * Omega);
@@ -269,43 +331,76 @@ __device__ void invariants(const FrameConst& F, double r, double th, double pt, 
     inv[2] = fmaxf(inv[2], (float)(fabs(pph - pph0) / fmax(1e-12, fabs(pph0))));
 }
 
-struct Ray {
-    Y5 y;
-    Deriv k1;
-    double pt, pph, pt0, pph0;
-    double prev_r, prev_th, min_r, h_adapt, h;
+// ---------------------------------------------------------------------------
+// Per-lane ray bookkeeping shared by both steppers (render.cu:171-183).
+// ---------------------------------------------------------------------------
+struct RayBase {
+    double pt, pph, pt0, pph0;       // covariant p_t, p_phi (exactly conserved)
+    double prev_r, prev_th, min_r;   // previous accepted state, closest approach
     float disk[3];
     float inv[3];
     float cam_x, cam_y;
     int step, attempt, orders, attempts_total;
-    bool k1_valid;
 };
 
-__device__ void start_ray(const FrameConst& F, int x, int yy, uint32_t sid, Ray& R) {
-    float n_r, n_th, n_ph;
-    camera_dir(F, x, yy, sid, n_r, n_th, n_ph, R.cam_x, R.cam_y);
-    R.y.r = F.r_cam;
-    R.y.th = F.th_cam;
-    R.y.ph = F.ph_cam;
-    init_momentum(F, n_r, n_th, n_ph, R.pt, R.y.pr, R.y.pth, R.pph);
+__device__ __forceinline__ void base_start(const FrameConst& F, RayBase& R, float cam_x, float cam_y) {
     R.pt0 = R.pt;
     R.pph0 = R.pph;
-    R.prev_r = R.y.r;
-    R.prev_th = R.y.th;
-    R.min_r = R.y.r;
-    R.h_adapt = 0.0;
-    R.h = 0.0;
+    R.prev_r = F.r_cam;
+    R.prev_th = F.th_cam;
+    R.min_r = F.r_cam;
     R.disk[0] = R.disk[1] = R.disk[2] = 0.0f;
     R.inv[0] = R.inv[1] = R.inv[2] = 0.0f;
+    R.cam_x = cam_x;
+    R.cam_y = cam_y;
     R.step = 0;
     R.attempt = 0;
     R.orders = 0;
     R.attempts_total = 0;
-    R.k1_valid = false;
+}
+
+// Accepted-step bookkeeping of render.cu:210-250: closest approach,
+// equatorial crossing with linear interpolation of r, disk shading,
+// escape and step budget.  Returns -1 while the ray keeps going.
+template <bool INV>
+__device__ __forceinline__ int after_accept(const FrameConst& F, RayBase& R, double r, double th,
+                                            double pr, double pth) {
+    R.min_r = fmin(R.min_r, r);
+    const double d_prev = R.prev_th - NP_PI_D * 0.5;
+    const double d_curr = th - NP_PI_D * 0.5;
+    const double eps = 1e-8;
+    const bool sign_change = (d_prev > 0 && d_curr < 0) || (d_prev < 0 && d_curr > 0);
+    const bool near_zero = (fabs(d_prev) < eps && fabs(d_curr) >= eps) ||
+                           (fabs(d_curr) < eps && fabs(d_prev) >= eps);
+    if ((sign_change || near_zero) && R.orders < F.disk_max_orders) {
+        const double denom = fabs(d_prev) + fabs(d_curr) + 1e-30;
+        const double alpha = fabs(d_prev) / denom;
+        const double r_cross = R.prev_r + alpha * (r - R.prev_r);
+        if (r_cross >= F.rin && r_cross <= F.rout) {
+            // p_t and p_phi are constant, so their interpolants equal them
+            const float g = (float)doppler_g(F, r_cross, R.pt, R.pph);
+            const float rr = F.emiss_m_units ? (float)(r_cross / F.M) : (float)r_cross;
+            const float j = powf(fmaxf(rr, 1.0f), -F.emiss_q);
+            const float I = j * g * g * g;
+            R.disk[0] += (1.0f * I) * F.disk_gain;
+            R.disk[1] += (0.75f * I) * F.disk_gain;
+            R.disk[2] += (0.45f * I) * F.disk_gain;
+            R.orders += 1;
+            if (INV)
+                invariants(F, r_cross, NP_PI_D * 0.5, R.pt, pr, pth, R.pph, R.pt0, R.pph0, R.inv);
+        }
+    }
+    R.prev_r = r;
+    R.prev_th = th;
+    const int step = R.step;
+    R.step += 1;
+    if (r > F.r_far && step > 10) return NP_RAY_ESCAPED;
+    if (R.step >= F.max_steps) return NP_RAY_STEPCAP;
+    return -1;
 }
 
 // Colour of a finished sample (render.cu:312-359).
-__device__ void shade_final(const FrameConst& F, const Ray& R, int cls, float col[3]) {
+__device__ void shade_final(const FrameConst& F, const RayBase& R, int cls, float col[3]) {
     const float Rs = F.Rs;
     if (cls == NP_RAY_CAPTURED) {
         if (F.bg_mode == NP_BG_POINT) {
@@ -346,21 +441,55 @@ __device__ void shade_final(const FrameConst& F, const Ray& R, int cls, float co
     }
 }
 
-// One flattened integrator iteration for one lane.  Returns -1 while the
-// ray is still running, else its np_ray_class.
-template <bool ADAPTIVE>
-__device__ __forceinline__ int advance(const FrameConst& F, Ray& R, int& rhs_count, int& attempts) {
-    const double Rs = F.Rs_d;
-    bool ok = false;
-    attempts += 1;
-    R.attempts_total += 1;
-    if (ADAPTIVE) {
+// ---------------------------------------------------------------------------
+// FP64 stepper: the reference formulas (bh_kerr.h) in double precision.
+// ---------------------------------------------------------------------------
+struct StepF64 {
+    struct Ray : RayBase {
+        Y5 y;
+        Deriv k1;
+        double h_adapt, h;
+        bool k1_valid;
+    };
+
+    __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
+                                 float cam_x, float cam_y) {
+        init_momentum(F, n_r, n_th, n_ph, R.pt, R.y.pr, R.y.pth, R.pph);
+        R.y.r = F.r_cam;
+        R.y.th = F.th_cam;
+        R.y.ph = F.ph_cam;
+        R.h_adapt = 0.0;
+        R.h = 0.0;
+        R.k1_valid = false;
+        base_start(F, R, cam_x, cam_y);
+    }
+
+    __device__ static double r(const Ray& R) { return R.y.r; }
+    __device__ static double th(const Ray& R) { return R.y.th; }
+    __device__ static double pr(const Ray& R) { return R.y.pr; }
+    __device__ static double pth(const Ray& R) { return R.y.pth; }
+
+    // One integrator attempt.  Returns 1 accepted, 0 rejected (retry),
+    // 2 failed (the ray is captured: horizon, pole or rejection budget).
+    template <bool ADAPTIVE>
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count) {
+        const double Rs = F.Rs_d;
+        if (!ADAPTIVE) {
+            // integrateGeodesicKerrD (bh_kerr.h:734-774), h = 0.01 * scale
+            const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
+            const double h = (double)STEP0 * scale;
+            const Deriv k1 = kerr_rhs(F.M, F.a, F.r_plus, R.y.r, R.y.th, R.pt, R.y.pr, R.y.pth, R.pph);
+            rhs_count += 1;
+            R.y = rk4_from_k1(F, R.y, k1, R.pt, R.pph, h, rhs_count);
+            const bool ok = R.y.r > F.r_plus + 1e-9 && R.y.th > 1e-9 && R.y.th < NP_PI_D - 1e-9;
+            return ok ? 1 : 2;
+        }
         if (R.attempt == 0) {
             // step-size choice at the start of a step (render.cu:185-191)
             const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
             double h_base = (double)STEP0 * scale;
             h_base = fmin(fmax(h_base, HMIN), HMAX);
-            double h = (R.h_adapt > 0.0) ? R.h_adapt : h_base;
+            const double h = (R.h_adapt > 0.0) ? R.h_adapt : h_base;
             R.h = fmin(fmax(h, HMIN), HMAX);
         }
         const double h = R.h;
@@ -370,12 +499,9 @@ __device__ __forceinline__ int advance(const FrameConst& F, Ray& R, int& rhs_cou
             R.k1_valid = true;
         }
         // rk4_adaptive_step_d (bh_kerr.h:1058-1105)
-        const Y5 full = rk4_from_k1(F, R.y, R.k1, R.pt, R.pph, h, rhs_count);
-        const double h2 = 0.5 * h;
-        const Y5 half1 = rk4_from_k1(F, R.y, R.k1, R.pt, R.pph, h2, rhs_count);
-        const Deriv k1b = kerr_rhs(F.M, F.a, F.r_plus, half1.r, half1.th, R.pt, half1.pr, half1.pth, R.pph);
-        rhs_count += 1;
-        const Y5 half2 = rk4_from_k1(F, half1, k1b, R.pt, R.pph, h2, rhs_count);
+        Y5 full, half1, half2;
+        step_doubling_f64(F, R.y, R.k1, R.pt, R.pph, h, full, half1, half2);
+        rhs_count += 10;
         const double lim = NP_PI_D - 1e-9;
         const bool ok1 = full.r > F.r_plus + 1e-12 && full.th > 1e-9 && full.th < lim;
         const bool ok2 = half1.r > F.r_plus + 1e-12 && half1.th > 1e-9 && half1.th < lim;
@@ -402,80 +528,296 @@ __device__ __forceinline__ int advance(const FrameConst& F, Ray& R, int& rhs_cou
             }
         }
         if (fac > 0.0) {
-            ok = true;
             R.h_adapt = fmin(fmax(h_try * fac, HMIN), HMAX);
             R.attempt = 0;
-        } else {
-            // retry loop of render.cu:193-205
-            R.h = fmax(h_try, HMIN);
-            R.attempt += 1;
-            if (R.h <= HMIN * 1.01 || R.attempt >= 8) return NP_RAY_CAPTURED;
-            return -1;
+            return 1;
         }
-    } else {
-        // integrateGeodesicKerrD (bh_kerr.h:734-774), h = 0.01 * scale
-        const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
-        const double h = (double)STEP0 * scale;
-        const Deriv k1 = kerr_rhs(F.M, F.a, F.r_plus, R.y.r, R.y.th, R.pt, R.y.pr, R.y.pth, R.pph);
-        rhs_count += 1;
-        R.y = rk4_from_k1(F, R.y, k1, R.pt, R.pph, h, rhs_count);
-        ok = R.y.r > F.r_plus + 1e-9 && R.y.th > 1e-9 && R.y.th < NP_PI_D - 1e-9;
+        // retry loop of render.cu:193-205
+        R.h = fmax(h_try, HMIN);
+        R.attempt += 1;
+        if (R.h <= HMIN * 1.01 || R.attempt >= 8) return 2;
+        return 0;
     }
-    if (!ok) return NP_RAY_CAPTURED;
+};
 
-    // accepted step: bookkeeping of render.cu:210-250
-    R.min_r = fmin(R.min_r, R.y.r);
-    const double d_prev = R.prev_th - NP_PI_D * 0.5;
-    const double d_curr = R.y.th - NP_PI_D * 0.5;
-    const double eps = 1e-8;
-    const bool sign_change = (d_prev > 0 && d_curr < 0) || (d_prev < 0 && d_curr > 0);
-    const bool near_zero = (fabs(d_prev) < eps && fabs(d_curr) >= eps) ||
-                           (fabs(d_curr) < eps && fabs(d_prev) >= eps);
-    if ((sign_change || near_zero) && R.orders < F.disk_max_orders) {
-        const double denom = fabs(d_prev) + fabs(d_curr) + 1e-30;
-        const double alpha = fabs(d_prev) / denom;
-        const double r_cross = R.prev_r + alpha * (R.y.r - R.prev_r);
-        if (r_cross >= F.rin && r_cross <= F.rout) {
-            // p_t and p_phi are constant, so their interpolants equal them
-            const float g = (float)doppler_g(F, r_cross, R.pt, R.pph);
-            const float rr = F.emiss_m_units ? (float)(r_cross / F.M) : (float)r_cross;
-            const float j = powf(fmaxf(rr, 1.0f), -F.emiss_q);
-            const float I = j * g * g * g;
-            R.disk[0] += (1.0f * I) * F.disk_gain;
-            R.disk[1] += (0.75f * I) * F.disk_gain;
-            R.disk[2] += (0.45f * I) * F.disk_gain;
-            R.orders += 1;
-            if (F.check_inv)
-                invariants(F, r_cross, NP_PI_D * 0.5, R.pt, R.y.pr, R.y.pth, R.pph, R.pt0, R.pph0, R.inv);
-        }
-    }
-    R.prev_r = R.y.r;
-    R.prev_th = R.y.th;
-    const int step = R.step;
-    R.step += 1;
-    if (R.y.r > F.r_far && step > 10) return NP_RAY_ESCAPED;
-    if (R.step >= F.max_steps) return NP_RAY_STEPCAP;
-    return -1;
+// ---------------------------------------------------------------------------
+// FP32-compensated stepper.
+//
+// * State (r, theta, phi, p_r, p_theta) is a float-float pair per
+//   component; each RK4 update is added with an error-free two-sum, so the
+//   long far-field legs (thousands of steps of a few 0.1 M) lose nothing to
+//   rounding of the state itself.
+// * The right-hand side is evaluated in FP32 from the separable (Carter)
+//   form of the Kerr Hamiltonian, 2 Sigma H = Delta p_r^2 - P^2/Delta + p_th^2
+//   + (L/sin - a E sin)^2 with P = (r^2+a^2)E - aL, and the radial force
+//   taken on the constraint surface H = 0, which removes the r/M-fold
+//   cancellation of the quotient-rule derivatives:
+//     dr/dl   = Delta p_r / Sigma
+//     dth/dl  = p_th / Sigma
+//     dph/dl  = (a (2MrE - aL)/Delta + L/sin^2) / Sigma
+//     dpr/dl  = ((r-M) Q/Delta - P (2EM(r^2-a^2) - 2aL(r-M))/Delta^2) / Sigma
+//     dpth/dl = cos sin w (L/sin^2 + aE) / Sigma,  w = L/sin^2 - aE,
+//   with Q = p_th^2 + sin^2 w^2 and Delta = (r - r+)(r - r-).
+// * Step control is the reference's step doubling (same tolerances, safety
+//   factor, growth caps and retry budget) on the compensated state.
+// ---------------------------------------------------------------------------
+struct ff {
+    float hi, lo;
+};
+
+__device__ __forceinline__ ff ff_from_d(double d) {
+    ff x;
+    x.hi = (float)d;
+    x.lo = (float)(d - (double)x.hi);
+    return x;
+}
+__device__ __forceinline__ double ff_to_d(ff x) { return (double)x.hi + (double)x.lo; }
+__device__ __forceinline__ float ff_f(ff x) { return x.hi + x.lo; }
+
+// x + y with an error-free transformation (Knuth two-sum) and a final
+// renormalisation (Dekker fast two-sum).
+__device__ __forceinline__ ff ff_add(ff x, float y) {
+#pragma clang fp contract(off)
+    const float s = x.hi + y;
+    const float bb = s - x.hi;
+    const float err = (x.hi - (s - bb)) + (y - bb);
+    const float lo = err + x.lo;
+    ff o;
+    o.hi = s + lo;
+    o.lo = lo - (o.hi - s);
+    return o;
 }
 
-template <bool ADAPTIVE>
+// (x - y) of two float-float numbers, rounded to float.
+__device__ __forceinline__ float ff_diff(ff x, ff y) {
+#pragma clang fp contract(off)
+    return (x.hi - y.hi) + (x.lo - y.lo);
+}
+
+struct DerivF {
+    float r, th, ph, pr, pth;
+};
+
+struct FF5 {
+    ff r, th, ph, pr, pth;
+};
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, float th, float pr,
+                                               float pth, float E, float L) {
+    DerivF d;
+    if (r <= F.rpf_guard) {
+        d.r = d.th = d.ph = d.pr = d.pth = 0.0f;
+        return d;
+    }
+    float s, c;
+    sincosf(th, &s, &c);
+    const float M = F.Mf, a = F.af, a2 = F.a2f;
+    const float r2 = r * r;
+    const float s2 = fmaxf(s * s, 1e-12f);
+    const float Sigma = r2 + a2 * (c * c);
+    const float Delta = (r - F.rpf) * (r - F.rmf);
+    const float iS = rcpf(Sigma);
+    const float iD = rcpf(Delta);
+    const float is2 = rcpf(s2);
+    const float P = (r2 + a2) * E - a * L;
+    const float w = L * is2 - a * E;
+    const float Q = pth * pth + s2 * (w * w);
+    const float X = 2.0f * E * M * (r2 - a2) - 2.0f * a * L * (r - M);
+    d.r = Delta * pr * iS;
+    d.th = pth * iS;
+    d.ph = (a * (2.0f * M * r * E - a * L) * iD + L * is2) * iS;
+    d.pr = ((r - M) * Q * iD - P * X * (iD * iD)) * iS;
+    d.pth = c * s * w * (L * is2 + a * E) * iS;
+    return d;
+}
+
+__device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const DerivF& k1, float E,
+                                       float L, float h, int& rhs_count) {
+    const float hh = 0.5f * h;
+    const float yr = ff_f(y.r), yth = ff_f(y.th), ypr = ff_f(y.pr), ypth = ff_f(y.pth);
+    const DerivF k2 = kerr_rhs_f32(F, yr + hh * k1.r, yth + hh * k1.th, ypr + hh * k1.pr,
+                                   ypth + hh * k1.pth, E, L);
+    const DerivF k3 = kerr_rhs_f32(F, yr + hh * k2.r, yth + hh * k2.th, ypr + hh * k2.pr,
+                                   ypth + hh * k2.pth, E, L);
+    const DerivF k4 = kerr_rhs_f32(F, yr + h * k3.r, yth + h * k3.th, ypr + h * k3.pr,
+                                   ypth + h * k3.pth, E, L);
+    rhs_count += 3;
+    const float h6 = h * (1.0f / 6.0f);
+    FF5 o;
+    o.r = ff_add(y.r, h6 * (k1.r + 2.0f * (k2.r + k3.r) + k4.r));
+    o.th = ff_add(y.th, h6 * (k1.th + 2.0f * (k2.th + k3.th) + k4.th));
+    o.ph = ff_add(y.ph, h6 * (k1.ph + 2.0f * (k2.ph + k3.ph) + k4.ph));
+    o.pr = ff_add(y.pr, h6 * (k1.pr + 2.0f * (k2.pr + k3.pr) + k4.pr));
+    o.pth = ff_add(y.pth, h6 * (k1.pth + 2.0f * (k2.pth + k3.pth) + k4.pth));
+    return o;
+}
+
+// FP32 counterpart of step_doubling_f64: same stage sequence, float RHS,
+// float stage sums, float-float updates of the state.
+__device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5& y, const DerivF& k1,
+                                                  const float E, const float L, const float h,
+                                                  FF5& full, FF5& half1, FF5& half2) {
+    const float h2 = 0.5f * h;
+    DerivF S = k1, kp = k1;
+    float br = ff_f(y.r), bth = ff_f(y.th), bpr = ff_f(y.pr), bpth = ff_f(y.pth);
+#pragma clang loop unroll(disable)
+    for (int j = 0; j < 10; ++j) {
+        const float c = (j <= 1) ? 0.5f * h : (j == 2) ? h : (j == 5 || j == 9) ? h2 : 0.5f * h2;
+        const float cc = (j == 6) ? 0.0f : c;
+        const DerivF k = kerr_rhs_f32(F, br + cc * kp.r, bth + cc * kp.th, bpr + cc * kp.pr,
+                                      bpth + cc * kp.pth, E, L);
+        if (j == 6) {
+            S = k;
+        } else {
+            const float w = (j == 2 || j == 5 || j == 9) ? 1.0f : 2.0f;
+            S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
+        }
+        kp = k;
+        if (j == 2 || j == 5 || j == 9) {
+            const float h6 = (j == 2 ? h : h2) * (1.0f / 6.0f);
+            const FF5& b = (j == 9) ? half1 : y;
+            FF5 o;
+            o.r = ff_add(b.r, h6 * S.r);
+            o.th = ff_add(b.th, h6 * S.th);
+            o.ph = ff_add(b.ph, h6 * S.ph);
+            o.pr = ff_add(b.pr, h6 * S.pr);
+            o.pth = ff_add(b.pth, h6 * S.pth);
+            if (j == 2) {
+                full = o;
+                S = k1;
+                kp = k1;
+            } else if (j == 5) {
+                half1 = o;
+                br = ff_f(o.r); bth = ff_f(o.th); bpr = ff_f(o.pr); bpth = ff_f(o.pth);
+            } else {
+                half2 = o;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float errf(ff x, ff y, float rt, float at) {
+    const float sc = at + rt * fmaxf(fabsf(x.hi), fabsf(y.hi));
+    return fabsf(ff_diff(x, y)) / fmaxf(sc, 1e-30f);
+}
+
+__device__ __forceinline__ bool okf(const FrameConst& F, const FF5& y) {
+    const double r = ff_to_d(y.r), th = ff_to_d(y.th);
+    return r > F.r_plus + 1e-12 && th > 1e-9 && th < NP_PI_D - 1e-9;
+}
+
+struct StepF32C {
+    struct Ray : RayBase {
+        FF5 y;
+        DerivF k1;
+        float E, L, h_adapt, h;
+        bool k1_valid;
+    };
+
+    __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
+                                 float cam_x, float cam_y) {
+        double pr, pth;
+        init_momentum(F, n_r, n_th, n_ph, R.pt, pr, pth, R.pph);
+        R.y.r = ff_from_d(F.r_cam);
+        R.y.th = ff_from_d(F.th_cam);
+        R.y.ph = ff_from_d(F.ph_cam);
+        R.y.pr = ff_from_d(pr);
+        R.y.pth = ff_from_d(pth);
+        R.E = (float)(-R.pt);
+        R.L = (float)R.pph;
+        R.h_adapt = 0.0f;
+        R.h = 0.0f;
+        R.k1_valid = false;
+        base_start(F, R, cam_x, cam_y);
+    }
+
+    __device__ static double r(const Ray& R) { return ff_to_d(R.y.r); }
+    __device__ static double th(const Ray& R) { return ff_to_d(R.y.th); }
+    __device__ static double pr(const Ray& R) { return ff_to_d(R.y.pr); }
+    __device__ static double pth(const Ray& R) { return ff_to_d(R.y.pth); }
+
+    template <bool ADAPTIVE>
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count) {
+        const float hmin = 1e-6f, hmax = 0.2f;
+        if (!ADAPTIVE) {
+            const float scale = fminf(fmaxf(1.0f, ff_f(R.y.r) / (0.5f * F.Rs)), 100.0f);
+            const float h = STEP0 * scale;
+            const DerivF k1 = kerr_rhs_f32(F, ff_f(R.y.r), ff_f(R.y.th), ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
+            rhs_count += 1;
+            R.y = rk4_f32(F, R.y, k1, R.E, R.L, h, rhs_count);
+            const double rr = ff_to_d(R.y.r), tt = ff_to_d(R.y.th);
+            const bool ok = rr > F.r_plus + 1e-9 && tt > 1e-9 && tt < NP_PI_D - 1e-9;
+            return ok ? 1 : 2;
+        }
+        if (R.attempt == 0) {
+            const float scale = fminf(fmaxf(1.0f, ff_f(R.y.r) / (0.5f * F.Rs)), 100.0f);
+            float h_base = fminf(fmaxf(STEP0 * scale, hmin), hmax);
+            const float h = (R.h_adapt > 0.0f) ? R.h_adapt : h_base;
+            R.h = fminf(fmaxf(h, hmin), hmax);
+        }
+        const float h = R.h;
+        if (!R.k1_valid) {
+            R.k1 = kerr_rhs_f32(F, ff_f(R.y.r), ff_f(R.y.th), ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
+            rhs_count += 1;
+            R.k1_valid = true;
+        }
+        FF5 full, half1, half2;
+        step_doubling_f32(F, R.y, R.k1, R.E, R.L, h, full, half1, half2);
+        rhs_count += 10;
+        float h_try = h;
+        float fac = 0.0f;
+        if (!(okf(F, full) && okf(F, half1) && okf(F, half2))) {
+            h_try *= 0.5f;
+        } else {
+            const float rt = F.rtolf, at = F.atolf;
+            const float e = fmaxf(fmaxf(errf(full.r, half2.r, rt, at), errf(full.th, half2.th, rt, at)),
+                                  fmaxf(errf(full.ph, half2.ph, rt, at),
+                                        fmaxf(errf(full.pr, half2.pr, rt, at), errf(full.pth, half2.pth, rt, at))));
+            const float le = __log2f(fmaxf(1e-12f, e));
+            if (e <= 1.0f) {
+                fac = fminf(2.0f, 0.9f * exp2f(-0.2f * le));
+                R.y = half2;
+                R.k1_valid = false;
+            } else {
+                h_try *= fmaxf(0.1f, 0.9f * exp2f(-0.25f * le));
+            }
+        }
+        if (fac > 0.0f) {
+            R.h_adapt = fminf(fmaxf(h_try * fac, hmin), hmax);
+            R.attempt = 0;
+            return 1;
+        }
+        R.h = fmaxf(h_try, hmin);
+        R.attempt += 1;
+        if (R.h <= hmin * 1.01f || R.attempt >= 8) return 2;
+        return 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Persistent renderer: work item = pixel, lane-level refill, one integrator
+// attempt per loop iteration.
+// ---------------------------------------------------------------------------
+template <class S, bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256)
-kerr_render_fp64(const FrameConst F, const int y0, const int rows, const int spp,
-                 const int sample_offset, float* __restrict__ out, const int accumulate,
-                 float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
-                 np_ray_record* __restrict__ records, unsigned long long* __restrict__ work) {
+kerr_render(const FrameConst F, const int y0, const int rows, const int spp,
+            const int sample_offset, float* __restrict__ out, const int accumulate,
+            float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
+            np_ray_record* __restrict__ records, unsigned long long* __restrict__ work) {
     const int W = F.width;
     const int y_end = min(y0 + rows, F.height);
     const uint64_t npix = (y_end > y0) ? (uint64_t)(y_end - y0) * (uint64_t)W : 0ull;
 
-    Ray R;
+    typename S::Ray R;
     bool has = false, done = false;
     uint64_t pix = 0;
     int sample = 0;
     float acc[3] = {0.0f, 0.0f, 0.0f};
     float inv_lane[3] = {0.0f, 0.0f, 0.0f};
-    int rhs_count = 0, attempts = 0;
-    uint64_t c_rhs = 0, c_att = 0, c_steps = 0, c_rays = 0, c_cap = 0, c_esc = 0, c_cap_steps = 0,
+    // per-lane counters, flushed to the u64 totals before they can wrap
+    uint32_t c_rhs = 0, c_att = 0, c_steps = 0, c_rays = 0, c_cap = 0, c_esc = 0, c_cap_steps = 0,
              c_disk = 0;
     if (spp <= 0 || F.max_steps <= 0) done = true;
 
@@ -490,7 +832,9 @@ kerr_render_fp64(const FrameConst F, const int y0, const int rows, const int spp
                     acc[0] = acc[1] = acc[2] = 0.0f;
                     const int x = (int)(pix % (uint64_t)W);
                     const int yy = y0 + (int)(pix / (uint64_t)W);
-                    start_ray(F, x, yy, (uint32_t)(sample_offset + sample), R);
+                    float n_r, n_th, n_ph, cx, cy;
+                    camera_dir(F, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
+                    S::start(F, R, n_r, n_th, n_ph, cx, cy);
                     has = true;
                 } else {
                     done = true;
@@ -498,17 +842,26 @@ kerr_render_fp64(const FrameConst F, const int y0, const int rows, const int spp
             }
         }
         if (__ballot(has) == 0) break;
+        if (counters && __ballot(c_rhs > 0x40000000u || c_steps > 0x40000000u)) {
+            wave_add_u64(&counters->steps, c_steps);
+            wave_add_u64(&counters->attempts, c_att);
+            wave_add_u64(&counters->rhs_evals, c_rhs);
+            c_steps = c_att = c_rhs = 0;
+        }
         if (has) {
-            rhs_count = 0;
-            attempts = 0;
-            const int cls = advance<ADAPTIVE>(F, R, rhs_count, attempts);
-            c_rhs += (uint64_t)rhs_count;
-            c_att += (uint64_t)attempts;
+            int rhs_count = 0;
+            R.attempts_total += 1;
+            c_att += 1;
+            const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count);
+            c_rhs += (uint32_t)rhs_count;
+            int cls = -1;
+            if (st == 2) cls = NP_RAY_CAPTURED;
+            else if (st == 1) cls = after_accept<INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
             if (cls >= 0) {
                 float col[3];
                 shade_final(F, R, cls, col);
-                if (F.check_inv && cls != NP_RAY_CAPTURED && !(R.orders > 0 && F.disk_on))
-                    invariants(F, R.y.r, R.y.th, R.pt, R.y.pr, R.y.pth, R.pph, R.pt0, R.pph0, R.inv);
+                if (INV && cls != NP_RAY_CAPTURED && !(R.orders > 0 && F.disk_on))
+                    invariants(F, S::r(R), S::th(R), R.pt, S::pr(R), S::pth(R), R.pph, R.pt0, R.pph0, R.inv);
                 acc[0] += col[0];
                 acc[1] += col[1];
                 acc[2] += col[2];
@@ -516,7 +869,7 @@ kerr_render_fp64(const FrameConst F, const int y0, const int rows, const int spp
                 inv_lane[1] = fmaxf(inv_lane[1], R.inv[1]);
                 inv_lane[2] = fmaxf(inv_lane[2], R.inv[2]);
                 c_rays += 1;
-                c_steps += (uint64_t)R.step;
+                c_steps += (uint32_t)R.step;
                 c_cap += (cls == NP_RAY_CAPTURED);
                 c_esc += (cls == NP_RAY_ESCAPED);
                 c_cap_steps += (cls == NP_RAY_STEPCAP);
@@ -528,14 +881,16 @@ kerr_render_fp64(const FrameConst F, const int y0, const int rows, const int spp
                     rec.steps = R.step;
                     rec.attempts = R.attempts_total;
                     rec.min_r = R.min_r;
-                    rec.r_end = R.y.r;
+                    rec.r_end = S::r(R);
                     records[pix * (uint64_t)spp + (uint64_t)sample] = rec;
                 }
                 sample += 1;
                 const int x = (int)(pix % (uint64_t)W);
                 const int yy = y0 + (int)(pix / (uint64_t)W);
                 if (sample < spp) {
-                    start_ray(F, x, yy, (uint32_t)(sample_offset + sample), R);
+                    float n_r, n_th, n_ph, cx, cy;
+                    camera_dir(F, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
+                    S::start(F, R, n_r, n_th, n_ph, cx, cy);
                 } else {
                     float* o = out + 3 * pix;
                     if (accumulate) {
@@ -562,36 +917,57 @@ kerr_render_fp64(const FrameConst F, const int y0, const int rows, const int spp
         wave_add_u64(&counters->stepcap, c_cap_steps);
         wave_add_u64(&counters->disk_hits, c_disk);
     }
-    if (F.check_inv && inv_max3) {
+    if (INV && inv_max3) {
         atomic_max_nonneg(&inv_max3[0], inv_lane[0]);
         atomic_max_nonneg(&inv_max3[1], inv_lane[1]);
         atomic_max_nonneg(&inv_max3[2], inv_lane[2]);
     }
 }
 
+template <class S, bool A>
+const void* kernel_ptr(bool inv) {
+    return inv ? reinterpret_cast<const void*>(&kerr_render<S, A, true>)
+               : reinterpret_cast<const void*>(&kerr_render<S, A, false>);
+}
+
+const void* select_kernel(int precision, bool adaptive, bool inv) {
+    if (precision == NP_PREC_FP32C)
+        return adaptive ? kernel_ptr<StepF32C, true>(inv) : kernel_ptr<StepF32C, false>(inv);
+    return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
+}
+
 }  // namespace
 
-int launch_kerr_fp64(const FrameConst& F, bool adaptive, int y0, int rows, int spp,
-                     int sample_offset, float* d_out, int accumulate, float* d_inv,
-                     np_trace_counters* d_counters, np_ray_record* d_records,
-                     unsigned long long* d_work, hipStream_t stream, int grid) {
+int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int rows, int spp,
+                int sample_offset, float* d_out, int accumulate, float* d_inv,
+                np_trace_counters* d_counters, np_ray_record* d_records,
+                unsigned long long* d_work, hipStream_t stream, int grid) {
     const dim3 block(256);
-    if (adaptive) {
-        hipLaunchKernelGGL(kerr_render_fp64<true>, dim3(grid), block, 0, stream, F, y0, rows, spp,
-                           sample_offset, d_out, accumulate, d_inv, d_counters, d_records, d_work);
+#define NP_LAUNCH(S, A)                                                                          \
+    do {                                                                                         \
+        if (F.check_inv)                                                                         \
+            hipLaunchKernelGGL((kerr_render<S, A, true>), dim3(grid), block, 0, stream, F, y0,   \
+                               rows, spp, sample_offset, d_out, accumulate, d_inv, d_counters,   \
+                               d_records, d_work);                                               \
+        else                                                                                     \
+            hipLaunchKernelGGL((kerr_render<S, A, false>), dim3(grid), block, 0, stream, F, y0,  \
+                               rows, spp, sample_offset, d_out, accumulate, d_inv, d_counters,   \
+                               d_records, d_work);                                               \
+    } while (0)
+    if (precision == NP_PREC_FP32C) {
+        if (adaptive) NP_LAUNCH(StepF32C, true);
+        else NP_LAUNCH(StepF32C, false);
     } else {
-        hipLaunchKernelGGL(kerr_render_fp64<false>, dim3(grid), block, 0, stream, F, y0, rows, spp,
-                           sample_offset, d_out, accumulate, d_inv, d_counters, d_records, d_work);
+        if (adaptive) NP_LAUNCH(StepF64, true);
+        else NP_LAUNCH(StepF64, false);
     }
+#undef NP_LAUNCH
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
-int occupancy_kerr_fp64(bool adaptive, int* blocks_per_cu) {
-    hipError_t e;
-    if (adaptive)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kerr_render_fp64<true>, 256, 0);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kerr_render_fp64<false>, 256, 0);
+int occupancy_kerr(int precision, bool adaptive, bool inv, int* blocks_per_cu) {
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, select_kernel(precision, adaptive, inv), 256, 0);
     return e == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

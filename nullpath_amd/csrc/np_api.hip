@@ -72,9 +72,9 @@ int take_slot(DeviceState* d, hipStream_t stream, unsigned long long** slot) {
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
-int persistent_grid(DeviceState* d, bool adaptive, uint64_t items) {
+int persistent_grid(DeviceState* d, int precision, bool adaptive, bool inv, uint64_t items) {
     int per_cu = 0;
-    if (occupancy_kerr_fp64(adaptive, &per_cu) != NP_OK || per_cu <= 0) per_cu = 1;
+    if (occupancy_kerr(precision, adaptive, inv, &per_cu) != NP_OK || per_cu <= 0) per_cu = 1;
     uint64_t grid = (uint64_t)per_cu * (uint64_t)d->cus;
     uint64_t need = (items + 255) / 256;
     if (need < grid) grid = need;
@@ -86,7 +86,7 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
                   float* d_rgb, int accumulate, float* d_inv, np_trace_counters* d_counters,
                   np_ray_record* d_records, hipStream_t stream) {
     if (!p || !d_rgb || spp < 0 || rows < 0 || y0 < 0 || sample_offset < 0) return NP_EINVAL;
-    if (p->precision != NP_PREC_FP64) return NP_EINVAL;
+    if (p->precision != NP_PREC_FP64 && p->precision != NP_PREC_FP32C) return NP_EINVAL;
     FrameConst F;
     int err = make_frame(p, &F);
     if (err != NP_OK) return err;
@@ -99,12 +99,19 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     err = take_slot(d, stream, &slot);
     if (err != NP_OK) return err;
     const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
-    const int grid = persistent_grid(d, adaptive, npix);
-    return launch_kerr_fp64(F, adaptive, y0, rows, spp, sample_offset, d_rgb, accumulate, d_inv,
-                            d_counters, d_records, slot, stream, grid);
+    const int grid = persistent_grid(d, p->precision, adaptive, p->check_inv != 0, npix);
+    return launch_kerr(F, p->precision, adaptive, y0, rows, spp, sample_offset, d_rgb, accumulate,
+                       d_inv, d_counters, d_records, slot, stream, grid);
 }
 
 }  // namespace
+
+int take_work_slot(hipStream_t stream, unsigned long long** slot) {
+    int err = NP_OK;
+    DeviceState* d = device_state(&err);
+    if (!d) return err;
+    return take_slot(d, stream, slot);
+}
 
 }  // namespace np
 

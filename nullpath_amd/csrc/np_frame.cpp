@@ -101,6 +101,16 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->rout = p->disk_rout;
     f->rtol = p->rtol;
     f->atol = p->atol;
+    f->s_eq = std::sin(3.141592653589793238462643383279502884 * 0.5);
+    f->c_eq = std::cos(3.141592653589793238462643383279502884 * 0.5);
+    f->Mf = (float)M;
+    f->af = (float)a;
+    f->a2f = f->af * f->af;
+    f->rpf = (float)f->r_plus;
+    f->rmf = (float)(M - std::sqrt(disc));
+    f->rpf_guard = (float)(f->r_plus + 1e-12);
+    f->rtolf = (float)p->rtol;
+    f->atolf = (float)p->atol;
 
     double g[5];
     cov_blocks(M, a, f->r_cam, f->th_cam, g);

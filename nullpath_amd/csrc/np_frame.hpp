@@ -16,6 +16,7 @@ struct FrameConst {
     // geometry (absolute units)
     double M, a, r_plus, r_cam, th_cam, ph_cam, Rs_d, r_far, rin, rout;
     double rtol, atol;
+    double s_eq, c_eq;   // sin, cos of pi/2 as the host libm returns them
     // observer tetrad: p^mu components as functions of the local direction n
     //   p^t   = ct0 + n_phi * ct_ph
     //   p^r   = n_r * cr
@@ -24,6 +25,8 @@ struct FrameConst {
     double ct0, ct_ph, cr, cth, cph0, cph_ph;
     // covariant metric at the camera, for lowering p^mu
     double g_tt, g_tphi, g_phiphi, g_rr, g_thth;
+    // FP32 copies for the compensated stepper
+    float Mf, af, a2f, rpf, rmf, rpf_guard, rtolf, atolf;
     // float pinhole camera (render.cu:137-139)
     float aspect, tan_half, Rs, mass, disk_gain, emiss_q, bg_sigma_rad, bg_peak;
     uint32_t salt;        // jitter salt (0 = reference hash keys)

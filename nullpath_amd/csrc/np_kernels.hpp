@@ -8,11 +8,15 @@
 
 namespace np {
 
-int launch_kerr_fp64(const FrameConst& F, bool adaptive, int y0, int rows, int spp,
-                     int sample_offset, float* d_out, int accumulate, float* d_inv,
-                     np_trace_counters* d_counters, np_ray_record* d_records,
-                     unsigned long long* d_work,
-                     hipStream_t stream, int grid);
-int occupancy_kerr_fp64(bool adaptive, int* blocks_per_cu);
+// Kerr renderer; precision is an np_precision value.
+int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int rows, int spp,
+                int sample_offset, float* d_out, int accumulate, float* d_inv,
+                np_trace_counters* d_counters, np_ray_record* d_records,
+                unsigned long long* d_work, hipStream_t stream, int grid);
+int occupancy_kerr(int precision, bool adaptive, bool inv, int* blocks_per_cu);
+
+// A cleared work-queue counter on the current device (cleared by a kernel
+// on `stream`, so the call may be graph-captured).
+int take_work_slot(hipStream_t stream, unsigned long long** slot);
 
 }  // namespace np

@@ -255,7 +255,7 @@ double orc_elldoublecomplex(const int p[5], double f1, double g1, double h1, dou
 /* ---------------- mu integrals ---------------- */
 
 /* calcimuasym (geokerr_original.f:3137-3172). */
-static void calcimuasym(double a, double mneg, double mpos, double mu0, double muplus,
+void orc_calcimuasym(double a, double mneg, double mpos, double mu0, double muplus,
                         double* imu0, double* imum, double* rf0, double* rfc) {
     double fac = fabs(a) * muplus;
     double m1 = mneg / mpos;
@@ -273,7 +273,7 @@ static void calcimuasym(double a, double mneg, double mpos, double mu0, double m
 }
 
 /* calcimusym (geokerr_original.f:3202-3240). */
-static void calcimusym(double a, double mneg, double mpos, double mu0, double muplus,
+void orc_calcimusym(double a, double mneg, double mpos, double mu0, double muplus,
                        double* imu0, double* imum, double* rf0, double* rfc) {
     double fac = fabs(a) * sqrt(mpos - mneg);
     double m1 = -mneg / (mpos - mneg);
@@ -644,7 +644,7 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
         if (mneg < 0.0) {
             if (muplus < mu0) { muplus = mu0; mpos = muplus * muplus; }
             double muminus = -muplus;
-            calcimusym(a, mneg, mpos, mu0, muplus, &i1mu, &i3mu, &rffmu1, &rffmu3);
+            orc_calcimusym(a, mneg, mpos, mu0, muplus, &i1mu, &i3mu, &rffmu1, &rffmu3);
             double a1 = sm;
             if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0);
             else tpm = f_int((iu + i1mu) / i3mu);
@@ -661,7 +661,7 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             double muminus = sgn1(mu0) * sqrt(mneg);
             if (fabs(muminus) > fabs(mu0)) { muminus = mu0; mneg = muminus * muminus; }
             if (fabs(mu0) > fabs(muplus)) { muplus = mu0; mpos = muplus * muplus; }
-            calcimuasym(a, mneg, mpos, mu0, muplus, &i1mu, &i3mu, &rffmu1, &rffmu3);
+            orc_calcimuasym(a, mneg, mpos, mu0, muplus, &i1mu, &i3mu, &rffmu1, &rffmu3);
             double a1 = sm;
             if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int(fabs((1 + sgn1((iu - i1mu) / i3mu)) / 2.0));
             else tpm = f_int((iu + i1mu) / i3mu);

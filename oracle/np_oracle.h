@@ -119,6 +119,23 @@ typedef struct orc_geomu_out {
 void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
                double su, double sm, orc_geomu_out* o);
 void orc_sncndn(double uu, double emmc, double* sn, double* cn, double* dn);
+/* coefficients and roots are C99 double _Complex arrays, constant term first */
+void orc_zroots(const double _Complex* a, int m, double _Complex* roots, int polish);
+double orc_ellcubicreal(const int p[5], double a1, double b1, double a2, double b2, double a3,
+                        double b3, double a4, double b4, double* rff, double y, double x);
+double orc_ellcubiccomplex(const int p[5], double a1, double b1, double a4, double b4, double f,
+                           double g, double h, double* rff, double y, double x);
+double orc_ellquarticreal(const int p[5], double a1, double b1, double a2, double b2, double a3,
+                          double b3, double a4, double b4, double a5, double b5, double* rff,
+                          double y, double x);
+double orc_ellquarticcomplex(const int p[5], double a1, double b1, double a4, double b4, double a5,
+                             double b5, double f, double g, double h, double* rff, double y, double x);
+double orc_elldoublecomplex(const int p[5], double f1, double g1, double h1, double f2, double g2,
+                            double h2, double a5, double b5, double* rff, double y, double x);
+void orc_calcimuasym(double a, double mneg, double mpos, double mu0, double muplus,
+                     double* imu0, double* imum, double* rf0, double* rfc);
+void orc_calcimusym(double a, double mneg, double mpos, double mu0, double muplus,
+                    double* imu0, double* imum, double* rf0, double* rfc);
 double orc_asech(double x);
 void orc_geokerr_camera(double alpha, double beta, double mu0, double a_in, double* L,
                         double* Q2, double* a_out, double* su, double* sm);

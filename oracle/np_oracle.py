@@ -275,3 +275,99 @@ def rd(x, y, z):
 
 def rj(x, y, z, p):
     return lib().orc_rj(x, y, z, p)
+
+
+# ---- GEOKERR GEOMU (geomu_oracle.c) ----
+class GeomuOut(C.Structure):
+    _fields_ = [("muf", C.c_double), ("iu", C.c_double), ("u1", C.c_double), ("u2", C.c_double),
+                ("u3", C.c_double), ("u4", C.c_double), ("u0_used", C.c_double),
+                ("uf_used", C.c_double), ("i1mu", C.c_double), ("i3mu", C.c_double),
+                ("h1", C.c_double), ("tpm", C.c_int32), ("ncase", C.c_int32), ("tpr", C.c_int32),
+                ("warned", C.c_int32)]
+
+
+def _declare_geomu(L):
+    d, i32 = C.c_double, C.c_int
+    L.orc_geomu.argtypes = [d, d, d, d, d, d, i32, d, d, C.POINTER(GeomuOut)]
+    L.orc_geomu.restype = None
+    L.orc_sncndn.argtypes = [d, d, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+    L.orc_sncndn.restype = None
+    L.orc_zroots.argtypes = [C.c_void_p, i32, C.c_void_p, i32]
+    L.orc_zroots.restype = None
+    L.orc_geokerr_camera.argtypes = [d, d, d, d] + [C.POINTER(d)] * 5
+    L.orc_geokerr_camera.restype = None
+    L.orc_geomu_batch.argtypes = [C.c_void_p] * 5 + [d, i32] + [C.c_void_p] * 4 + [i32]
+    L.orc_geomu_batch.restype = None
+    pi = C.POINTER(C.c_int)
+    for name, nargs in (("orc_ellcubicreal", 8), ("orc_ellquarticreal", 10)):
+        f = getattr(L, name)
+        f.argtypes = [pi] + [d] * nargs + [C.POINTER(d), d, d]
+        f.restype = d
+    L.orc_ellcubiccomplex.argtypes = [pi] + [d] * 7 + [C.POINTER(d), d, d]
+    L.orc_ellcubiccomplex.restype = d
+    L.orc_ellquarticcomplex.argtypes = [pi] + [d] * 9 + [C.POINTER(d), d, d]
+    L.orc_ellquarticcomplex.restype = d
+    L.orc_elldoublecomplex.argtypes = [pi] + [d] * 8 + [C.POINTER(d), d, d]
+    L.orc_elldoublecomplex.restype = d
+
+
+_geomu_declared = False
+
+
+def _glib():
+    global _geomu_declared
+    L = lib()
+    if not _geomu_declared:
+        _declare_geomu(L)
+        _geomu_declared = True
+    return L
+
+
+def geomu(u0, uf, mu0, a, l, q2, tpr=0, su=1.0, sm=1.0) -> dict:
+    o = GeomuOut()
+    _glib().orc_geomu(u0, uf, mu0, a, l, q2, tpr, su, sm, C.byref(o))
+    return {k: getattr(o, k) for k, _ in GeomuOut._fields_}
+
+
+def sncndn(u, emmc):
+    sn, cn, dn = C.c_double(), C.c_double(), C.c_double()
+    _glib().orc_sncndn(u, emmc, C.byref(sn), C.byref(cn), C.byref(dn))
+    return sn.value, cn.value, dn.value
+
+
+def zroots(coefs_low_first, polish=True):
+    a = np.ascontiguousarray(coefs_low_first, dtype=np.complex128)
+    m = a.size - 1
+    roots = np.zeros(m, dtype=np.complex128)
+    _glib().orc_zroots(a.ctypes.data, m, roots.ctypes.data, 1 if polish else 0)
+    return roots
+
+
+def geokerr_camera(alpha, beta, mu0, a):
+    L, Q2, aa, su, sm = (C.c_double() for _ in range(5))
+    _glib().orc_geokerr_camera(alpha, beta, mu0, a, C.byref(L), C.byref(Q2), C.byref(aa),
+                               C.byref(su), C.byref(sm))
+    return L.value, Q2.value, aa.value, su.value, sm.value
+
+
+def geomu_batch(alpha, beta, a, mu0, uf, u0, nthreads=0):
+    arrs = [_arr(v) for v in (alpha, beta, a, mu0, uf)]
+    n = arrs[0].size
+    muf = np.zeros(n)
+    tpm = np.zeros(n, np.int32)
+    ncase = np.zeros(n, np.int32)
+    iu = np.zeros(n)
+    _glib().orc_geomu_batch(*[x.ctypes.data for x in arrs], u0, n, muf.ctypes.data,
+                            tpm.ctypes.data, ncase.ctypes.data, iu.ctypes.data, nthreads)
+    return muf, tpm, ncase, iu
+
+
+def ell(kind, p, *args):
+    """Call one of the Carlson reductions: args are the coefficient list
+    followed by (y, x); returns (value, rff)."""
+    pp = (C.c_int * 5)(*p)
+    rff = C.c_double(0.0)
+    f = getattr(_glib(), "orc_" + kind)
+    *coef, y, x = args
+    val = f(pp, *[float(c) for c in coef], C.byref(rff), float(y), float(x))
+    return val, rff.value

@@ -1,0 +1,88 @@
+"""GPU parity of the GEOKERR kernels (np_carlson_batch, np_geomu_batch,
+np_geomu_camera_batch) against the CPU oracle and mpmath."""
+import numpy as np
+import pytest
+import torch
+
+import nullpath_amd as npa
+
+U0 = min(1e-4, 1.0 / (100.0 * (144.0 ** 2 + 144.0 ** 2)))
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["rf", "rc", "rd", "rj"])
+def test_carlson_batch_matches_oracle_and_mpmath(oracle, device, kind):
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 20
+    rng = np.random.default_rng(21)
+    n = 4096
+    x, y, z = (10 ** rng.uniform(-6, 3, size=n) for _ in range(3))
+    p = 10 ** rng.uniform(-6, 3, size=n)
+    p[: n // 8] *= -1.0   # principal-value branch of RJ
+    x[n // 8: n // 4] = 0.0
+    print(f"[gpu] np_carlson_batch {kind} n={n}", flush=True)
+    out = npa.carlson_batch(kind, _t(x), _t(y), _t(z), _t(p)).cpu().numpy()
+    f = {"rf": lambda i: oracle.rf(x[i], y[i], z[i]), "rc": lambda i: oracle.rc(x[i] + 0.0, y[i]),
+         "rd": lambda i: oracle.rd(x[i], y[i], z[i]), "rj": lambda i: oracle.rj(x[i], y[i], z[i], p[i])}[kind]
+    ref = np.array([f(i) for i in range(n)])
+    ok = np.isfinite(ref)
+    rel = np.abs(out[ok] - ref[ok]) / np.abs(ref[ok])
+    assert rel.max() < 1e-13, rel.max()
+    assert np.array_equal(np.isfinite(out), np.isfinite(ref))
+    mpf = {"rf": mpmath.elliprf, "rc": mpmath.elliprc, "rd": mpmath.elliprd, "rj": mpmath.elliprj}[kind]
+    for i in range(n // 4, n // 4 + 40):
+        args = {"rf": (x[i], y[i], z[i]), "rc": (x[i], y[i]), "rd": (x[i], y[i], z[i]),
+                "rj": (x[i], y[i], z[i], p[i])}[kind]
+        m = float(mpmath.re(mpf(*args)))
+        assert abs(out[i] - m) <= 1e-13 * abs(m)
+
+
+@pytest.mark.gpu
+def test_geomu_camera_batch_matches_oracle(oracle, device):
+    rng = np.random.default_rng(31)
+    n = 1 << 16
+    alpha = rng.uniform(-12, 12, n)
+    beta = rng.uniform(-12, 12, n)
+    a = rng.uniform(0, 0.998, n)
+    mu0 = rng.uniform(0.05, 0.99, n)
+    uf = rng.uniform(0, 1, n) / (1 + np.sqrt(1 - a * a))
+    print(f"[gpu] np_geomu_camera_batch n={n}", flush=True)
+    g_muf, g_tpm, g_nc, g_iu = (t.cpu().numpy() for t in
+                                npa.geomu_camera_batch(_t(alpha), _t(beta), _t(a), _t(mu0), _t(uf), U0))
+    o_muf, o_tpm, o_nc, o_iu = oracle.geomu_batch(alpha, beta, a, mu0, uf, U0)
+    assert np.array_equal(g_nc, o_nc)
+    assert np.all(np.abs(g_iu - o_iu) <= 1e-12 * np.abs(o_iu) + 1e-15)
+    same_tpm = g_tpm == o_tpm
+    assert same_tpm.mean() > 0.9999
+    good = same_tpm & np.isfinite(o_muf)
+    assert np.all(np.abs(g_muf[good] - o_muf[good]) <= 1e-9), np.abs(g_muf[good] - o_muf[good]).max()
+    assert np.all(np.abs(g_muf) <= 1.0 + 1e-12)
+
+
+@pytest.mark.gpu
+def test_geomu_batch_constants_interface(oracle, device):
+    rng = np.random.default_rng(41)
+    n = 4096
+    a = rng.uniform(0.05, 0.99, n)
+    L = rng.uniform(-6, 6, n)
+    Q2 = rng.uniform(0.5, 40, n)
+    mu0 = rng.uniform(-0.9, 0.9, n)
+    uf = rng.uniform(0.01, 0.4, n)
+    u0 = np.full(n, 1e-4)
+    su = np.ones(n)
+    sm = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    print(f"[gpu] np_geomu_batch n={n}", flush=True)
+    g_muf, g_tpm, g_nc, g_iu = (t.cpu().numpy() for t in npa.geomu_batch(
+        _t(u0), _t(uf), _t(mu0), _t(a), _t(L), _t(Q2), _t(su), _t(sm)))
+    ref = [oracle.geomu(u0[i], uf[i], mu0[i], a[i], L[i], Q2[i], 0, su[i], sm[i]) for i in range(n)]
+    o_nc = np.array([r["ncase"] for r in ref])
+    o_tpm = np.array([r["tpm"] for r in ref])
+    o_muf = np.array([r["muf"] for r in ref])
+    assert np.array_equal(g_nc, o_nc)
+    ok = g_tpm == o_tpm
+    assert ok.mean() > 0.999
+    assert np.all(np.abs(g_muf[ok] - o_muf[ok]) <= 1e-9)
