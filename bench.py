@@ -1,15 +1,17 @@
 #!/usr/bin/env python3
-"""Benchmark: Kerr thin-disk still on one MI355X.
+"""Benchmark: Kerr thin-disk still on one MI355X, FP32-compensated vs FP64.
 
 Workload (one "step" = one full frame): 1920x1080 pinhole camera at
 r = 1000 M, a = 0.9 M, theta_obs = 80 deg, fov 55 deg, 4 jittered samples
-per pixel, adaptive step-doubling RK4 (rtol 1e-8, atol 1e-12), disk from
-the ISCO to 20 M with emissivity (r/M)^-3 and g^3 redshift, up to 3
-equatorial crossings, float32 RGB output.  Inputs (frame constants) are
-resident before the timed region; the output stays in HBM.
+per pixel (seed 1), adaptive step-doubling RK4 (rtol 1e-8, atol 1e-12),
+disk from the ISCO to 20 M with emissivity (r/M)^-3 and g^3 redshift, up
+to 3 equatorial crossings, float32 RGB output.  The frame constants are
+resident before the timed region and the image stays in HBM.
 
-Prints one JSON line (see the keys below).  --gpus above 1 is refused: this
-build measures one GPU.
+Each step renders the frame with the FP64 path (reference formulas) and
+with the FP32-compensated path; `value` is the FP32-compensated rays/s and
+the FP64 rays/s, their ratio and the FP32-vs-FP64 error statistics sit
+next to it.  --gpus above 1 is refused: this build measures one GPU.
 """
 from __future__ import annotations
 
@@ -23,16 +25,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Kerr rays/s @1080p a=0.9, FP32-comp vs FP64 at ≤1e-5 err; adjoint grads/s"
-FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
+PEAK_TFLOPS = {"fp64": 78.6, "fp32c": 157.3}
 
-# Algorithmic FP64 operation counts of the reference formulas (add, sub,
-# mul, div, sqrt = 1; fma = 2), derived line by line from
-# bh_kerr.h:600-727 and 987-1105 and listed in DESIGN.md.  Transcendentals
-# (one sincos per RHS, one pow per attempt) are not included.
-FLOPS_RHS = 200          # one Hamilton right-hand side (metric, derivatives, two Klein sums)
-FLOPS_RK4_COMBINE = 196  # stage states + compensated k1+2k2+2k3+k4 + update, per RK4 step
-FLOPS_ATTEMPT_EXTRA = 29 # error norm and step-factor arithmetic per adaptive attempt
+# Algorithmic operation counts per unit (add, sub, mul, div, sqrt, rcp = 1;
+# fma = 2; sin/cos/pow counted apart), derived line by line from the kernel
+# source in DESIGN.md:
+#   fp64: the reference right-hand side (bh_kerr.h:600-727) is 200 flops;
+#         one RK4 stage combination with compensated sums + update is 196;
+#         error norm and step factor of one attempt is 29.
+#   fp32c: the Carter-form right-hand side is 55 flops; stage input plus
+#          stage-sum accumulation is 18 per stage (10 per attempt); each of
+#          the 3 float-float updates is 56; error norm and factor is 40.
+FLOPS = {
+    "fp64": {"rhs": 200, "attempt": 3 * 196 + 29},
+    "fp32c": {"rhs": 55, "attempt": 10 * 18 + 3 * 56 + 40},
+}
 
 
 def parse():
@@ -43,9 +50,10 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=4)
-    ap.add_argument("--cpu-pixels", type=int, default=160,
+    ap.add_argument("--cpu-pixels", type=int, default=3000,
                     help="pixels (x spp rays) in the CPU-baseline sample; 0 skips it")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--precisions", default="fp64,fp32c")
     return ap.parse_args()
 
 
@@ -59,48 +67,55 @@ def main() -> int:
     import torch
 
     import nullpath_amd as npa
+    from nullpath_amd._lib import ray_record_dtype
 
     dev = torch.device("cuda:0")
-    p = npa.kerr_disk_params(width=args.width, height=args.height, spin=0.9, theta_deg=80.0,
-                             fov=55.0, rtol=1e-8, max_orders=3, q=3.0, seed=1)
-    npix = p.width * p.height
+    precs = args.precisions.split(",")
+    npix = args.width * args.height
     rays_per_step = npix * args.spp
-    out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
-    cnt = torch.zeros(8, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
+    res = {}
+    for prec in precs:
+        pid = 1 if prec == "fp32c" else 0
+        p = npa.kerr_disk_params(width=args.width, height=args.height, spin=0.9, theta_deg=80.0,
+                                 fov=55.0, rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid)
+        out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
+        cnt = torch.zeros(8, dtype=torch.int64, device=dev)
+        rec = torch.zeros(rays_per_step * 32, dtype=torch.uint8, device=dev)
+        warm = npa.kerr_disk_params(width=64, height=36, seed=1, precision=pid)
+        npa.render_tile(warm, out, spp=1)
+        for _ in range(args.warmup):
+            npa.render_tile(p, out, spp=args.spp)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            cnt.zero_()
+            ev[k][0].record(stream)
+            npa.render_tile(p, out, spp=args.spp, counters=cnt, records=rec)
+            ev[k][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        print(f"[bench] {prec}: {args.steps} step(s) in {wall:.2f} s", file=sys.stderr, flush=True)
+        c = npa.counters_dict(cnt)
+        assert c["rays"] == rays_per_step, c
+        kern_ms = [a.elapsed_time(b) for a, b in ev]
+        mean_s = sum(kern_ms) / len(kern_ms) / 1e3
+        fl = FLOPS[prec]
+        flops = c["rhs_evals"] * fl["rhs"] + c["attempts"] * fl["attempt"]
+        res[prec] = {
+            "wall": wall, "kern_ms": kern_ms, "mean_s": mean_s, "counters": c, "flops": flops,
+            "achieved": flops / mean_s / 1e12,
+            "img": out.view(args.height, args.width, 3).cpu().numpy(),
+            "rec": rec.cpu().numpy().view(ray_record_dtype()),
+            "params": p,
+        }
 
-    # load the code object and prime the queue slots on a tiny frame
-    warm = npa.kerr_disk_params(width=64, height=36, spin=0.9, theta_deg=80.0, fov=55.0,
-                                rtol=1e-8, max_orders=3, q=3.0, seed=1)
-    npa.render_tile(warm, out, spp=1)
-    for _ in range(args.warmup):
-        npa.render_tile(p, out, spp=args.spp)
-    torch.cuda.synchronize()
-
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        cnt.zero_()
-        ev[k][0].record(stream)
-        npa.render_tile(p, out, spp=args.spp, counters=cnt)
-        ev[k][1].record(stream)
-        print(f"[bench] step {k + 1}/{args.steps} launched", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
-    c = npa.counters_dict(cnt)
-    assert c["rays"] == rays_per_step, c
-
-    ms_per_step = wall * 1e3 / args.steps
-    value = rays_per_step * args.steps / wall
-    mean_kern_s = sum(kern_ms) / len(kern_ms) / 1e3
-    rk4_steps = 3 * c["attempts"]
-    flops = c["rhs_evals"] * FLOPS_RHS + rk4_steps * FLOPS_RK4_COMBINE + c["attempts"] * FLOPS_ATTEMPT_EXTRA
-    achieved = flops / mean_kern_s / 1e12
-
-    img = out.view(p.height, p.width, 3).cpu().numpy()
+    head = "fp32c" if "fp32c" in res else precs[0]
+    R = res[head]
+    c = R["counters"]
+    value = rays_per_step * args.steps / R["wall"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -108,49 +123,81 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": R["wall"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if head == "fp32c" else "f64",
         "data": "synthetic (seeded jittered pinhole rays; no external data)",
         "config": {
-            "workload": f"Kerr thin-disk still {p.width}x{p.height}x{args.spp}spp a=0.9 "
-                        f"theta=80deg fov=55deg r_cam=1000M adaptive rtol=1e-8 disk ISCO-20M q=3 "
-                        f"<=3 crossings, FP64 path",
-            "precision": "fp64",
+            "workload": (f"Kerr thin-disk still {args.width}x{args.height}x{args.spp}spp a=0.9 "
+                         f"theta=80deg fov=55deg r_cam=1000M adaptive rtol=1e-8 disk ISCO-20M q=3 "
+                         f"<=3 crossings"),
+            "precision": head,
             "rays_per_step": rays_per_step,
-            "accepted_steps_per_ray": c["steps"] / c["rays"],
-            "attempts_per_ray": c["attempts"] / c["rays"],
-            "rk4_steps_per_s": rk4_steps / mean_kern_s,
-            "rhs_evals_per_s": c["rhs_evals"] / mean_kern_s,
-            "captured": c["captured"], "escaped": c["escaped"], "stepcap": c["stepcap"],
-            "disk_rays": c["disk_hits"],
-            "kernel_ms": kern_ms,
-            "image_mean_rgb": [float(v) for v in img.reshape(-1, 3).mean(axis=0)],
         },
         "roofline": {
             "bound": "valu",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
+            "achieved": R["achieved"],
+            "peak": PEAK_TFLOPS[head],
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
+            "frac": R["achieved"] / PEAK_TFLOPS[head],
             "traffic": None,
-            "kernel": "kerr_render_fp64<adaptive>",
-            "flops_per_launch": flops,
+            "kernel": f"kerr_render<{'StepF32C' if head == 'fp32c' else 'StepF64'}, adaptive>",
+            "flops_per_launch": R["flops"],
         },
         "cpu_baseline": None,
     }
+    for prec, r in res.items():
+        cc = r["counters"]
+        line["config"][prec] = {
+            "rays_per_s": rays_per_step * args.steps / r["wall"],
+            "kernel_ms": r["kern_ms"],
+            "accepted_steps_per_ray": cc["steps"] / cc["rays"],
+            "attempts_per_ray": cc["attempts"] / cc["rays"],
+            "rk4_steps_per_s": 3 * cc["attempts"] / r["mean_s"],
+            "rhs_evals_per_s": cc["rhs_evals"] / r["mean_s"],
+            "achieved_tflops": r["achieved"],
+            "frac_of_peak": r["achieved"] / PEAK_TFLOPS[prec],
+            "captured": cc["captured"], "escaped": cc["escaped"], "stepcap": cc["stepcap"],
+            "disk_rays": cc["disk_hits"],
+        }
+    if "fp64" in res and "fp32c" in res:
+        a, b = res["fp64"], res["fp32c"]
+        line["config"]["speedup_fp32c_over_fp64"] = a["wall"] / b["wall"]
+        ra = a["rec"].reshape(npix, args.spp)
+        rb = b["rec"].reshape(npix, args.spp)
+        same_ray = (ra["cls"] == rb["cls"]) & (ra["orders"] == rb["orders"])
+        same_pix = same_ray.all(axis=1)
+        ia = a["img"].reshape(-1, 3)
+        ib = b["img"].reshape(-1, 3)
+        disk = (ra["orders"] > 0).any(axis=1) & same_pix
+        rel = np.abs(ib - ia) / np.maximum(np.abs(ia), 1e-30)
+        rel_disk = rel[disk].max(axis=1) if disk.any() else np.zeros(1)
+        sky = same_pix & ~(ra["orders"] > 0).any(axis=1)
+        line["config"]["fp32c_vs_fp64"] = {
+            "ray_class_mismatch": int((~same_ray).sum()),
+            "ray_class_mismatch_frac": float((~same_ray).mean()),
+            "mismatch_min_r_fp64_max": float(ra["min_r"][~same_ray].max()) if (~same_ray).any() else None,
+            "disk_pixels_compared": int(disk.sum()),
+            "disk_rel_err_max": float(rel_disk.max()),
+            "disk_rel_err_p999": float(np.quantile(rel_disk, 0.999)),
+            "disk_rel_err_p99": float(np.quantile(rel_disk, 0.99)),
+            "disk_rel_err_median": float(np.median(rel_disk)),
+            "disk_frac_within_1e-5": float(np.mean(rel_disk <= 1e-5)),
+            "sky_pixels_max_abs_diff": float(np.abs(ib[sky] - ia[sky]).max()) if sky.any() else 0.0,
+        }
 
-    if args.cpu_pixels > 0:
+    if args.cpu_pixels > 0 and "fp64" in res:
         from oracle import np_oracle as O
+        p = res["fp64"]["params"]
         q = O.default_params()
         for name, _ in npa.RenderParams._fields_:
             setattr(q, name, getattr(p, name))
         rng = np.random.default_rng(12345)
-        sel = rng.choice(npix, size=args.cpu_pixels, replace=False)
-        xs = (sel % p.width).astype(np.int32)
-        ys = (sel // p.width).astype(np.int32)
+        sel = rng.choice(npix, size=min(args.cpu_pixels, npix), replace=False)
+        xs = (sel % args.width).astype(np.int32)
+        ys = (sel // args.width).astype(np.int32)
         try:
             ncpu = len(os.sched_getaffinity(0))
         except AttributeError:
@@ -159,16 +206,16 @@ def main() -> int:
         t1 = time.perf_counter()
         o_rgb, _ = O.render_pixels(q, xs, ys, 0, args.spp, nthreads=threads, with_info=False)
         cpu_s = time.perf_counter() - t1
-        g_rgb = img.reshape(-1, 3)[sel]
+        g_rgb = res["fp64"]["img"].reshape(-1, 3)[sel]
         rel = np.abs(g_rgb - o_rgb) / np.maximum(np.abs(o_rgb), 1e-6)
         line["cpu_baseline"] = {
-            "value": args.cpu_pixels * args.spp / cpu_s,
+            "value": len(sel) * args.spp / cpu_s,
             "unit": "rays/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{args.cpu_pixels} seeded random pixels x {args.spp} spp of the same frame "
-                      f"({args.cpu_pixels * args.spp} rays, {cpu_s:.1f} s), FP64 oracle restatement, OpenMP",
-            "gpu_vs_oracle_max_rel_err_on_sample": float(rel.max()),
+            "sample": (f"{len(sel)} seeded random pixels x {args.spp} spp of the same frame "
+                       f"({len(sel) * args.spp} rays, {cpu_s:.1f} s), FP64 oracle restatement, OpenMP"),
+            "gpu_fp64_vs_oracle_max_rel_err_on_sample": float(rel.max()),
         }
     print(json.dumps(line))
     return 0

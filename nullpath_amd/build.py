@@ -38,8 +38,20 @@ def build_lib() -> str:
     return LIB
 
 
+def build_cli() -> str:
+    """The `render` command line, linked against libnullpath.so next to it."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    exe = os.path.join(BIN_DIR, "render")
+    tmp = exe + ".tmp"
+    _run([HIPCC, *COMMON, "-x", "hip", os.path.join(CSRC, "render_main.cpp"), "-x", "none",
+          f"-L{PKG}", "-lnullpath", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+    os.replace(tmp, exe)
+    return exe
+
+
 def build_all() -> None:
     build_lib()
+    build_cli()
 
 
 if __name__ == "__main__":
