@@ -54,7 +54,64 @@ def parse():
                     help="pixels (x spp rays) in the CPU-baseline sample; 0 skips it")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--precisions", default="fp64,fp32c")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also time the differentiable 512x512 render (forward + adjoint)")
+    ap.add_argument("--adjoint-size", type=int, default=512)
     return ap.parse_args()
+
+
+def bench_adjoint(args) -> dict:
+    """Gradients/s of the differentiable disk render: 512x512, 4096 fixed
+    RK4 steps per ray (FP32-compensated), checkpoint every 64 steps; loss
+    sum((I - I_target)^2) with the target at (a=0.7, theta=60 deg, q=3,
+    gain=1) and the gradient taken at (0.6, 65 deg, 2.5, 0.8).  One gradient
+    = one forward + one backward."""
+    import math
+
+    import torch
+
+    from nullpath_amd import adjoint as A
+    n = args.adjoint_size
+    base = A.adjoint_params(width=n, height=n)
+    ws = A.Workspace(base)
+    target = A.with_params4(base, (0.7, math.radians(60.0), 3.0, 1.0))
+    start = A.with_params4(base, (0.6, math.radians(65.0), 2.5, 0.8))
+    tgt = A.forward(target, A.PREC_FP32C, ws=ws).clone()
+    img = torch.empty_like(tgt)
+    gimg = torch.empty_like(tgt)
+    g4 = torch.empty(4, dtype=torch.float64, device=tgt.device)
+    stream = torch.cuda.current_stream()
+
+    def one():
+        A.forward(start, A.PREC_FP32C, image=img, ws=ws)
+        torch.sub(img, tgt, out=gimg).mul_(2.0)
+        A.backward(start, gimg, ws, grad4=g4)
+
+    one()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        evs[k][0].record(stream)
+        A.forward(start, A.PREC_FP32C, image=img, ws=ws)
+        evs[k][1].record(stream)
+        torch.sub(img, tgt, out=gimg).mul_(2.0)
+        A.backward(start, gimg, ws, grad4=g4)
+        evs[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd = [a.elapsed_time(b) for a, b, _ in evs]
+    tot = [a.elapsed_time(c) for a, _, c in evs]
+    return {
+        "workload": f"{n}x{n}, 4096 fixed RK4 steps/ray FP32-comp, checkpoint every 64",
+        "grads_per_s": steps / wall,
+        "ms_per_grad": wall * 1e3 / steps,
+        "forward_ms": fwd,
+        "forward_backward_ms": tot,
+        "grad": g4.cpu().tolist(),
+        "workspace_bytes": ws.nbytes,
+    }
 
 
 def main() -> int:
@@ -187,6 +244,9 @@ def main() -> int:
             "disk_frac_within_1e-5": float(np.mean(rel_disk <= 1e-5)),
             "sky_pixels_max_abs_diff": float(np.abs(ib[sky] - ia[sky]).max()) if sky.any() else 0.0,
         }
+
+    if args.adjoint:
+        line["config"]["adjoint"] = bench_adjoint(args)
 
     if args.cpu_pixels > 0 and "fp64" in res:
         from oracle import np_oracle as O

@@ -168,6 +168,74 @@ int np_schw_sweep(double M, double r0, double r_capture, double ps_band, int64_t
                   const double* d_b, int64_t n, int32_t* d_cls, double* d_dphi, int64_t* d_steps,
                   uint64_t* d_total_steps, void* stream);
 
+/*
+ * Differentiable disk render (no counterpart in the reference, which has
+ * no gradients; it reuses the reference's camera (render.cu:143-159,
+ * bh_kerr.h:897-964), Keplerian redshift (bh_kerr.h:833-862) and crossing
+ * interpolation (render.cu:211-226)).  M = 1.  One sample per pixel; fixed
+ * RK4 steps of size h in s, dlambda = r ds; a crossing at radius r adds
+ * gain W(r) r^-q g^3 (1, 0.75, 0.45), W a C^1 smoothstep over edge_width
+ * rising from the ISCO and falling to disk_rout.  Gradients are taken with
+ * respect to (spin, theta_obs, emiss_q, gain).
+ */
+typedef struct np_adjoint_params {
+    int32_t width, height;
+    float fov_y_deg;
+    uint32_t seed;       /* pixel jitter salt, 0 = the renderer's unsalted jitter */
+    double r_cam;        /* camera radius (M) */
+    double theta_obs;    /* camera polar angle (rad) */
+    double spin;         /* a/M, |a| < 1 */
+    double emiss_q;      /* emissivity index q */
+    double gain;         /* disk gain */
+    double disk_rout;    /* outer disk edge (M) */
+    double edge_width;   /* smoothstep width of both edges (M) */
+    double h;            /* step in s */
+    int32_t n_steps;     /* steps per ray */
+    int32_t ckpt_every;  /* checkpoint interval (steps) */
+    int32_t max_orders;  /* crossings counted, 1..3 */
+    int32_t reserved;
+} np_adjoint_params;
+
+/* 512 x 512, r_cam = 1000, theta_obs = 60 deg, a = 0.7, q = 3, gain = 1,
+ * disk to 20 M with 0.5 M edges, 4096 steps of h = 0.005, checkpoint every
+ * 64 steps, 3 crossings, fov 2.8 deg. */
+static inline void np_adjoint_params_default(np_adjoint_params* p) {
+    p->width = 512;
+    p->height = 512;
+    p->fov_y_deg = 2.8f;
+    p->seed = 1u;
+    p->r_cam = 1000.0;
+    p->theta_obs = 60.0 * 3.14159265358979323846 / 180.0;
+    p->spin = 0.7;
+    p->emiss_q = 3.0;
+    p->gain = 1.0;
+    p->disk_rout = 20.0;
+    p->edge_width = 0.5;
+    p->h = 0.005;
+    p->n_steps = 4096;
+    p->ckpt_every = 64;
+    p->max_orders = 3;
+    p->reserved = 0;
+}
+
+/* Device workspace bytes np_adjoint_forward (FP32C) and np_adjoint_backward
+ * need (checkpoints, crossing steps, segment recompute), or -1 for invalid
+ * parameters.  About 2.6 KB per pixel at the defaults. */
+int64_t np_adjoint_workspace_bytes(const np_adjoint_params* p);
+
+/* Forward render into d_image (float RGB, width*height*3).  precision
+ * NP_PREC_FP32C writes the checkpoints into d_workspace (required);
+ * NP_PREC_FP64 needs no workspace and, when d_isum is not NULL, also
+ * writes the per-pixel FP64 radiance sum (for finite differences). */
+int np_adjoint_forward(const np_adjoint_params* p, int precision, float* d_image, void* d_workspace,
+                       double* d_isum, void* stream);
+
+/* Backward: given dLoss/dImage (float RGB), writes the four gradients
+ * dLoss/d(spin, theta_obs, emiss_q, gain) into d_grad4 (double[4]).  Uses
+ * the workspace of the last FP32C forward with the same parameters. */
+int np_adjoint_backward(const np_adjoint_params* p, const float* d_grad_image, void* d_workspace,
+                        double* d_grad4, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,24 @@ class RenderParams(C.Structure):
         return q
 
 
+class AdjointParams(C.Structure):
+    """np_adjoint_params (include/nullpath.h)."""
+
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("fov_y_deg", C.c_float),
+        ("seed", C.c_uint32), ("r_cam", C.c_double), ("theta_obs", C.c_double),
+        ("spin", C.c_double), ("emiss_q", C.c_double), ("gain", C.c_double),
+        ("disk_rout", C.c_double), ("edge_width", C.c_double), ("h", C.c_double),
+        ("n_steps", C.c_int32), ("ckpt_every", C.c_int32), ("max_orders", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+    def copy(self) -> "AdjointParams":
+        q = AdjointParams()
+        C.pointer(q)[0] = self
+        return q
+
+
 class TraceCounters(C.Structure):
     """np_trace_counters (include/nullpath.h)."""
 
@@ -95,6 +113,13 @@ def _declare(L: C.CDLL) -> None:
     L.np_precompute_geodesics.restype = i32
     L.np_free_lookup_table.argtypes = [C.POINTER(LookupTable)]
     L.np_free_lookup_table.restype = None
+    pa = C.POINTER(AdjointParams)
+    L.np_adjoint_workspace_bytes.argtypes = [pa]
+    L.np_adjoint_workspace_bytes.restype = i64
+    L.np_adjoint_forward.argtypes = [pa, i32, vp, vp, vp, vp]
+    L.np_adjoint_forward.restype = i32
+    L.np_adjoint_backward.argtypes = [pa, vp, vp, vp, vp]
+    L.np_adjoint_backward.restype = i32
 
 
 class LookupTable(C.Structure):

@@ -584,13 +584,16 @@ __device__ GeomuResult geomu(double u0, double uf, const double mu0, const doubl
             const double fac = fabs(a) * sqrt(mpos - mneg);
             const double mm1 = -mneg / (mpos - mneg);
             const double phi0 = acos(mu0 / muplus);
+            // tan(phi0) < 0 for phi0 in [0, pi] is phi0 > pi/2; compared
+            // against the double nearest pi/2 so that the branch does not
+            // hinge on the last bit of a device tan() at pi/2.
             const double s0 = sin(phi0);
             const double ak = sqrt(1.0 - mm1);
             const double q0 = (1.0 - s0 * ak) * (1.0 + s0 * ak);
             const double rf0 = rf_d(1.0 - s0 * s0, q0, 1.0);
             const double rfc = rf_d(0.0, mm1, 1.0);
             double imum = 2.0 * rfc, imu0 = s0 * rf0;
-            if (tan(phi0) < 0.0) imu0 = imum - imu0;
+            if (phi0 > 1.5707963267948966) imu0 = imum - imu0;
             i1mu = imu0 / fac;
             i3mu = imum / fac;
         } else {
@@ -601,14 +604,16 @@ __device__ GeomuResult geomu(double u0, double uf, const double mu0, const doubl
             // calcimuasym (geokerr_original.f:3137-3172)
             const double fac = fabs(a) * muplus;
             const double mm1 = mneg / mpos;
-            const double phi0 = asin(sqrt((mpos - mu0 * mu0) / (mpos - mneg)));
+            // the reference's tan(asin(s0)) < 0 test (geokerr_original.f:3165)
+            // never holds for s0 in [0, 1] with a correctly rounded asin
+            // (asin(1) rounds below pi/2), so the branch is omitted; a device
+            // asin(1) one ulp above pi/2 would otherwise flip it.
             const double s0 = sqrt((mpos - mu0 * mu0) / (mpos - mneg));
             const double ak = sqrt(1.0 - mm1);
             const double q0 = (1.0 - s0 * ak) * (1.0 + s0 * ak);
             const double rf0 = rf_d(1.0 - s0 * s0, q0, 1.0);
             const double rfc = rf_d(0.0, mm1, 1.0);
             double imum = 2.0 * rfc, imu0 = s0 * rf0;
-            if (tan(phi0) < 0.0) imu0 = imum - imu0;
             i1mu = imu0 / fac;
             i3mu = imum / fac / 2.0;
         }

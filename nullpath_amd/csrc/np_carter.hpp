@@ -125,4 +125,26 @@ NP_HD inline CState<Real> carter_vjp(const CState<Real>& y, Real E, Real L, Real
     return u;
 }
 
+// The same flow in the parameter s with dlambda = r ds: F = r f.  A fixed
+// step in s is a geometric step in r far from the hole (dr/ds ~ -r), so a
+// fixed-step integration spans r = 1000 to the horizon in a few hundred
+// steps while keeping small steps near the photon orbit.
+template <typename Real>
+NP_HD inline CState<Real> carter_rhs_r(const CState<Real>& y, Real E, Real L, Real a, Real rp, Real rm) {
+    const CState<Real> f = carter_rhs(y, E, L, a, rp, rm);
+    return CState<Real>{y.r * f.r, y.r * f.th, y.r * f.pr, y.r * f.pth};
+}
+
+// VJP of F = r f: (dF/dy)^T v = r (df/dy)^T v + e_r (f . v), and the
+// E, L, a partials are those of f weighted by r v.
+template <typename Real>
+NP_HD inline CState<Real> carter_vjp_r(const CState<Real>& y, Real E, Real L, Real a, Real rp, Real rm,
+                                       const CState<Real>& v, Real& gE, Real& gL, Real& ga) {
+    const CState<Real> f = carter_rhs(y, E, L, a, rp, rm);
+    const CState<Real> rv{y.r * v.r, y.r * v.th, y.r * v.pr, y.r * v.pth};
+    CState<Real> u = carter_vjp(y, E, L, a, rp, rm, rv, gE, gL, ga);
+    u.r += f.r * v.r + f.th * v.th + f.pr * v.pr + f.pth * v.pth;
+    return u;
+}
+
 }  // namespace np

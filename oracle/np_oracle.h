@@ -15,6 +15,7 @@
 
 #include <stdint.h>
 #include "../include/np_render.h"
+#include "../include/nullpath.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -143,6 +144,16 @@ void orc_geokerr_camera(double alpha, double beta, double mu0, double a_in, doub
 void orc_geomu_batch(const double* alpha, const double* beta, const double* a, const double* mu0,
                      const double* uf, double u0, int n, double* muf, int32_t* tpm, int32_t* ncase,
                      double* iu, int nthreads);
+
+/* ---- Differentiable disk render, FP64 forward mode (adjoint_oracle.c) ---- */
+/* Per pixel: I and dI/d(spin, theta_obs, q, gain) (dI is npix x 4). */
+void orc_adj_pixels(const np_adjoint_params* p, int npix, const int32_t* xs, const int32_t* ys,
+                    double* rad, double* drad, int32_t* ncross, int nthreads);
+double orc_adj_isco(double a, double* d_da);
+double orc_adj_disk_term(double r, double E, double L, double a, double q, double gain, double rout,
+                         double ew);
+void orc_adj_zamo_init(double a, double th, double r_cam, const float n[3], double out[4]);
+void orc_adj_rhs(const double y[4], double E, double L, double a, double out[4]);
 
 #ifdef __cplusplus
 }

@@ -131,6 +131,91 @@ def _declare(L: C.CDLL) -> None:
         f.restype = d
 
 
+class AdjointParams(C.Structure):
+    """np_adjoint_params (include/nullpath.h)."""
+
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("fov_y_deg", C.c_float),
+        ("seed", C.c_uint32), ("r_cam", C.c_double), ("theta_obs", C.c_double),
+        ("spin", C.c_double), ("emiss_q", C.c_double), ("gain", C.c_double),
+        ("disk_rout", C.c_double), ("edge_width", C.c_double), ("h", C.c_double),
+        ("n_steps", C.c_int32), ("ckpt_every", C.c_int32), ("max_orders", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+def adjoint_params(**kw) -> AdjointParams:
+    """np_adjoint_params_default (include/nullpath.h), then keyword overrides."""
+    p = AdjointParams(width=512, height=512, fov_y_deg=2.8, seed=1, r_cam=1000.0,
+                      theta_obs=60.0 * np.pi / 180.0, spin=0.7, emiss_q=3.0, gain=1.0,
+                      disk_rout=20.0, edge_width=0.5, h=0.005, n_steps=4096, ckpt_every=64,
+                      max_orders=3, reserved=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+_adj_declared = False
+
+
+def _alib():
+    global _adj_declared
+    L = lib()
+    if not _adj_declared:
+        d, pd = C.c_double, C.POINTER(C.c_double)
+        L.orc_adj_pixels.argtypes = [C.POINTER(AdjointParams), C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_adj_pixels.restype = None
+        L.orc_adj_isco.argtypes = [d, pd]
+        L.orc_adj_isco.restype = d
+        L.orc_adj_disk_term.argtypes = [d] * 8
+        L.orc_adj_disk_term.restype = d
+        L.orc_adj_zamo_init.argtypes = [d, d, d, C.POINTER(C.c_float), pd]
+        L.orc_adj_zamo_init.restype = None
+        L.orc_adj_rhs.argtypes = [pd, d, d, d, pd]
+        L.orc_adj_rhs.restype = None
+        _adj_declared = True
+    return L
+
+
+def adj_pixels(p: AdjointParams, xs, ys, nthreads=0):
+    """FP64 radiance sum and its gradient d/d(spin, theta_obs, q, gain) per
+    pixel, plus the number of crossings counted."""
+    xs = _arr(xs, np.int32)
+    ys = _arr(ys, np.int32)
+    n = len(xs)
+    rad = np.zeros(n)
+    drad = np.zeros((n, 4))
+    nc = np.zeros(n, dtype=np.int32)
+    _alib().orc_adj_pixels(C.byref(p), n, xs.ctypes.data, ys.ctypes.data, rad.ctypes.data,
+                           drad.ctypes.data, nc.ctypes.data, nthreads or (os.cpu_count() or 1))
+    return rad, drad, nc
+
+
+def adj_isco(a):
+    g = C.c_double()
+    v = _alib().orc_adj_isco(a, C.byref(g))
+    return v, g.value
+
+
+def adj_disk_term(r, E, L, a, q, gain, rout, ew):
+    return _alib().orc_adj_disk_term(r, E, L, a, q, gain, rout, ew)
+
+
+def adj_zamo_init(a, th, r_cam, n):
+    nf = _arr(n, np.float32)
+    out = np.zeros(4)
+    _alib().orc_adj_zamo_init(a, th, r_cam, nf.ctypes.data_as(C.POINTER(C.c_float)), _ptr(out))
+    return out
+
+
+def adj_rhs(y, E, L, a):
+    y = _arr(y)
+    out = np.zeros(4)
+    _alib().orc_adj_rhs(_ptr(y), E, L, a, _ptr(out))
+    return out
+
+
 def _arr(x, dtype=np.float64):
     return np.ascontiguousarray(x, dtype=dtype)
 

@@ -19,8 +19,10 @@ def probe(tmp_path_factory):
                     os.path.join(HERE, "support", "carter_probe.cpp"), "-o", str(out)], check=True)
     L = C.CDLL(str(out))
     pd = C.POINTER(C.c_double)
-    L.probe_rhs.argtypes = [pd, C.c_double, C.c_double, C.c_double, pd]
-    L.probe_vjp.argtypes = [pd, C.c_double, C.c_double, C.c_double, pd, pd]
+    for name in ("probe_rhs", "probe_rhs_r"):
+        getattr(L, name).argtypes = [pd, C.c_double, C.c_double, C.c_double, pd]
+    for name in ("probe_vjp", "probe_vjp_r"):
+        getattr(L, name).argtypes = [pd, C.c_double, C.c_double, C.c_double, pd, pd]
     return L
 
 
@@ -28,14 +30,16 @@ def _p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def rhs(probe, y, E, L, a):
+def rhs(probe, y, E, L, a, suffix=""):
     y = np.ascontiguousarray(y, dtype=np.float64)
     o = np.zeros(4)
-    probe.probe_rhs(_p(y), E, L, a, _p(o))
+    getattr(probe, "probe_rhs" + suffix)(_p(y), E, L, a, _p(o))
     return o
 
 
-def test_vjp_matches_finite_differences(probe):
+@pytest.mark.parametrize("suffix", ["", "_r"])
+def test_vjp_matches_finite_differences(probe, suffix):
+    """suffix "" is the affine-parameter field f, "_r" the r-scaled field r f."""
     rng = np.random.default_rng(61)
     worst = 0.0
     for _ in range(300):
@@ -44,18 +48,18 @@ def test_vjp_matches_finite_differences(probe):
         E, L = rng.uniform(0.5, 1.5), rng.normal() * 4
         v = rng.normal(size=4)
         o = np.zeros(7)
-        probe.probe_vjp(_p(y), E, L, a, _p(np.ascontiguousarray(v)), _p(o))
+        getattr(probe, "probe_vjp" + suffix)(_p(y), E, L, a, _p(np.ascontiguousarray(v)), _p(o))
         fd = np.zeros(7)
         for i in range(4):
             h = 1e-6 * max(1.0, abs(y[i]))
             yp, ym = y.copy(), y.copy()
             yp[i] += h
             ym[i] -= h
-            fd[i] = v @ (rhs(probe, yp, E, L, a) - rhs(probe, ym, E, L, a)) / (2 * h)
+            fd[i] = v @ (rhs(probe, yp, E, L, a, suffix) - rhs(probe, ym, E, L, a, suffix)) / (2 * h)
         for j, (dE, dL, da) in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
             h = 1e-6
-            fp = rhs(probe, y, E + h * dE, L + h * dL, a + h * da)
-            fm = rhs(probe, y, E - h * dE, L - h * dL, a - h * da)
+            fp = rhs(probe, y, E + h * dE, L + h * dL, a + h * da, suffix)
+            fm = rhs(probe, y, E - h * dE, L - h * dL, a - h * da, suffix)
             fd[4 + j] = v @ (fp - fm) / (2 * h)
         # error relative to the largest component of each vector (FD round-off
         # of a component near zero is ~1e-10 absolute)

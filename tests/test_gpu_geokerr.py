@@ -83,6 +83,13 @@ def test_geomu_batch_constants_interface(oracle, device):
     o_tpm = np.array([r["tpm"] for r in ref])
     o_muf = np.array([r["muf"] for r in ref])
     assert np.array_equal(g_nc, o_nc)
-    ok = g_tpm == o_tpm
+    # mu0 beyond the lower turning point (mu0 < -mu_plus, about 7% of these
+    # random constants) is unphysical: GEOMU's acos argument leaves [-1, 1],
+    # muf is NaN and tpm = INT(NaN) is undefined, so those rays are compared
+    # on muf = NaN only
+    phys = np.isfinite(o_muf)
+    assert np.array_equal(np.isnan(g_muf), ~phys)
+    ok = (g_tpm == o_tpm) | ~phys
     assert ok.mean() > 0.999
-    assert np.all(np.abs(g_muf[ok] - o_muf[ok]) <= 1e-9)
+    good = ok & phys
+    assert np.all(np.abs(g_muf[good] - o_muf[good]) <= 1e-9)

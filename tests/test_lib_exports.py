@@ -6,7 +6,7 @@ import os
 import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DECL = re.compile(r"^\s*(?:int|double|void|const char\*|GeodesicLookupTable\*)\s+(\w+)\s*\(", re.M)
+DECL = re.compile(r"^\s*(?:int|int64_t|double|void|const char\*|GeodesicLookupTable\*)\s+(\w+)\s*\(", re.M)
 
 
 def declared_symbols():
