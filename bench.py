@@ -9,9 +9,13 @@ to 3 equatorial crossings, float32 RGB output.  The frame constants are
 resident before the timed region and the image stays in HBM.
 
 Each step renders the frame with the FP64 path (reference formulas) and
-with the FP32-compensated path; `value` is the FP32-compensated rays/s and
-the FP64 rays/s, their ratio and the FP32-vs-FP64 error statistics sit
-next to it.  --gpus above 1 is refused: this build measures one GPU.
+with the FP32-compensated path, both with the cubic Hermite crossing radius;
+`value` is the FP32-compensated rays/s.  Beside it: the FP64 rays/s, their
+ratio, the FP32c-vs-FP64 error statistics, the same comparison with the
+reference's linear crossing interpolation (960x540x1), the differentiable
+512x512 render's gradients/s, and the CPU baseline (the FP64 oracle on the
+host cores, a bounded sample of the same frame).  --gpus above 1 is
+refused: this build measures one GPU.
 """
 from __future__ import annotations
 
@@ -25,20 +29,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Kerr rays/s @1080p a=0.9, FP32-comp vs FP64 at ≤1e-5 err; adjoint grads/s"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32c": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 # Algorithmic operation counts per unit (add, sub, mul, div, sqrt, rcp = 1;
-# fma = 2; sin/cos/pow counted apart), derived line by line from the kernel
-# source in DESIGN.md:
-#   fp64: the reference right-hand side (bh_kerr.h:600-727) is 200 flops;
-#         one RK4 stage combination with compensated sums + update is 196;
-#         error norm and step factor of one attempt is 29.
-#   fp32c: the Carter-form right-hand side is 55 flops; stage input plus
-#          stage-sum accumulation is 18 per stage (10 per attempt); each of
-#          the 3 float-float updates is 56; error norm and factor is 40.
+# fma = 2; library sin/cos/pow counted apart), derived from the kernel source:
+#   fp64 path: the reference right-hand side (bh_kerr.h:600-727) is 200
+#     flops; per attempt three RK4 combinations with compensated sums and
+#     updates (3 x 196) plus the error norm and step factor (29).
+#   fp32c float steps: the Carter-form right-hand side with the split-theta
+#     sin/cos correction is 70 flops; per attempt 10 stage inputs and stage
+#     sums (10 x 18), 3 float-float updates (3 x 56) and the noise-floored
+#     error norm and step factor (85).
+#   fp32c FP64 steps: the FP64 Carter right-hand side with its polynomial
+#     sincos is 105 flops; per attempt 12 stage inputs and sums (12 x 18),
+#     3 updates (3 x 10) and the error norm (40).
 FLOPS = {
     "fp64": {"rhs": 200, "attempt": 3 * 196 + 29},
-    "fp32c": {"rhs": 55, "attempt": 10 * 18 + 3 * 56 + 40},
+    "fp32c": {"rhs": 70, "attempt": 10 * 18 + 3 * 56 + 85,
+              "rhs64": 105, "attempt64": 12 * 18 + 3 * 10 + 40},
 }
 
 
@@ -54,10 +62,119 @@ def parse():
                     help="pixels (x spp rays) in the CPU-baseline sample; 0 skips it")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--precisions", default="fp64,fp32c")
+    ap.add_argument("--crossing", default="hermite", choices=["hermite", "linear"])
+    ap.add_argument("--linear-check", default="960x540x1",
+                    help="WxHxSPP of the linear-crossing comparison; 'none' skips it")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="also time the differentiable 512x512 render (forward + adjoint)")
     ap.add_argument("--adjoint-size", type=int, default=512)
     return ap.parse_args()
+
+
+def render_case(prec: str, crossing: str, width: int, height: int, spp: int, steps: int,
+                warmup: int) -> dict:
+    import torch
+
+    import nullpath_amd as npa
+    from nullpath_amd._lib import ray_record_dtype
+
+    dev = torch.device("cuda:0")
+    pid = 1 if prec == "fp32c" else 0
+    cr = 1 if crossing == "hermite" else 0
+    npix = width * height
+    rays = npix * spp
+    p = npa.kerr_disk_params(width=width, height=height, spin=0.9, theta_deg=80.0, fov=55.0,
+                             rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid, crossing=cr)
+    out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
+    cnt = torch.zeros(10, dtype=torch.int64, device=dev)
+    rec = torch.zeros(rays * 32, dtype=torch.uint8, device=dev)
+    npa.render_tile(npa.kerr_disk_params(width=64, height=36, seed=1, precision=pid, crossing=cr),
+                    out, spp=1)
+    for _ in range(warmup):
+        npa.render_tile(p, out, spp=spp)
+    torch.cuda.synchronize()
+    # events on the stream the kernels are launched on (torch's current stream)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        cnt.zero_()
+        ev[k][0].record(stream)
+        npa.render_tile(p, out, spp=spp, counters=cnt, records=rec, stream=stream)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"[bench] {prec}/{crossing} {width}x{height}x{spp}: {steps} step(s) in {wall:.2f} s",
+          file=sys.stderr, flush=True)
+    c = npa.counters_dict(cnt)
+    assert c["rays"] == rays, c
+    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    mean_s = sum(kern_ms) / len(kern_ms) / 1e3
+    fl = FLOPS[prec]
+    if prec == "fp32c":
+        f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + \
+              (c["attempts"] - c["attempts_f64"]) * fl["attempt"]
+        f64 = c["rhs_evals_f64"] * fl["rhs64"] + c["attempts_f64"] * fl["attempt64"]
+    else:
+        f32 = 0
+        f64 = c["rhs_evals"] * fl["rhs"] + c["attempts"] * fl["attempt"]
+    # fraction of the VALU peak: FP32 work at the FP32 rate, FP64 at the FP64 rate
+    valu_s = f32 / (PEAK_TFLOPS["fp32"] * 1e12) + f64 / (PEAK_TFLOPS["fp64"] * 1e12)
+    return {
+        "prec": prec, "crossing": crossing, "wall": wall, "kern_ms": kern_ms, "mean_s": mean_s,
+        "counters": c, "flops32": f32, "flops64": f64, "frac": valu_s / mean_s,
+        "rays": rays * steps, "npix": npix, "spp": spp,
+        "img": out.view(height, width, 3).cpu().numpy(),
+        "rec": rec.cpu().numpy().view(ray_record_dtype()), "params": p,
+    }
+
+
+def case_summary(r: dict) -> dict:
+    cc = r["counters"]
+    return {
+        "rays_per_s": r["rays"] / r["wall"],
+        "kernel_ms": r["kern_ms"],
+        "accepted_steps_per_ray": cc["steps"] / cc["rays"],
+        "attempts_per_ray": cc["attempts"] / cc["rays"],
+        "fp64_attempt_frac": cc["attempts_f64"] / max(1, cc["attempts"]),
+        "rk4_steps_per_s": 3 * cc["attempts"] / r["mean_s"],
+        "rhs_evals_per_s": cc["rhs_evals"] / r["mean_s"],
+        "tflops_fp32": r["flops32"] / r["mean_s"] / 1e12,
+        "tflops_fp64": r["flops64"] / r["mean_s"] / 1e12,
+        "valu_peak_frac": r["frac"],
+        "captured": cc["captured"], "escaped": cc["escaped"], "stepcap": cc["stepcap"],
+        "disk_rays": cc["disk_hits"],
+    }
+
+
+def compare(a: dict, b: dict) -> dict:
+    """FP32c (b) against FP64 (a) on the same frame: per-ray class and
+    crossing count, per-pixel disk radiance relative error, sky pixels."""
+    import numpy as np
+    npix, spp = a["npix"], a["spp"]
+    ra = a["rec"].reshape(npix, spp)
+    rb = b["rec"].reshape(npix, spp)
+    same_ray = (ra["cls"] == rb["cls"]) & (ra["orders"] == rb["orders"])
+    same_pix = same_ray.all(axis=1)
+    ia = a["img"].reshape(-1, 3).astype(np.float64)
+    ib = b["img"].reshape(-1, 3).astype(np.float64)
+    disk = (ra["orders"] > 0).any(axis=1) & same_pix
+    rel = np.abs(ib - ia) / np.maximum(np.abs(ia), 1e-30)
+    rel_disk = rel[disk].max(axis=1) if disk.any() else np.zeros(1)
+    sky = same_pix & ~(ra["orders"] > 0).any(axis=1)
+    return {
+        "ray_class_mismatch": int((~same_ray).sum()),
+        "ray_class_mismatch_frac": float((~same_ray).mean()),
+        "mismatch_min_r_fp64_max": float(ra["min_r"][~same_ray].max()) if (~same_ray).any() else None,
+        "disk_pixels_compared": int(disk.sum()),
+        "disk_rel_err_max": float(rel_disk.max()),
+        "disk_rel_err_p999": float(np.quantile(rel_disk, 0.999)),
+        "disk_rel_err_p99": float(np.quantile(rel_disk, 0.99)),
+        "disk_rel_err_median": float(np.median(rel_disk)),
+        "disk_frac_within_1e-5": float(np.mean(rel_disk <= 1e-5)),
+        "sky_pixels_max_abs_diff": float(np.abs(ib[sky] - ia[sky]).max()) if sky.any() else 0.0,
+    }
 
 
 def bench_adjoint(args) -> dict:
@@ -114,65 +231,57 @@ def bench_adjoint(args) -> dict:
     }
 
 
+def cpu_baseline(args, fp64: dict) -> dict:
+    """The FP64 oracle (a plain-C restatement of the reference integrator)
+    on a bounded random sample of the same frame, on the host cores."""
+    import numpy as np
+
+    import nullpath_amd as npa
+    from oracle import np_oracle as O
+    p = fp64["params"]
+    q = O.default_params()
+    for name, _ in npa.RenderParams._fields_:
+        setattr(q, name, getattr(p, name))
+    npix = fp64["npix"]
+    rng = np.random.default_rng(12345)
+    sel = rng.choice(npix, size=min(args.cpu_pixels, npix), replace=False)
+    xs = (sel % args.width).astype(np.int32)
+    ys = (sel // args.width).astype(np.int32)
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = args.cpu_threads or min(16, ncpu)
+    t1 = time.perf_counter()
+    o_rgb, _ = O.render_pixels(q, xs, ys, 0, args.spp, nthreads=threads, with_info=False)
+    cpu_s = time.perf_counter() - t1
+    g_rgb = fp64["img"].reshape(-1, 3)[sel]
+    rel = np.abs(g_rgb - o_rgb) / np.maximum(np.abs(o_rgb), 1e-6)
+    return {
+        "value": len(sel) * args.spp / cpu_s,
+        "unit": "rays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{len(sel)} seeded random pixels x {args.spp} spp of the same frame "
+                   f"({len(sel) * args.spp} rays, {cpu_s:.1f} s), FP64 oracle restatement, OpenMP"),
+        "gpu_fp64_vs_oracle_max_rel_err_on_sample": float(rel.max()),
+    }
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build",
               file=sys.stderr)
         return 2
-    import numpy as np
-    import torch
-
-    import nullpath_amd as npa
-    from nullpath_amd._lib import ray_record_dtype
-
-    dev = torch.device("cuda:0")
     precs = args.precisions.split(",")
-    npix = args.width * args.height
-    rays_per_step = npix * args.spp
-    stream = torch.cuda.current_stream()
-    res = {}
-    for prec in precs:
-        pid = 1 if prec == "fp32c" else 0
-        p = npa.kerr_disk_params(width=args.width, height=args.height, spin=0.9, theta_deg=80.0,
-                                 fov=55.0, rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid)
-        out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
-        cnt = torch.zeros(8, dtype=torch.int64, device=dev)
-        rec = torch.zeros(rays_per_step * 32, dtype=torch.uint8, device=dev)
-        warm = npa.kerr_disk_params(width=64, height=36, seed=1, precision=pid)
-        npa.render_tile(warm, out, spp=1)
-        for _ in range(args.warmup):
-            npa.render_tile(p, out, spp=args.spp)
-        torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            cnt.zero_()
-            ev[k][0].record(stream)
-            npa.render_tile(p, out, spp=args.spp, counters=cnt, records=rec)
-            ev[k][1].record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        print(f"[bench] {prec}: {args.steps} step(s) in {wall:.2f} s", file=sys.stderr, flush=True)
-        c = npa.counters_dict(cnt)
-        assert c["rays"] == rays_per_step, c
-        kern_ms = [a.elapsed_time(b) for a, b in ev]
-        mean_s = sum(kern_ms) / len(kern_ms) / 1e3
-        fl = FLOPS[prec]
-        flops = c["rhs_evals"] * fl["rhs"] + c["attempts"] * fl["attempt"]
-        res[prec] = {
-            "wall": wall, "kern_ms": kern_ms, "mean_s": mean_s, "counters": c, "flops": flops,
-            "achieved": flops / mean_s / 1e12,
-            "img": out.view(args.height, args.width, 3).cpu().numpy(),
-            "rec": rec.cpu().numpy().view(ray_record_dtype()),
-            "params": p,
-        }
-
+    res = {prec: render_case(prec, args.crossing, args.width, args.height, args.spp, args.steps,
+                             args.warmup) for prec in precs}
     head = "fp32c" if "fp32c" in res else precs[0]
     R = res[head]
-    c = R["counters"]
-    value = rays_per_step * args.steps / R["wall"]
+    value = R["rays"] / R["wall"]
+    kernel = "kerr_render_f32c_w3<adaptive>" if head == "fp32c" else "kerr_render<StepF64, adaptive>"
+    achieved = (R["flops32"] + 2.0 * R["flops64"]) / R["mean_s"] / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -189,94 +298,43 @@ def main() -> int:
         "config": {
             "workload": (f"Kerr thin-disk still {args.width}x{args.height}x{args.spp}spp a=0.9 "
                          f"theta=80deg fov=55deg r_cam=1000M adaptive rtol=1e-8 disk ISCO-20M q=3 "
-                         f"<=3 crossings"),
+                         f"<=3 crossings, {args.crossing} crossing radius"),
             "precision": head,
-            "rays_per_step": rays_per_step,
+            "rays_per_step": R["rays"] // args.steps,
         },
         "roofline": {
             "bound": "valu",
-            "achieved": R["achieved"],
-            "peak": PEAK_TFLOPS[head],
+            "achieved": achieved,
+            "peak": PEAK_TFLOPS["fp32"],
             "unit": "TFLOP/s",
-            "frac": R["achieved"] / PEAK_TFLOPS[head],
+            "frac": achieved / PEAK_TFLOPS["fp32"],
             "traffic": None,
-            "kernel": f"kerr_render<{'StepF32C' if head == 'fp32c' else 'StepF64'}, adaptive>",
-            "flops_per_launch": R["flops"],
+            "kernel": kernel,
+            "flops_per_launch": {"fp32": R["flops32"] / args.steps, "fp64": R["flops64"] / args.steps},
+            "note": "FP64 flops count twice (FP32-equivalent) since the FP64 rate is half",
         },
         "cpu_baseline": None,
     }
     for prec, r in res.items():
-        cc = r["counters"]
-        line["config"][prec] = {
-            "rays_per_s": rays_per_step * args.steps / r["wall"],
-            "kernel_ms": r["kern_ms"],
-            "accepted_steps_per_ray": cc["steps"] / cc["rays"],
-            "attempts_per_ray": cc["attempts"] / cc["rays"],
-            "rk4_steps_per_s": 3 * cc["attempts"] / r["mean_s"],
-            "rhs_evals_per_s": cc["rhs_evals"] / r["mean_s"],
-            "achieved_tflops": r["achieved"],
-            "frac_of_peak": r["achieved"] / PEAK_TFLOPS[prec],
-            "captured": cc["captured"], "escaped": cc["escaped"], "stepcap": cc["stepcap"],
-            "disk_rays": cc["disk_hits"],
-        }
+        line["config"][prec] = case_summary(r)
     if "fp64" in res and "fp32c" in res:
         a, b = res["fp64"], res["fp32c"]
         line["config"]["speedup_fp32c_over_fp64"] = a["wall"] / b["wall"]
-        ra = a["rec"].reshape(npix, args.spp)
-        rb = b["rec"].reshape(npix, args.spp)
-        same_ray = (ra["cls"] == rb["cls"]) & (ra["orders"] == rb["orders"])
-        same_pix = same_ray.all(axis=1)
-        ia = a["img"].reshape(-1, 3)
-        ib = b["img"].reshape(-1, 3)
-        disk = (ra["orders"] > 0).any(axis=1) & same_pix
-        rel = np.abs(ib - ia) / np.maximum(np.abs(ia), 1e-30)
-        rel_disk = rel[disk].max(axis=1) if disk.any() else np.zeros(1)
-        sky = same_pix & ~(ra["orders"] > 0).any(axis=1)
-        line["config"]["fp32c_vs_fp64"] = {
-            "ray_class_mismatch": int((~same_ray).sum()),
-            "ray_class_mismatch_frac": float((~same_ray).mean()),
-            "mismatch_min_r_fp64_max": float(ra["min_r"][~same_ray].max()) if (~same_ray).any() else None,
-            "disk_pixels_compared": int(disk.sum()),
-            "disk_rel_err_max": float(rel_disk.max()),
-            "disk_rel_err_p999": float(np.quantile(rel_disk, 0.999)),
-            "disk_rel_err_p99": float(np.quantile(rel_disk, 0.99)),
-            "disk_rel_err_median": float(np.median(rel_disk)),
-            "disk_frac_within_1e-5": float(np.mean(rel_disk <= 1e-5)),
-            "sky_pixels_max_abs_diff": float(np.abs(ib[sky] - ia[sky]).max()) if sky.any() else 0.0,
-        }
+        line["config"]["fp32c_vs_fp64"] = compare(a, b)
+        if args.linear_check != "none":
+            w, h, s = (int(v) for v in args.linear_check.split("x"))
+            la = render_case("fp64", "linear", w, h, s, 1, 0)
+            lb = render_case("fp32c", "linear", w, h, s, 1, 0)
+            lin = compare(la, lb)
+            lin["size"] = args.linear_check
+            lin["speedup"] = la["wall"] / lb["wall"]
+            line["config"]["fp32c_vs_fp64_linear_crossing"] = lin
 
     if args.adjoint:
         line["config"]["adjoint"] = bench_adjoint(args)
 
     if args.cpu_pixels > 0 and "fp64" in res:
-        from oracle import np_oracle as O
-        p = res["fp64"]["params"]
-        q = O.default_params()
-        for name, _ in npa.RenderParams._fields_:
-            setattr(q, name, getattr(p, name))
-        rng = np.random.default_rng(12345)
-        sel = rng.choice(npix, size=min(args.cpu_pixels, npix), replace=False)
-        xs = (sel % args.width).astype(np.int32)
-        ys = (sel // args.width).astype(np.int32)
-        try:
-            ncpu = len(os.sched_getaffinity(0))
-        except AttributeError:
-            ncpu = os.cpu_count() or 1
-        threads = args.cpu_threads or min(16, ncpu)
-        t1 = time.perf_counter()
-        o_rgb, _ = O.render_pixels(q, xs, ys, 0, args.spp, nthreads=threads, with_info=False)
-        cpu_s = time.perf_counter() - t1
-        g_rgb = res["fp64"]["img"].reshape(-1, 3)[sel]
-        rel = np.abs(g_rgb - o_rgb) / np.maximum(np.abs(o_rgb), 1e-6)
-        line["cpu_baseline"] = {
-            "value": len(sel) * args.spp / cpu_s,
-            "unit": "rays/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": (f"{len(sel)} seeded random pixels x {args.spp} spp of the same frame "
-                       f"({len(sel) * args.spp} rays, {cpu_s:.1f} s), FP64 oracle restatement, OpenMP"),
-            "gpu_fp64_vs_oracle_max_rel_err_on_sample": float(rel.max()),
-        }
+        line["cpu_baseline"] = cpu_baseline(args, res["fp64"])
     print(json.dumps(line))
     return 0
 

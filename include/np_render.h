@@ -27,6 +27,13 @@ enum np_integrator { NP_INTEGRATOR_FIXED = 0, NP_INTEGRATOR_ADAPTIVE = 1 };
 enum np_camera     { NP_CAMERA_STATIC = 0, NP_CAMERA_ZAMO = 1 };
 enum np_bg         { NP_BG_GRADIENT = 0, NP_BG_POINT = 1 };
 enum np_precision  { NP_PREC_FP64 = 0, NP_PREC_FP32C = 1 };
+/* Radius of an equatorial crossing between two accepted states: LINEAR is
+ * the reference's interpolation in theta (render.cu:211-226), whose error
+ * (~h^2 r'') depends on where the adaptive steps fall; HERMITE solves the
+ * cubic Hermite interpolant of theta(lambda) through both states and their
+ * derivatives, so the radius no longer depends on the step sequence at the
+ * ~1e-4 level. */
+enum np_crossing   { NP_CROSSING_LINEAR = 0, NP_CROSSING_HERMITE = 1 };
 
 /* Per-ray termination class written by the tracers (same values as
  * RayStatus, bh_common.h:295-331, plus the renderer's own outcomes). */
@@ -65,6 +72,13 @@ typedef struct np_render_params {
     double  disk_rout;        /* disk outer edge, absolute units (reference: 20 Rs) */
     float   emiss_q;          /* emissivity index q (reference: 2) */
     uint32_t seed;            /* jitter seed; 0 gives the reference's hash keys */
+    double  fp64_radius;      /* NP_PREC_FP32C only: steps that start inside this radius (absolute
+                                 units) run in FP64; 0 = auto, max(30 M, 1.5 disk_rout) */
+    int32_t waves_per_simd;   /* NP_PREC_FP32C kernel variant: 0 or 3 = capped at 168 VGPRs
+                                 (3 waves per SIMD; the rare FP64 branch spills), 2 = the
+                                 compiler's own register budget (2 waves per SIMD) */
+    int32_t crossing;         /* np_crossing: disk-crossing radius between accepted steps
+                                 (reference: linear in theta) */
 } np_render_params;
 
 /* Fill *p with the reference renderer's defaults (render.cu:15-46 with
@@ -79,7 +93,9 @@ static inline void np_render_params_default(np_render_params* p) {
     p->check_inv = 0; p->max_steps = 10000; p->precision = NP_PREC_FP64;
     p->emiss_m_units = 0; p->rtol = 1e-10; p->atol = 1e-12;
     p->disk_rin = 3.0 * 2.0 * (double)p->mass; p->disk_rout = 20.0 * 2.0 * (double)p->mass;
-    p->emiss_q = 2.0f; p->seed = 0u;
+    p->emiss_q = 2.0f; p->seed = 0u; p->fp64_radius = 0.0;
+    p->waves_per_simd = 0;
+    p->crossing = NP_CROSSING_LINEAR;
 }
 
 /* Kerr prograde ISCO radius in units of M (Bardeen, Press & Teukolsky 1972,

@@ -31,7 +31,9 @@ const char* np_status_string(int code);
 /* Counters a tracer launch adds to (device memory, caller-zeroed, may be
  * NULL).  Steps are accepted integration steps; attempts include rejected
  * adaptive attempts; rhs_evals counts Hamilton right-hand-side evaluations
- * actually performed. */
+ * actually performed.  attempts_f64 and rhs_evals_f64 count the part of
+ * attempts and rhs_evals that an NP_PREC_FP32C launch ran in FP64 (inside
+ * fp64_radius, or a float attempt's FP64 re-run); they stay 0 for FP64. */
 typedef struct np_trace_counters {
     uint64_t rays;
     uint64_t steps;
@@ -41,6 +43,8 @@ typedef struct np_trace_counters {
     uint64_t escaped;
     uint64_t stepcap;
     uint64_t disk_hits;
+    uint64_t attempts_f64;
+    uint64_t rhs_evals_f64;
 } np_trace_counters;
 
 /* Optional per-sample diagnostic record (extension; the reference's kernel

@@ -73,6 +73,8 @@ def _stream_ptr(stream) -> int | None:
     if stream is None:
         import torch
         return torch.cuda.current_stream().cuda_stream
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
     return int(stream)
 
 
@@ -92,7 +94,7 @@ def render_tile(p: RenderParams, out, y0=0, rows=None, spp=1, sample_offset=0, i
 
 
 def counters_dict(t) -> dict:
-    """Decode a uint64 counter tensor (8 elements) into a dict."""
+    """Decode a uint64 counter tensor (np_trace_counters, 10 elements) into a dict."""
     vals = t.cpu().tolist()
     return {k: int(v) & ((1 << 64) - 1) for k, v in zip(COUNTER_FIELDS, vals)}
 

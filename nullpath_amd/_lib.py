@@ -31,6 +31,8 @@ class RenderParams(C.Structure):
         ("max_steps", C.c_int32), ("precision", C.c_int32), ("emiss_m_units", C.c_int32),
         ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
+        ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
+        ("crossing", C.c_int32),
     ]
 
     def copy(self) -> "RenderParams":
@@ -61,7 +63,8 @@ class TraceCounters(C.Structure):
     """np_trace_counters (include/nullpath.h)."""
 
     _fields_ = [(n, C.c_uint64) for n in (
-        "rays", "steps", "attempts", "rhs_evals", "captured", "escaped", "stepcap", "disk_hits")]
+        "rays", "steps", "attempts", "rhs_evals", "captured", "escaped", "stepcap", "disk_hits",
+        "attempts_f64", "rhs_evals_f64")]
 
 
 COUNTER_FIELDS = [f[0] for f in TraceCounters._fields_]

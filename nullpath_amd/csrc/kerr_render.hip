@@ -359,11 +359,38 @@ __device__ __forceinline__ void base_start(const FrameConst& F, RayBase& R, floa
     R.attempts_total = 0;
 }
 
+// Crossing radius from the cubic Hermite interpolants of theta(lambda) and
+// r(lambda) over the accepted step [0, h], given the end values and end
+// derivatives: Newton on theta(s) = pi/2 in s = lambda/h from the linear
+// estimate s0, kept inside [0, 1]; returns the linear radius when the
+// iteration leaves the interval or stalls.
+__device__ __forceinline__ double hermite_crossing(double h, double r0, double th0, double dr0,
+                                                   double dth0, double r1, double th1, double dr1,
+                                                   double dth1, double s0, double r_lin) {
+    const double m0 = h * dth0, m1 = h * dth1, n0 = h * dr0, n1 = h * dr1;
+    const double g0 = th0 - NP_PI_D * 0.5, g1 = th1 - NP_PI_D * 0.5;
+    double s = s0;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const double s2 = s * s, s3 = s2 * s;
+        const double f = (2 * s3 - 3 * s2 + 1) * g0 + (s3 - 2 * s2 + s) * m0 + (3 * s2 - 2 * s3) * g1 +
+                         (s3 - s2) * m1;
+        const double fp = (6 * s2 - 6 * s) * g0 + (3 * s2 - 4 * s + 1) * m0 + (6 * s - 6 * s2) * g1 +
+                          (3 * s2 - 2 * s) * m1;
+        if (!(fabs(fp) > 0.0)) return r_lin;
+        s -= f / fp;
+        if (!(s >= 0.0 && s <= 1.0)) return r_lin;
+    }
+    const double s2 = s * s, s3 = s2 * s;
+    return (2 * s3 - 3 * s2 + 1) * r0 + (s3 - 2 * s2 + s) * n0 + (3 * s2 - 2 * s3) * r1 + (s3 - s2) * n1;
+}
+
 // Accepted-step bookkeeping of render.cu:210-250: closest approach,
-// equatorial crossing with linear interpolation of r, disk shading,
-// escape and step budget.  Returns -1 while the ray keeps going.
-template <bool INV>
-__device__ __forceinline__ int after_accept(const FrameConst& F, RayBase& R, double r, double th,
+// equatorial crossing with linear interpolation of r (or, with
+// F.crossing = 1, the cubic Hermite refinement above), disk shading, escape
+// and step budget.  Returns -1 while the ray keeps going.
+template <class S, bool INV>
+__device__ __forceinline__ int after_accept(const FrameConst& F, typename S::Ray& R, double r, double th,
                                             double pr, double pth) {
     R.min_r = fmin(R.min_r, r);
     const double d_prev = R.prev_th - NP_PI_D * 0.5;
@@ -375,7 +402,12 @@ __device__ __forceinline__ int after_accept(const FrameConst& F, RayBase& R, dou
     if ((sign_change || near_zero) && R.orders < F.disk_max_orders) {
         const double denom = fabs(d_prev) + fabs(d_curr) + 1e-30;
         const double alpha = fabs(d_prev) / denom;
-        const double r_cross = R.prev_r + alpha * (r - R.prev_r);
+        double r_cross = R.prev_r + alpha * (r - R.prev_r);
+        if (F.crossing == 1 && sign_change) {
+            double h, dr0, dth0, dr1, dth1;
+            S::cross_derivs(F, R, h, dr0, dth0, dr1, dth1);
+            r_cross = hermite_crossing(h, R.prev_r, R.prev_th, dr0, dth0, r, th, dr1, dth1, alpha, r_cross);
+        }
         if (r_cross >= F.rin && r_cross <= F.rout) {
             // p_t and p_phi are constant, so their interpolants equal them
             const float g = (float)doppler_g(F, r_cross, R.pt, R.pph);
@@ -469,17 +501,36 @@ struct StepF64 {
     __device__ static double pr(const Ray& R) { return R.y.pr; }
     __device__ static double pth(const Ray& R) { return R.y.pth; }
 
+    // Step size and (r, theta) derivatives at both ends of the step just
+    // accepted, for the Hermite crossing: R.k1 still holds f at the step's
+    // start; f at its end is evaluated here and kept as the next k1.
+    __device__ static void cross_derivs(const FrameConst& F, Ray& R, double& h, double& dr0,
+                                        double& dth0, double& dr1, double& dth1) {
+        h = R.h;
+        dr0 = R.k1.r;
+        dth0 = R.k1.th;
+        R.k1 = kerr_rhs(F.M, F.a, F.r_plus, R.y.r, R.y.th, R.pt, R.y.pr, R.y.pth, R.pph);
+        R.k1_valid = true;
+        dr1 = R.k1.r;
+        dth1 = R.k1.th;
+    }
+
     // One integrator attempt.  Returns 1 accepted, 0 rejected (retry),
     // 2 failed (the ray is captured: horizon, pole or rejection budget).
     template <bool ADAPTIVE>
-    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count) {
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& /*rhs64*/) {
         const double Rs = F.Rs_d;
         if (!ADAPTIVE) {
             // integrateGeodesicKerrD (bh_kerr.h:734-774), h = 0.01 * scale
             const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
             const double h = (double)STEP0 * scale;
-            const Deriv k1 = kerr_rhs(F.M, F.a, F.r_plus, R.y.r, R.y.th, R.pt, R.y.pr, R.y.pth, R.pph);
-            rhs_count += 1;
+            const Deriv k1 = (R.k1_valid) ? R.k1
+                                          : kerr_rhs(F.M, F.a, F.r_plus, R.y.r, R.y.th, R.pt, R.y.pr,
+                                                     R.y.pth, R.pph);
+            rhs_count += R.k1_valid ? 0 : 1;
+            R.k1 = k1;
+            R.k1_valid = false;
+            R.h = h;
             R.y = rk4_from_k1(F, R.y, k1, R.pt, R.pph, h, rhs_count);
             const bool ok = R.y.r > F.r_plus + 1e-9 && R.y.th > 1e-9 && R.y.th < NP_PI_D - 1e-9;
             return ok ? 1 : 2;
@@ -604,15 +655,24 @@ struct FF5 {
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-__device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, float th, float pr,
-                                               float pth, float E, float L) {
+// theta enters as th + thd with th a float and thd a small float offset (the
+// low word of the float-float state plus the stage increment): sin and cos
+// of th are corrected by the rotation through thd (Taylor to thd^4).  With
+// theta rounded to one float instead, cos(theta) near the equator carries a
+// relative error of tan(theta) * 6e-8, which is what limits dp_theta.
+__device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, float th, float thd,
+                                               float pr, float pth, float E, float L) {
     DerivF d;
     if (r <= F.rpf_guard) {
         d.r = d.th = d.ph = d.pr = d.pth = 0.0f;
         return d;
     }
-    float s, c;
-    sincosf(th, &s, &c);
+    float s0, c0;
+    sincosf(th, &s0, &c0);
+    const float d2 = thd * thd;
+    const float cd = 1.0f - 0.5f * d2 * (1.0f - d2 * (1.0f / 12.0f));
+    const float sd = thd * (1.0f - d2 * (1.0f / 6.0f));
+    const float s = s0 * cd + c0 * sd, c = c0 * cd - s0 * sd;
     const float M = F.Mf, a = F.af, a2 = F.a2f;
     const float r2 = r * r;
     const float s2 = fmaxf(s * s, 1e-12f);
@@ -633,15 +693,139 @@ __device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, flo
     return d;
 }
 
+// The same Carter-form field in FP64 (the FP64 steps of the FP32C path:
+// inside the FP64 radius and re-runs of float attempts).  Delta = r^2 -
+// 2Mr + a^2 as in the reference's FP64 metric; one division for the three
+// reciprocals; sincos_d instead of the library's large-argument sincos.
+__device__ __forceinline__ Deriv kerr_rhs_c64(const FrameConst& F, double r, double th, double pr,
+                                              double pth, double E, double L) {
+    Deriv d;
+    if (r <= F.r_plus + 1e-12) {
+        d.r = d.th = d.ph = d.pr = d.pth = 0.0;
+        return d;
+    }
+    double s, c;
+    sincos_d(th, s, c);
+    const double M = F.M, a = F.a, a2 = a * a, r2 = r * r;
+    const double s2 = fmax(s * s, 1e-12);
+    const double Sigma = r2 + a2 * (c * c);
+    const double Delta = r2 - 2.0 * M * r + a2;
+    const double SD = Sigma * Delta;
+    const double inv = 1.0 / (SD * s2);
+    const double iS = Delta * s2 * inv, iD = Sigma * s2 * inv, is2 = SD * inv;
+    const double P = (r2 + a2) * E - a * L;
+    const double w = L * is2 - a * E;
+    const double Q = pth * pth + s2 * (w * w);
+    const double X = 2.0 * E * M * (r2 - a2) - 2.0 * a * L * (r - M);
+    d.r = Delta * pr * iS;
+    d.th = pth * iS;
+    d.ph = (a * (2.0 * M * r * E - a * L) * iD + L * is2) * iS;
+    d.pr = ((r - M) * Q * iD - P * X * (iD * iD)) * iS;
+    d.pth = c * s * w * (L * is2 + a * E) * iS;
+    return d;
+}
+
+// FP64 image of a float-float state.
+__device__ __forceinline__ Y5 ff5_to_y5(const FF5& y) {
+    return Y5{ff_to_d(y.r), ff_to_d(y.th), ff_to_d(y.ph), ff_to_d(y.pr), ff_to_d(y.pth)};
+}
+
+__device__ __forceinline__ bool in_domain_d(const FrameConst& F, const Y5& y) {
+    return y.r > F.r_plus + 1e-12 && y.th > 1e-9 && y.th < NP_PI_D - 1e-9;
+}
+
+// Step doubling with kerr_rhs_c64: one RK4 step of h and two of h/2 from y,
+// plain FP64 stage sums, as three RK4 runs of four stages in one
+// non-unrolled loop.  This path is rare in a wave, so its live set is kept
+// small: k1 = f(y) is evaluated twice (runs 0 and 1) instead of being held,
+// the full step waits in LDS (full[c * blockDim.x], c = r, th, ph, pr, pth)
+// and the intermediate half step is only checked against the domain.
+// Returns true when all three results are inside the domain; k1r, k1th are
+// the r and theta components of f(y).
+__device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5& y, const double E,
+                                                  const double L, const double h, double* full,
+                                                  Y5& half2, double& k1r, double& k1th) {
+    Deriv S, kp;
+    Y5 base = ff5_to_y5(y);
+    S.r = S.th = S.ph = S.pr = S.pth = 0.0;
+    kp = S;
+    bool ok = true;
+    const int stride = blockDim.x;
+#pragma clang loop unroll(disable)
+    for (int j = 0; j < 12; ++j) {
+        const int st = j & 3, run = j >> 2;
+        const double hr = (run == 0) ? h : 0.5 * h;
+        const double c = (st == 0) ? 0.0 : (st == 3) ? hr : 0.5 * hr;
+        const Deriv k = kerr_rhs_c64(F, base.r + c * kp.r, base.th + c * kp.th, base.pr + c * kp.pr,
+                                     base.pth + c * kp.pth, E, L);
+        const double w = (st == 0 || st == 3) ? 1.0 : 2.0;
+        if (j == 0) {
+            k1r = k.r;
+            k1th = k.th;
+        }
+        if (st == 0) {
+            S = k;
+        } else {
+            S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
+        }
+        kp = k;
+        if (st == 3) {
+            const double h6 = hr / 6.0;
+            Y5 o;
+            o.r = base.r + h6 * S.r;
+            o.th = base.th + h6 * S.th;
+            o.ph = base.ph + h6 * S.ph;
+            o.pr = base.pr + h6 * S.pr;
+            o.pth = base.pth + h6 * S.pth;
+            ok = ok && in_domain_d(F, o);
+            if (run == 0) {
+                full[0 * stride] = o.r;
+                full[1 * stride] = o.th;
+                full[2 * stride] = o.ph;
+                full[3 * stride] = o.pr;
+                full[4 * stride] = o.pth;
+                base = ff5_to_y5(y);
+            } else if (run == 1) {
+                base = o;
+            } else {
+                half2 = o;
+            }
+        }
+    }
+    return ok;
+}
+
+__device__ __forceinline__ Y5 rk4_c64(const FrameConst& F, const Y5& y, const double E, const double L,
+                                      const double h, double& k1r, double& k1th) {
+    const double hh = 0.5 * h;
+    const Deriv k1 = kerr_rhs_c64(F, y.r, y.th, y.pr, y.pth, E, L);
+    k1r = k1.r;
+    k1th = k1.th;
+    const Deriv k2 = kerr_rhs_c64(F, y.r + hh * k1.r, y.th + hh * k1.th, y.pr + hh * k1.pr,
+                                  y.pth + hh * k1.pth, E, L);
+    const Deriv k3 = kerr_rhs_c64(F, y.r + hh * k2.r, y.th + hh * k2.th, y.pr + hh * k2.pr,
+                                  y.pth + hh * k2.pth, E, L);
+    const Deriv k4 = kerr_rhs_c64(F, y.r + h * k3.r, y.th + h * k3.th, y.pr + h * k3.pr,
+                                  y.pth + h * k3.pth, E, L);
+    const double h6 = h / 6.0;
+    Y5 o;
+    o.r = y.r + h6 * (k1.r + 2.0 * (k2.r + k3.r) + k4.r);
+    o.th = y.th + h6 * (k1.th + 2.0 * (k2.th + k3.th) + k4.th);
+    o.ph = y.ph + h6 * (k1.ph + 2.0 * (k2.ph + k3.ph) + k4.ph);
+    o.pr = y.pr + h6 * (k1.pr + 2.0 * (k2.pr + k3.pr) + k4.pr);
+    o.pth = y.pth + h6 * (k1.pth + 2.0 * (k2.pth + k3.pth) + k4.pth);
+    return o;
+}
+
 __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const DerivF& k1, float E,
                                        float L, float h, int& rhs_count) {
     const float hh = 0.5f * h;
-    const float yr = ff_f(y.r), yth = ff_f(y.th), ypr = ff_f(y.pr), ypth = ff_f(y.pth);
-    const DerivF k2 = kerr_rhs_f32(F, yr + hh * k1.r, yth + hh * k1.th, ypr + hh * k1.pr,
+    const float yr = ff_f(y.r), ypr = ff_f(y.pr), ypth = ff_f(y.pth);
+    const DerivF k2 = kerr_rhs_f32(F, yr + hh * k1.r, y.th.hi, y.th.lo + hh * k1.th, ypr + hh * k1.pr,
                                    ypth + hh * k1.pth, E, L);
-    const DerivF k3 = kerr_rhs_f32(F, yr + hh * k2.r, yth + hh * k2.th, ypr + hh * k2.pr,
+    const DerivF k3 = kerr_rhs_f32(F, yr + hh * k2.r, y.th.hi, y.th.lo + hh * k2.th, ypr + hh * k2.pr,
                                    ypth + hh * k2.pth, E, L);
-    const DerivF k4 = kerr_rhs_f32(F, yr + h * k3.r, yth + h * k3.th, ypr + h * k3.pr,
+    const DerivF k4 = kerr_rhs_f32(F, yr + h * k3.r, y.th.hi, y.th.lo + h * k3.th, ypr + h * k3.pr,
                                    ypth + h * k3.pth, E, L);
     rhs_count += 3;
     const float h6 = h * (1.0f / 6.0f);
@@ -661,12 +845,12 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
                                                   FF5& full, FF5& half1, FF5& half2) {
     const float h2 = 0.5f * h;
     DerivF S = k1, kp = k1;
-    float br = ff_f(y.r), bth = ff_f(y.th), bpr = ff_f(y.pr), bpth = ff_f(y.pth);
+    float br = ff_f(y.r), bth = y.th.hi, bthl = y.th.lo, bpr = ff_f(y.pr), bpth = ff_f(y.pth);
 #pragma clang loop unroll(disable)
     for (int j = 0; j < 10; ++j) {
         const float c = (j <= 1) ? 0.5f * h : (j == 2) ? h : (j == 5 || j == 9) ? h2 : 0.5f * h2;
         const float cc = (j == 6) ? 0.0f : c;
-        const DerivF k = kerr_rhs_f32(F, br + cc * kp.r, bth + cc * kp.th, bpr + cc * kp.pr,
+        const DerivF k = kerr_rhs_f32(F, br + cc * kp.r, bth, bthl + cc * kp.th, bpr + cc * kp.pr,
                                       bpth + cc * kp.pth, E, L);
         if (j == 6) {
             S = k;
@@ -690,7 +874,7 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
                 kp = k1;
             } else if (j == 5) {
                 half1 = o;
-                br = ff_f(o.r); bth = ff_f(o.th); bpr = ff_f(o.pr); bpth = ff_f(o.pth);
+                br = ff_f(o.r); bth = o.th.hi; bthl = o.th.lo; bpr = ff_f(o.pr); bpth = ff_f(o.pth);
             } else {
                 half2 = o;
             }
@@ -698,9 +882,18 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
     }
 }
 
-__device__ __forceinline__ float errf(ff x, ff y, float rt, float at) {
+// Error-norm term of one component for a float attempt: |full - half2|
+// over atol + rtol max(|full|, |half2|), after removing the part of the
+// difference that float rounding of the stage values alone can produce
+// (kNoise ulps of the step's magnitude, taken as the full step's increment
+// plus h |k1|).  Differences at that floor carry no truncation information;
+// counting them would reject sound far-field steps of components that pass
+// near zero (p_r at periapsis, p_theta at a theta turning point).
+constexpr float kNoise = 16.0f * 5.9604645e-8f;
+__device__ __forceinline__ float errf(ff x, ff y, ff y0, float hk1, float rt, float at) {
     const float sc = at + rt * fmaxf(fabsf(x.hi), fabsf(y.hi));
-    return fabsf(ff_diff(x, y)) / fmaxf(sc, 1e-30f);
+    const float floor_ = kNoise * (fabsf(ff_diff(x, y0)) + fabsf(hk1));
+    return fmaxf(fabsf(ff_diff(x, y)) - floor_, 0.0f) / fmaxf(sc, 1e-30f);
 }
 
 __device__ __forceinline__ bool okf(const FrameConst& F, const FF5& y) {
@@ -714,6 +907,9 @@ struct StepF32C {
         DerivF k1;
         float E, L, h_adapt, h;
         bool k1_valid;
+        // 0: float attempt; 1: FP64 (step started inside r_isl); 2: FP64
+        // re-run of a float attempt that failed or was rejected
+        uint8_t mode;
     };
 
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
@@ -730,6 +926,7 @@ struct StepF32C {
         R.h_adapt = 0.0f;
         R.h = 0.0f;
         R.k1_valid = false;
+        R.mode = 0;
         base_start(F, R, cam_x, cam_y);
     }
 
@@ -738,60 +935,143 @@ struct StepF32C {
     __device__ static double pr(const Ray& R) { return ff_to_d(R.y.pr); }
     __device__ static double pth(const Ray& R) { return ff_to_d(R.y.pth); }
 
+    __device__ static FF5 to_ff5(const Y5& y) {
+        return FF5{ff_from_d(y.r), ff_from_d(y.th), ff_from_d(y.ph), ff_from_d(y.pr), ff_from_d(y.pth)};
+    }
+
+    // Step size and (r, theta) derivatives at both ends of the step just
+    // accepted, for the Hermite crossing: R.k1 holds f at the step's start
+    // (float path, or the FP64 step's first stage); f at its end is
+    // evaluated in FP64 and kept as the next k1.
+    __device__ static void cross_derivs(const FrameConst& F, Ray& R, double& h, double& dr0,
+                                        double& dth0, double& dr1, double& dth1) {
+        h = (double)R.h;
+        dr0 = (double)R.k1.r;
+        dth0 = (double)R.k1.th;
+        const Deriv k = kerr_rhs_c64(F, ff_to_d(R.y.r), ff_to_d(R.y.th), ff_to_d(R.y.pr),
+                                     ff_to_d(R.y.pth), -R.pt, R.pph);
+        dr1 = k.r;
+        dth1 = k.th;
+        R.k1 = DerivF{(float)k.r, (float)k.th, (float)k.ph, (float)k.pr, (float)k.pth};
+        R.k1_valid = true;
+    }
+
+    // FP64 attempt with the Carter-form field: the step control of
+    // StepF64::attempt (rk4_adaptive_step_d, bh_kerr.h:1058-1105, and the
+    // retry loop of render.cu:193-205) on the FP64 image of the state.
+    // Step sizes are held in float (a 6e-8 relative change of h); so is the
+    // step factor.
+    __device__ static int attempt64(const FrameConst& F, Ray& R, int& rhs64) {
+        __shared__ double s_full[5 * 256];
+        double* full = s_full + threadIdx.x;
+        const double h = (double)R.h;
+        Y5 half2;
+        double k1r, k1th;
+        const bool ok = step_doubling_c64(F, R.y, -R.pt, R.pph, h, full, half2, k1r, k1th);
+        rhs64 += 12;
+        float h_try = R.h;
+        float fac = 0.0f;
+        if (!ok) {
+            h_try *= 0.5f;
+        } else {
+            const double rt = F.rtol, at = F.atol;
+            const int sd = blockDim.x;
+            double e = 0.0;
+            const double* hv = &half2.r;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double fv = full[c * sd];
+                e = fmax(e, err1(fv, hv[c], at + rt * fmax(fabs(fv), fabs(hv[c]))));
+            }
+            const float le = __log2f(fmaxf(1e-12f, (float)e));
+            if (e <= 1.0) {
+                fac = fminf(2.0f, 0.9f * exp2f(-0.2f * le));
+                R.y = to_ff5(half2);
+                R.k1.r = (float)k1r;
+                R.k1.th = (float)k1th;
+                R.k1_valid = false;
+            } else {
+                h_try *= fmaxf(0.1f, 0.9f * exp2f(-0.25f * le));
+            }
+        }
+        const float hmin = (float)HMIN, hmax = (float)HMAX;
+        if (fac > 0.0f) {
+            R.h_adapt = fminf(fmaxf(h_try * fac, hmin), hmax);
+            R.attempt = 0;
+            R.mode = 0;
+            return 1;
+        }
+        R.h = fmaxf(h_try, hmin);
+        R.attempt += 1;
+        if (R.h <= hmin * 1.01f || R.attempt >= 8) return 2;
+        return 0;
+    }
+
     template <bool ADAPTIVE>
-    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count) {
-        const float hmin = 1e-6f, hmax = 0.2f;
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& rhs64) {
+        const float hmin = (float)HMIN, hmax = (float)HMAX;
         if (!ADAPTIVE) {
-            const float scale = fminf(fmaxf(1.0f, ff_f(R.y.r) / (0.5f * F.Rs)), 100.0f);
+            const float rf = ff_f(R.y.r);
+            const float scale = fminf(fmaxf(1.0f, rf / (0.5f * F.Rs)), 100.0f);
             const float h = STEP0 * scale;
-            const DerivF k1 = kerr_rhs_f32(F, ff_f(R.y.r), ff_f(R.y.th), ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
-            rhs_count += 1;
-            R.y = rk4_f32(F, R.y, k1, R.E, R.L, h, rhs_count);
+            R.h = h;
+            if (ff_to_d(R.y.r) < F.r_isl) {
+                double k1r, k1th;
+                R.y = to_ff5(rk4_c64(F, ff5_to_y5(R.y), -R.pt, R.pph, (double)h, k1r, k1th));
+                R.k1.r = (float)k1r;
+                R.k1.th = (float)k1th;
+                rhs64 += 4;
+            } else {
+                const DerivF k1 = kerr_rhs_f32(F, rf, R.y.th.hi, R.y.th.lo, ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
+                rhs_count += 1;
+                R.k1 = k1;
+                R.y = rk4_f32(F, R.y, k1, R.E, R.L, h, rhs_count);
+            }
             const double rr = ff_to_d(R.y.r), tt = ff_to_d(R.y.th);
             const bool ok = rr > F.r_plus + 1e-9 && tt > 1e-9 && tt < NP_PI_D - 1e-9;
             return ok ? 1 : 2;
         }
-        if (R.attempt == 0) {
-            const float scale = fminf(fmaxf(1.0f, ff_f(R.y.r) / (0.5f * F.Rs)), 100.0f);
-            float h_base = fminf(fmaxf(STEP0 * scale, hmin), hmax);
+        if (R.attempt == 0 && R.mode != 2) {
+            // step-size choice at the start of a step (render.cu:185-191)
+            const float rf = ff_f(R.y.r);
+            const float scale = fminf(fmaxf(1.0f, rf / (0.5f * F.Rs)), 100.0f);
+            const float h_base = fminf(fmaxf(STEP0 * scale, hmin), hmax);
             const float h = (R.h_adapt > 0.0f) ? R.h_adapt : h_base;
             R.h = fminf(fmaxf(h, hmin), hmax);
+            R.mode = (ff_to_d(R.y.r) < F.r_isl) ? 1 : 0;
         }
+        if (R.mode != 0) return attempt64(F, R, rhs64);
         const float h = R.h;
         if (!R.k1_valid) {
-            R.k1 = kerr_rhs_f32(F, ff_f(R.y.r), ff_f(R.y.th), ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
+            R.k1 = kerr_rhs_f32(F, ff_f(R.y.r), R.y.th.hi, R.y.th.lo, ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
             rhs_count += 1;
             R.k1_valid = true;
         }
         FF5 full, half1, half2;
         step_doubling_f32(F, R.y, R.k1, R.E, R.L, h, full, half1, half2);
         rhs_count += 10;
-        float h_try = h;
-        float fac = 0.0f;
-        if (!(okf(F, full) && okf(F, half1) && okf(F, half2))) {
-            h_try *= 0.5f;
-        } else {
+        if (okf(F, full) && okf(F, half1) && okf(F, half2)) {
             const float rt = F.rtolf, at = F.atolf;
-            const float e = fmaxf(fmaxf(errf(full.r, half2.r, rt, at), errf(full.th, half2.th, rt, at)),
-                                  fmaxf(errf(full.ph, half2.ph, rt, at),
-                                        fmaxf(errf(full.pr, half2.pr, rt, at), errf(full.pth, half2.pth, rt, at))));
-            const float le = __log2f(fmaxf(1e-12f, e));
+            const DerivF& k = R.k1;
+            const float e = fmaxf(fmaxf(errf(full.r, half2.r, R.y.r, h * k.r, rt, at),
+                                        errf(full.th, half2.th, R.y.th, h * k.th, rt, at)),
+                                  fmaxf(errf(full.ph, half2.ph, R.y.ph, h * k.ph, rt, at),
+                                        fmaxf(errf(full.pr, half2.pr, R.y.pr, h * k.pr, rt, at),
+                                              errf(full.pth, half2.pth, R.y.pth, h * k.pth, rt, at))));
             if (e <= 1.0f) {
-                fac = fminf(2.0f, 0.9f * exp2f(-0.2f * le));
+                const float le = __log2f(fmaxf(1e-12f, e));
+                const float fac = fminf(2.0f, 0.9f * exp2f(-0.2f * le));
                 R.y = half2;
                 R.k1_valid = false;
-            } else {
-                h_try *= fmaxf(0.1f, 0.9f * exp2f(-0.25f * le));
+                R.h_adapt = fminf(fmaxf(h * fac, hmin), hmax);
+                R.attempt = 0;
+                return 1;
             }
         }
-        if (fac > 0.0f) {
-            R.h_adapt = fminf(fmaxf(h_try * fac, hmin), hmax);
-            R.attempt = 0;
-            return 1;
-        }
-        R.h = fmaxf(h_try, hmin);
-        R.attempt += 1;
-        if (R.h <= hmin * 1.01f || R.attempt >= 8) return 2;
+        // A float attempt that leaves the domain or fails the tolerance is
+        // re-run in FP64 with the same step; that run decides acceptance,
+        // the next step size and the retry budget exactly as the FP64 path.
+        R.mode = 2;
         return 0;
     }
 };
@@ -801,11 +1081,11 @@ struct StepF32C {
 // attempt per loop iteration.
 // ---------------------------------------------------------------------------
 template <class S, bool ADAPTIVE, bool INV>
-__global__ void __launch_bounds__(256)
-kerr_render(const FrameConst F, const int y0, const int rows, const int spp,
-            const int sample_offset, float* __restrict__ out, const int accumulate,
-            float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
-            np_ray_record* __restrict__ records, unsigned long long* __restrict__ work) {
+__device__ __forceinline__ void
+kerr_render_body(const FrameConst& F, const int y0, const int rows, const int spp,
+                 const int sample_offset, float* __restrict__ out, const int accumulate,
+                 float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
+                 np_ray_record* __restrict__ records, unsigned long long* __restrict__ work) {
     const int W = F.width;
     const int y_end = min(y0 + rows, F.height);
     const uint64_t npix = (y_end > y0) ? (uint64_t)(y_end - y0) * (uint64_t)W : 0ull;
@@ -818,7 +1098,7 @@ kerr_render(const FrameConst F, const int y0, const int rows, const int spp,
     float inv_lane[3] = {0.0f, 0.0f, 0.0f};
     // per-lane counters, flushed to the u64 totals before they can wrap
     uint32_t c_rhs = 0, c_att = 0, c_steps = 0, c_rays = 0, c_cap = 0, c_esc = 0, c_cap_steps = 0,
-             c_disk = 0;
+             c_disk = 0, c_rhs64 = 0, c_att64 = 0;
     if (spp <= 0 || F.max_steps <= 0) done = true;
 
     while (true) {
@@ -846,17 +1126,21 @@ kerr_render(const FrameConst F, const int y0, const int rows, const int spp,
             wave_add_u64(&counters->steps, c_steps);
             wave_add_u64(&counters->attempts, c_att);
             wave_add_u64(&counters->rhs_evals, c_rhs);
-            c_steps = c_att = c_rhs = 0;
+            wave_add_u64(&counters->attempts_f64, c_att64);
+            wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
+            c_steps = c_att = c_rhs = c_att64 = c_rhs64 = 0;
         }
         if (has) {
-            int rhs_count = 0;
+            int rhs_count = 0, rhs64 = 0;
             R.attempts_total += 1;
             c_att += 1;
-            const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count);
-            c_rhs += (uint32_t)rhs_count;
+            const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count, rhs64);
+            c_rhs += (uint32_t)(rhs_count + rhs64);
+            c_rhs64 += (uint32_t)rhs64;
+            c_att64 += (rhs64 > 0);
             int cls = -1;
             if (st == 2) cls = NP_RAY_CAPTURED;
-            else if (st == 1) cls = after_accept<INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
+            else if (st == 1) cls = after_accept<S, INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
             if (cls >= 0) {
                 float col[3];
                 shade_final(F, R, cls, col);
@@ -916,6 +1200,8 @@ kerr_render(const FrameConst F, const int y0, const int rows, const int spp,
         wave_add_u64(&counters->escaped, c_esc);
         wave_add_u64(&counters->stepcap, c_cap_steps);
         wave_add_u64(&counters->disk_hits, c_disk);
+        wave_add_u64(&counters->attempts_f64, c_att64);
+        wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
     }
     if (INV && inv_max3) {
         atomic_max_nonneg(&inv_max3[0], inv_lane[0]);
@@ -924,15 +1210,48 @@ kerr_render(const FrameConst F, const int y0, const int rows, const int spp,
     }
 }
 
+#define NP_KERNEL_ARGS                                                                          \
+    const FrameConst F, const int y0, const int rows, const int spp, const int sample_offset,    \
+        float* __restrict__ out, const int accumulate, float* __restrict__ inv_max3,            \
+        np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records,          \
+        unsigned long long* __restrict__ work
+#define NP_KERNEL_PASS F, y0, rows, spp, sample_offset, out, accumulate, inv_max3, counters, records, work
+
+// Kernel entry points.  Variant 0 leaves the register budget to the
+// compiler; variant 1 (FP32C only) asks for 3 waves per SIMD, which caps
+// the kernel at 168 VGPRs and moves the excess of the rarely taken FP64
+// branch to scratch.
+template <class S, bool ADAPTIVE, bool INV>
+__global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
+    kerr_render_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
+}
+
+template <bool ADAPTIVE, bool INV>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+kerr_render_f32c_w3(NP_KERNEL_ARGS) {
+    kerr_render_body<StepF32C, ADAPTIVE, INV>(NP_KERNEL_PASS);
+}
+
+#undef NP_KERNEL_ARGS
+#undef NP_KERNEL_PASS
+
 template <class S, bool A>
 const void* kernel_ptr(bool inv) {
     return inv ? reinterpret_cast<const void*>(&kerr_render<S, A, true>)
                : reinterpret_cast<const void*>(&kerr_render<S, A, false>);
 }
 
-const void* select_kernel(int precision, bool adaptive, bool inv) {
-    if (precision == NP_PREC_FP32C)
+template <bool A>
+const void* kernel_ptr_w3(bool inv) {
+    return inv ? reinterpret_cast<const void*>(&kerr_render_f32c_w3<A, true>)
+               : reinterpret_cast<const void*>(&kerr_render_f32c_w3<A, false>);
+}
+
+const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
+    if (precision == NP_PREC_FP32C) {
+        if (variant == 1) return adaptive ? kernel_ptr_w3<true>(inv) : kernel_ptr_w3<false>(inv);
         return adaptive ? kernel_ptr<StepF32C, true>(inv) : kernel_ptr<StepF32C, false>(inv);
+    }
     return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
 }
 
@@ -942,32 +1261,17 @@ int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int r
                 int sample_offset, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid) {
-    const dim3 block(256);
-#define NP_LAUNCH(S, A)                                                                          \
-    do {                                                                                         \
-        if (F.check_inv)                                                                         \
-            hipLaunchKernelGGL((kerr_render<S, A, true>), dim3(grid), block, 0, stream, F, y0,   \
-                               rows, spp, sample_offset, d_out, accumulate, d_inv, d_counters,   \
-                               d_records, d_work);                                               \
-        else                                                                                     \
-            hipLaunchKernelGGL((kerr_render<S, A, false>), dim3(grid), block, 0, stream, F, y0,  \
-                               rows, spp, sample_offset, d_out, accumulate, d_inv, d_counters,   \
-                               d_records, d_work);                                               \
-    } while (0)
-    if (precision == NP_PREC_FP32C) {
-        if (adaptive) NP_LAUNCH(StepF32C, true);
-        else NP_LAUNCH(StepF32C, false);
-    } else {
-        if (adaptive) NP_LAUNCH(StepF64, true);
-        else NP_LAUNCH(StepF64, false);
-    }
-#undef NP_LAUNCH
-    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+    const void* fn = select_kernel(precision, adaptive, F.check_inv != 0, F.variant);
+    FrameConst Fc = F;
+    void* args[] = {&Fc, &y0, &rows, &spp, &sample_offset, &d_out, &accumulate, &d_inv,
+                    &d_counters, &d_records, &d_work};
+    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, stream);
+    return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
 }
 
-int occupancy_kerr(int precision, bool adaptive, bool inv, int* blocks_per_cu) {
+int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int* blocks_per_cu) {
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, select_kernel(precision, adaptive, inv), 256, 0);
+        blocks_per_cu, select_kernel(precision, adaptive, inv, variant), 256, 0);
     return e == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

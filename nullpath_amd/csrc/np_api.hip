@@ -72,9 +72,10 @@ int take_slot(DeviceState* d, hipStream_t stream, unsigned long long** slot) {
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
-int persistent_grid(DeviceState* d, int precision, bool adaptive, bool inv, uint64_t items) {
+int persistent_grid(DeviceState* d, int precision, bool adaptive, bool inv, int variant,
+                    uint64_t items) {
     int per_cu = 0;
-    if (occupancy_kerr(precision, adaptive, inv, &per_cu) != NP_OK || per_cu <= 0) per_cu = 1;
+    if (occupancy_kerr(precision, adaptive, inv, variant, &per_cu) != NP_OK || per_cu <= 0) per_cu = 1;
     uint64_t grid = (uint64_t)per_cu * (uint64_t)d->cus;
     uint64_t need = (items + 255) / 256;
     if (need < grid) grid = need;
@@ -99,7 +100,7 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     err = take_slot(d, stream, &slot);
     if (err != NP_OK) return err;
     const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
-    const int grid = persistent_grid(d, p->precision, adaptive, p->check_inv != 0, npix);
+    const int grid = persistent_grid(d, p->precision, adaptive, p->check_inv != 0, F.variant, npix);
     return launch_kerr(F, p->precision, adaptive, y0, rows, spp, sample_offset, d_rgb, accumulate,
                        d_inv, d_counters, d_records, slot, stream, grid);
 }

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "np_math.hpp"
+
 #define NP_PI_D 3.141592653589793238462643383279502884
 #define NP_PI_F 3.14159265358979323846f
 

@@ -101,6 +101,9 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->rout = p->disk_rout;
     f->rtol = p->rtol;
     f->atol = p->atol;
+    f->crossing = p->crossing == NP_CROSSING_HERMITE ? 1 : 0;
+    f->variant = (p->precision == NP_PREC_FP32C && p->waves_per_simd != 2) ? 1 : 0;
+    f->r_isl = p->fp64_radius > 0.0 ? p->fp64_radius : std::fmax(30.0 * M, 1.5 * p->disk_rout);
     f->s_eq = std::sin(3.141592653589793238462643383279502884 * 0.5);
     f->c_eq = std::cos(3.141592653589793238462643383279502884 * 0.5);
     f->Mf = (float)M;

@@ -16,6 +16,7 @@ struct FrameConst {
     // geometry (absolute units)
     double M, a, r_plus, r_cam, th_cam, ph_cam, Rs_d, r_far, rin, rout;
     double rtol, atol;
+    double r_isl;        // FP32C: steps starting at r < r_isl run in FP64
     double s_eq, c_eq;   // sin, cos of pi/2 as the host libm returns them
     // observer tetrad: p^mu components as functions of the local direction n
     //   p^t   = ct0 + n_phi * ct_ph
@@ -33,6 +34,8 @@ struct FrameConst {
     int32_t width, height;
     int32_t zamo;         // 1 when the ZAMO tetrad is used
     int32_t bg_mode, disk_on, disk_max_orders, check_inv, max_steps, emiss_m_units;
+    int32_t variant;      // kernel variant (FP32C: 1 = 3 waves per SIMD)
+    int32_t crossing;     // 0 linear crossing radius (reference), 1 cubic Hermite
 };
 
 // Returns 0 on success, NP_EINVAL when the parameters are outside what the

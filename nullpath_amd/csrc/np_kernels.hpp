@@ -13,7 +13,7 @@ int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int r
                 int sample_offset, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid);
-int occupancy_kerr(int precision, bool adaptive, bool inv, int* blocks_per_cu);
+int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int* blocks_per_cu);
 
 // A cleared work-queue counter on the current device (cleared by a kernel
 // on `stream`, so the call may be graph-captured).

@@ -10,6 +10,10 @@
 //   --disk-rin isco|X        --disk-rout X        (X in units of M)
 //   --emissivity-index q     (emissivity (r/M)^-q; default r^-2 absolute)
 //   --raw-out file.f32       (linear float32 RGB, accum / spp, row-major)
+//   --fp64-radius X          (fp32c: FP64 steps inside X M; default max(30, 1.5 disk-rout))
+//   --waves-per-simd 2|3     (fp32c kernel variant: 3 = capped at 168 VGPRs, the default;
+//                             2 = the compiler's register budget)
+//   --crossing linear|hermite (disk-crossing radius; default linear as the reference)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,6 +66,9 @@ struct Options {
     const char* raw_out = nullptr;
     unsigned seed = 0;
     int max_steps = 10000;
+    double fp64_radius_M = 0.0;
+    int waves_per_simd = 0;
+    const char* crossing = "linear";
 };
 
 void parse_args(int argc, char** argv, Options& p) {
@@ -108,6 +115,9 @@ void parse_args(int argc, char** argv, Options& p) {
         else if (a == "--raw-out") nexts(p.raw_out);
         else if (a == "--seed" && i + 1 < argc) p.seed = (unsigned)strtoul(argv[++i], nullptr, 10);
         else if (a == "--max-steps") nexti(p.max_steps);
+        else if (a == "--fp64-radius") nextd(p.fp64_radius_M);
+        else if (a == "--waves-per-simd") nexti(p.waves_per_simd);
+        else if (a == "--crossing" && i + 1 < argc) p.crossing = argv[++i];
     }
 }
 
@@ -165,6 +175,11 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "Invalid --precision; expected fp64 or fp32c.\n");
         return 1;
     }
+    const std::string crossing = lower(o.crossing);
+    if (crossing != "linear" && crossing != "hermite") {
+        std::fprintf(stderr, "Invalid --crossing; expected linear or hermite.\n");
+        return 1;
+    }
 
     // mode mapping (render.cu:467-474)
     const std::string mode = lower(o.mode);
@@ -209,6 +224,9 @@ int main(int argc, char** argv) {
     if (o.atol >= 0.0) p.atol = o.atol;
     p.seed = o.seed;
     const double M = (double)o.mass;
+    p.fp64_radius = o.fp64_radius_M > 0.0 ? o.fp64_radius_M * M : 0.0;
+    p.waves_per_simd = o.waves_per_simd;
+    p.crossing = (crossing == "hermite") ? NP_CROSSING_HERMITE : NP_CROSSING_LINEAR;
     p.disk_rin = 3.0 * 2.0 * M;
     p.disk_rout = 20.0 * 2.0 * M;
     if (o.disk_rin) {

@@ -419,6 +419,30 @@ static void inv_update(float inv[3], const double st[8], double M, double a,
     inv[2] = fmaxf(inv[2], Lrel);
 }
 
+/* Crossing radius from the cubic Hermite interpolants of theta(lambda) and
+ * r(lambda) over an accepted step of size h (an extension of this library,
+ * np_crossing HERMITE): Newton on theta(s) = pi/2, s = lambda/h in [0, 1],
+ * from the linear estimate s0, four iterations; the linear radius is kept
+ * when an iterate leaves [0, 1] or the slope vanishes. */
+double orc_hermite_crossing(double h, double r0, double th0, double dr0, double dth0, double r1,
+                            double th1, double dr1, double dth1, double s0, double r_lin) {
+    const double m0 = h * dth0, m1 = h * dth1, n0 = h * dr0, n1 = h * dr1;
+    const double g0 = th0 - BH_PI_D * 0.5, g1 = th1 - BH_PI_D * 0.5;
+    double s = s0;
+    for (int it = 0; it < 4; ++it) {
+        double s2 = s * s, s3 = s2 * s;
+        double f = (2 * s3 - 3 * s2 + 1) * g0 + (s3 - 2 * s2 + s) * m0 + (3 * s2 - 2 * s3) * g1 +
+                   (s3 - s2) * m1;
+        double fp = (6 * s2 - 6 * s) * g0 + (3 * s2 - 4 * s + 1) * m0 + (6 * s - 6 * s2) * g1 +
+                    (3 * s2 - 2 * s) * m1;
+        if (!(fabs(fp) > 0.0)) return r_lin;
+        s -= f / fp;
+        if (!(s >= 0.0 && s <= 1.0)) return r_lin;
+    }
+    double s2 = s * s, s3 = s2 * s;
+    return (2 * s3 - 3 * s2 + 1) * r0 + (s3 - 2 * s2 + s) * n0 + (3 * s2 - 2 * s3) * r1 + (s3 - s2) * n1;
+}
+
 void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
                     orc_ray_info* info) {
     const float Rs = 2.0f * p->mass;
@@ -449,6 +473,7 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
     int reached_far = 0;
     float inv[3] = {0, 0, 0};
     int step;
+    double h_acc = 0.0;   /* size of the last accepted step */
     for (step = 0; step < p->max_steps; ++step) {
         double scale = fmin(fmax(1.0, state[R_] / (0.5 * (double)Rs)), 100.0);
         int ok = 0;
@@ -463,6 +488,7 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
                 double fac = orc_kerr_adaptive_step_d(state, M, a, &h_try, p->rtol, p->atol);
                 if (fac > 0.0) {
                     ok = 1;
+                    h_acc = h;
                     h_adapt = fmin(fmax(h_try * fac, BH_RK_HMIN), BH_RK_HMAX);
                     break;
                 }
@@ -471,7 +497,8 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
             }
         } else {
             ++attempts; rhs += 4;
-            ok = orc_kerr_rk4_fixed_d(state, M, a, (double)BH_INTEGRATION_STEP * scale);
+            h_acc = (double)BH_INTEGRATION_STEP * scale;
+            ok = orc_kerr_rk4_fixed_d(state, M, a, h_acc);
         }
         if (!ok) { captured = 1; break; }
         ++steps;
@@ -487,6 +514,13 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
                 double denom = fabs(d_prev) + fabs(d_curr) + 1e-30;
                 double alpha = fabs(d_prev) / denom;
                 double r_cross = prev[R_] + alpha * (state[R_] - prev[R_]);
+                if (p->crossing == NP_CROSSING_HERMITE && sign_change) {
+                    double d0[8], d1[8];
+                    orc_kerr_rhs_d(prev, M, a, d0);
+                    orc_kerr_rhs_d(state, M, a, d1);
+                    r_cross = orc_hermite_crossing(h_acc, prev[R_], prev[TH_], d0[R_], d0[TH_], state[R_],
+                                                   state[TH_], d1[R_], d1[TH_], alpha, r_cross);
+                }
                 if (r_cross >= rin && r_cross <= rout) {
                     double cross[8];
                     memcpy(cross, state, sizeof(cross));

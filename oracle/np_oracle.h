@@ -73,6 +73,9 @@ void orc_camera_dir(const np_render_params* p, int x, int y, uint32_t sample_id,
 uint32_t orc_wanghash(uint32_t a);
 
 /* Trace one sample; returns its colour in info->rgb. */
+/* Hermite crossing radius (extension, np_crossing HERMITE). */
+double orc_hermite_crossing(double h, double r0, double th0, double dr0, double dth0, double r1,
+                            double th1, double dr1, double dth1, double s0, double r_lin);
 void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
                     orc_ray_info* info);
 /* Sum of `spp` samples starting at sample_offset for each listed pixel, in

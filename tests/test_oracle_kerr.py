@@ -244,3 +244,38 @@ def test_classification_matches_radial_potential(oracle):
         n_bad += captured == tp
     assert n_checked > 1000
     assert n_bad == 0, n_bad
+
+
+def _disk_pixels(oracle, p, xs, ys):
+    q = oracle.default_params()
+    for name, _ in npa.RenderParams._fields_:
+        setattr(q, name, getattr(p, name))
+    rgb, info = oracle.render_pixels(q, xs, ys, 0, 1)
+    return rgb[:, 0].astype(np.float64), info
+
+
+def test_hermite_crossing_converges_where_linear_does_not(oracle):
+    """The Hermite crossing radius (np_crossing HERMITE) at rtol 1e-8 agrees
+    with a tightly converged render (rtol 1e-11, Hermite) far better than the
+    reference's linear interpolation does at the same tolerance; and at rtol
+    1e-11 the two crossing modes approach each other (the linear error falls
+    as the step shrinks), so both are estimating the same radius."""
+    p = npa.kerr_disk_params(width=40, height=24, fov=3.0)
+    ys, xs = np.mgrid[0:24, 0:40]
+    xs, ys = xs.ravel(), ys.ravel()
+    lin, info = _disk_pixels(oracle, p, xs, ys)
+    p.crossing = 1
+    her, _ = _disk_pixels(oracle, p, xs, ys)
+    p.rtol = 1e-11
+    ref, info_ref = _disk_pixels(oracle, p, xs, ys)
+    p.crossing = 0
+    lin_fine, _ = _disk_pixels(oracle, p, xs, ys)
+    disk = (info["orders"] > 0) & (info_ref["orders"] == info["orders"]) & (ref > 0)
+    assert disk.sum() > 50
+    e_lin = np.abs(lin[disk] - ref[disk]) / ref[disk]
+    e_her = np.abs(her[disk] - ref[disk]) / ref[disk]
+    e_lin_fine = np.abs(lin_fine[disk] - ref[disk]) / ref[disk]
+    assert np.median(e_lin) > 1e-5     # the linear interpolant's error is visible
+    assert np.median(e_her) < 0.1 * np.median(e_lin)
+    assert np.quantile(e_her, 0.99) < 1e-6
+    assert np.median(e_lin_fine) < 0.5 * np.median(e_lin)
