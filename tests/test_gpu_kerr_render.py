@@ -104,11 +104,11 @@ def test_fp32c_matches_fp64_on_disk_workload(device):
     workload zoomed on the hole: identical class and crossing count for every
     ray whose FP64 closest approach is not within 0.05 M of the photon-orbit
     region's edge of capture, disk radiance within 1e-5 (p99) and 1e-4 (max)."""
-    (ia, ca, ra), (ib, cb, rb) = _disk_pair(160, 90, 6.0, crossing=1)
+    (ia, ca, ra), (ib, cb, rb) = _disk_pair(160, 90, 3.0, crossing=1)
     same = (ra["cls"] == rb["cls"]) & (ra["orders"] == rb["orders"])
     assert np.mean(same) > 0.999, int((~same).sum())
     disk = (ra["orders"] > 0) & same
-    assert disk.sum() > 500
+    assert disk.sum() > 1000
     rel = (np.abs(ib - ia) / np.maximum(np.abs(ia), 1e-30)).max(axis=1)[disk]
     print(f"[gpu] fp32c vs fp64 disk rel err: median {np.median(rel):.3g} p99 "
           f"{np.quantile(rel, 0.99):.3g} max {rel.max():.3g}", flush=True)
