@@ -31,6 +31,12 @@ sys.path.insert(0, ROOT)
 METRIC = "Kerr rays/s @1080p a=0.9, FP32-comp vs FP64 at ≤1e-5 err; adjoint grads/s"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
+# HBM bytes per ray of the FP32C renderer, measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on the 480x270x1 frame and scaled by a copy kernel of known size
+# (profiles/kerr_render_f32c_480x270x1_hbm_traffic.txt): 13.8 MB read + 35.0 MB
+# written for 129,600 rays.
+FP32C_HBM_BYTES_PER_RAY = 376.0
+
 # Algorithmic operation counts per unit (add, sub, mul, div, sqrt, rcp = 1;
 # fma = 2; library sin/cos/pow counted apart), derived from the kernel source:
 #   fp64 path: the reference right-hand side (bh_kerr.h:600-727) is 200
@@ -68,7 +74,25 @@ def parse():
     ap.add_argument("--adjoint", type=int, default=1,
                     help="also time the differentiable 512x512 render (forward + adjoint)")
     ap.add_argument("--adjoint-size", type=int, default=512)
+    ap.add_argument("--side-workloads", type=int, default=1,
+                    help="also time the Schwarzschild 1e5 sweep and the GEOMU 2^20 batch")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (a reference "
+                         "kernel of known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) "
+                         "and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> int:
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
+    return 0
 
 
 def render_case(prec: str, crossing: str, width: int, height: int, spp: int, steps: int,
@@ -231,6 +255,69 @@ def bench_adjoint(args) -> dict:
     }
 
 
+def bench_schwarzschild(n: int) -> dict:
+    """Plain-method FP64 Schwarzschild sweep: M = 10, n equatorial photons
+    from r0 = 1000 M, b uniform on [0.5 M, 15 M] (seed 2024), captured at
+    r < 2.01 M, escaped at r > r0 outbound, photon sphere |b - 3 sqrt(3) M| <
+    1e-3 M.  Timed with HIP events on the launch stream."""
+    import numpy as np
+    import torch
+
+    import nullpath_amd as npa
+    M = 10.0
+    rng = np.random.default_rng(2024)
+    b = torch.tensor(rng.uniform(0.5 * M, 15.0 * M, n), dtype=torch.float64, device="cuda")
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    npa.schw_sweep(b[:256], M=M)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    cls, dphi, steps = npa.schw_sweep(b, M=M, total_steps=total, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    c = np.bincount(cls.cpu().numpy(), minlength=4)
+    return {
+        "workload": f"Schwarzschild M=10, {n} equatorial photons from r0=1000M, b~U[0.5M,15M], FP64 RK4",
+        "rays_per_s": n / (ms / 1e3), "kernel_ms": ms,
+        "steps_per_s": int(total.item()) / (ms / 1e3),
+        "escaped": int(c[0]), "captured": int(c[1]), "photon_sphere": int(c[2]), "stepcap": int(c[3]),
+    }
+
+
+def bench_geomu(n: int) -> dict:
+    """GEOKERR GEOMU batch: n rays with alpha, beta ~ U[-12, 12] M, a ~
+    U[0, 0.998], mu0 ~ U[0.05, 0.99], target u ~ U(0, u_+) (seed 2025), from
+    u0 = 1e-4, FP64 with Carlson RF; timed with HIP events."""
+    import numpy as np
+    import torch
+
+    import nullpath_amd as npa
+    rng = np.random.default_rng(2025)
+    t = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda")
+    a = rng.uniform(0.0, 0.998, n)
+    uplus = 1.0 / (1.0 + np.sqrt(1.0 - a * a))
+    args = [t(rng.uniform(-12, 12, n)), t(rng.uniform(-12, 12, n)), t(a),
+            t(rng.uniform(0.05, 0.99, n)), t(rng.uniform(0.0, 1.0, n) * uplus)]
+    npa.geomu_camera_batch(*[x[:256] for x in args], 1e-4)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    muf, tpm, ncase, iu = npa.geomu_camera_batch(*args, 1e-4, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    nc = np.bincount(ncase.cpu().numpy(), minlength=9)
+    return {
+        "workload": f"GEOMU {n} rays, alpha,beta~U[-12,12]M, a~U[0,0.998], mu0~U[0.05,0.99], u~U(0,u+), FP64",
+        "rays_per_s": n / (ms / 1e3), "kernel_ms": ms,
+        "ncase_counts": [int(v) for v in nc],
+        "nan_muf": int(torch.isnan(muf).sum().item()),
+    }
+
+
 def cpu_baseline(args, fp64: dict) -> dict:
     """The FP64 oracle (a plain-C restatement of the reference integrator)
     on a bounded random sample of the same frame, on the host cores."""
@@ -274,6 +361,8 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build",
               file=sys.stderr)
         return 2
+    if args.copy_calibration_mib > 0:
+        return copy_calibration(args.copy_calibration_mib)
     precs = args.precisions.split(",")
     res = {prec: render_case(prec, args.crossing, args.width, args.height, args.spp, args.steps,
                              args.warmup) for prec in precs}
@@ -308,7 +397,8 @@ def main() -> int:
             "peak": PEAK_TFLOPS["fp32"],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS["fp32"],
-            "traffic": None,
+            "traffic": (FP32C_HBM_BYTES_PER_RAY * R["rays"] / args.steps) if head == "fp32c" else None,
+            "traffic_note": "HBM bytes per launch: measured bytes per ray x rays (rocprofv3 --pmc)",
             "kernel": kernel,
             "flops_per_launch": {"fp32": R["flops32"] / args.steps, "fp64": R["flops64"] / args.steps},
             "note": "FP64 flops count twice (FP32-equivalent) since the FP64 rate is half",
@@ -332,6 +422,9 @@ def main() -> int:
 
     if args.adjoint:
         line["config"]["adjoint"] = bench_adjoint(args)
+    if args.side_workloads:
+        line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000)
+        line["config"]["geomu_batch"] = bench_geomu(1 << 20)
 
     if args.cpu_pixels > 0 and "fp64" in res:
         line["cpu_baseline"] = cpu_baseline(args, res["fp64"])
