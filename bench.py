@@ -255,6 +255,86 @@ def bench_adjoint(args) -> dict:
     }
 
 
+def batch_frames(nframes: int = 64, seed: int = 4096):
+    """The uneven multi-frame workload: nframes frames, resolution drawn from
+    {256^2, 512^2, 768^2, 1024^2}, spin from {0, 0.5, 0.9, 0.998}, theta_obs
+    ~ U[5, 89] deg (seeded); pinhole at r = 50 M, fov 50 deg, adaptive
+    rtol 1e-8, disk ISCO..20 M, q = 3, <= 3 crossings, FP32C, Hermite
+    crossing.  Sky rays take a few hundred steps, photon-ring rays ~10^4."""
+    import numpy as np
+
+    import nullpath_amd as npa
+    rng = np.random.default_rng(seed)
+    res = rng.choice([256, 512, 768, 1024], size=nframes)
+    spins = rng.choice([0.0, 0.5, 0.9, 0.998], size=nframes)
+    thetas = rng.uniform(5.0, 89.0, size=nframes)
+    return [npa.kerr_disk_params(width=int(r), height=int(r), spin=float(a), theta_deg=float(t),
+                                 fov=50.0, r_cam_Rs=25.0, seed=1 + i, precision=1, crossing=1)
+            for i, (r, a, t) in enumerate(zip(res, spins, thetas))]
+
+
+def bench_batch(spp: int = 2) -> dict:
+    """One persistent launch over the 64-frame uneven batch, after the
+    autotuner picked block size, refill chunk, FP32C kernel variant and
+    FP64 radius on a 1/8-resolution copy; beside it the same frames as 64
+    separate single-frame launches with the default launch shape."""
+    import torch
+
+    import nullpath_amd as npa
+    frames = batch_frames()
+    total = sum(f.width * f.height for f in frames)
+    t0 = time.perf_counter()
+    best, trials = npa.autotune_batch(frames, spp=spp, subsample=8)
+    tune_s = time.perf_counter() - t0
+    out = torch.zeros(3 * total, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(10, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    npa.render_batch(frames, out, spp=spp, counters=cnt, tuning=best, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1)
+    c = npa.counters_dict(cnt)
+    # the same frames one launch each, with the tuned kernel variant and FP64 radius
+    for f in frames:
+        f.waves_per_simd = best.waves_per_simd
+        if best.fp64_radius > 0.0:
+            f.fp64_radius = best.fp64_radius
+    outs = [torch.zeros(3 * f.width * f.height, dtype=torch.float32, device="cuda") for f in frames]
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for f, o in zip(frames, outs):
+        npa.render_tile(f, o, spp=spp, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    seq_ms = e0.elapsed_time(e1)
+    same = all(torch.equal(o, out[3 * off:3 * (off + f.width * f.height)])
+               for o, f, off in zip(outs, frames,
+                                    [sum(g.width * g.height for g in frames[:i]) for i in range(len(frames))]))
+    fl = FLOPS["fp32c"]
+    f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + (c["attempts"] - c["attempts_f64"]) * fl["attempt"]
+    f64 = c["rhs_evals_f64"] * fl["rhs64"] + c["attempts_f64"] * fl["attempt64"]
+    achieved = (f32 + 2.0 * f64) / (batch_ms / 1e3) / 1e12
+    return {
+        "workload": f"{len(frames)} frames (256^2..1024^2, a in {{0,0.5,0.9,0.998}}, theta U[5,89] deg), "
+                    f"{spp} spp, r_cam=50M fov 50deg adaptive rtol 1e-8, FP32C, one persistent launch",
+        "rays": total * spp,
+        "rays_per_s": total * spp / (batch_ms / 1e3),
+        "batch_kernel_ms": batch_ms,
+        "per_frame_launches_ms": seq_ms,
+        "speedup_over_per_frame_launches": seq_ms / batch_ms,
+        "outputs_bitwise_equal_to_per_frame": bool(same),
+        "tuning": best.as_dict(),
+        "autotune_s": tune_s,
+        "autotune_trials": [{"tuning": t, "ms": ms} for t, ms in trials],
+        "steps_per_ray_mean": c["steps"] / max(1, c["rays"]),
+        "fp64_attempt_frac": c["attempts_f64"] / max(1, c["attempts"]),
+        "valu_peak_frac": achieved / PEAK_TFLOPS["fp32"],
+    }
+
+
 def bench_schwarzschild(n: int) -> dict:
     """Plain-method FP64 Schwarzschild sweep: M = 10, n equatorial photons
     from r0 = 1000 M, b uniform on [0.5 M, 15 M] (seed 2024), captured at
@@ -423,6 +503,7 @@ def main() -> int:
     if args.adjoint:
         line["config"]["adjoint"] = bench_adjoint(args)
     if args.side_workloads:
+        line["config"]["uneven_batch"] = bench_batch()
         line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000)
         line["config"]["geomu_batch"] = bench_geomu(1 << 20)
 

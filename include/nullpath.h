@@ -86,6 +86,47 @@ int np_render_tile(const np_render_params* p, int y0, int rows, int spp, int sam
 int np_render_accumulate(const np_render_params* p, int spp_done, int samples_per_pass,
                          int passes, int tile_h, float* h_accum, float* inv_max);
 
+/*
+ * Launch-shape choices of the batch renderer, picked by np_autotune_batch.
+ * block_size: 64, 128 or 256 threads; waves_per_simd: FP32C kernel variant
+ * (3 = 168-VGPR cap, 2 = compiler budget); refill_chunk: consecutive pixels
+ * a lane takes per refill; fp64_radius: FP32C radius (absolute units) of
+ * FP64 steps, 0 = each frame's own.
+ */
+typedef struct np_tuning {
+    int32_t block_size;
+    int32_t waves_per_simd;
+    int32_t refill_chunk;
+    double  fp64_radius;
+} np_tuning;
+
+/*
+ * Render nframes whole frames of uneven resolution, spin and inclination in
+ * one persistent launch (an extension: the reference renders one frame per
+ * process, render.cu:523-638).  All frames share one precision and one
+ * integrator; check_inv must be 0.  d_rgb holds the frames back to back
+ * (frame i starts at 3 * sum_{j<i} width_j * height_j floats; colour sums
+ * over spp samples); d_records (optional) likewise with spp records per
+ * pixel; d_counters (optional) sums over all frames.  tuning NULL = block
+ * 256, 3-wave FP32C variant, chunk 1, frame radii.
+ */
+int np_render_batch(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                    float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
+                    const np_tuning* tuning, void* stream);
+
+/*
+ * Time every candidate tuning (block 128/256 x chunk 1/4, and for FP32C the
+ * kernel variant 3/2 x FP64 radius 1.25/1.5/2.0 x the largest disk_rout) on
+ * the batch with each frame's resolution divided by `subsample`, and write
+ * the fastest to *best.  Optional trials / trial_ms (capacity max_trials)
+ * receive every candidate and its kernel time; *n_trials their count.  The
+ * choice is cached per batch signature for the life of the process (a
+ * cached answer reports no trials).  Synchronizes the device.
+ */
+int np_autotune_batch(const np_render_params* frames, int nframes, int spp, int subsample,
+                      np_tuning* best, np_tuning* trials, double* trial_ms, int max_trials,
+                      int* n_trials);
+
 /* ---------------------------------------------------------------------
  * GEOKERR semi-analytic solver (geokerr_port/reference/geokerr_original.f)
  * ------------------------------------------------------------------- */

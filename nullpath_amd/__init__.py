@@ -57,11 +57,12 @@ def isco_radius(a: float) -> float:
 
 def kerr_disk_params(width=1920, height=1080, spin=0.9, theta_deg=80.0, fov=55.0,
                      rtol=1e-8, max_orders=3, q=3.0, gain=1.0, max_steps=200000,
-                     adaptive=True, **kw) -> RenderParams:
-    """Kerr thin-disk still: pinhole at r = 1000 M (M = 1), disk from the
-    ISCO to 20 M with emissivity (r/M)^-q and g^3 redshift."""
+                     adaptive=True, r_cam_Rs=500.0, **kw) -> RenderParams:
+    """Kerr thin-disk still: pinhole at r = 2 r_cam_Rs M (default 1000 M;
+    M = 1), disk from the ISCO to 20 M with emissivity (r/M)^-q and g^3
+    redshift."""
     p = default_params(width=width, height=height, fov_y_deg=fov, mass=1.0, spin=spin,
-                       r_cam_Rs=500.0, theta_cam=deg2rad_f32(theta_deg),
+                       r_cam_Rs=r_cam_Rs, theta_cam=deg2rad_f32(theta_deg),
                        integrator=1 if adaptive else 0, camera=1 if adaptive else 0,
                        rtol=rtol, atol=1e-12, disk_max_orders=max_orders, emiss_q=q,
                        emiss_m_units=1, disk_gain=gain, max_steps=max_steps,
@@ -214,3 +215,41 @@ def render_accumulate(p: RenderParams, accum, spp_done=0, samples_per_pass=1, pa
     code = lib().np_render_accumulate(p, spp_done, samples_per_pass, passes, tile_h,
                                       accum.ctypes.data_as(C.c_void_p), invp)
     check(code, "np_render_accumulate")
+
+
+def render_batch(frames, out, spp=1, sample_offset=0, records=None, counters=None, tuning=None,
+                 stream=None) -> None:
+    """All frames (a list of RenderParams) in one persistent launch
+    (np_render_batch); `out` is one float32 device tensor holding the frames'
+    colour sums back to back."""
+    import ctypes as C
+
+    from ._lib import Tuning
+    n = len(frames)
+    arr = (RenderParams * n)(*frames)
+    total = sum(f.width * f.height for f in frames)
+    if out.numel() < 3 * total or out.dtype.itemsize != 4 or not out.is_contiguous():
+        raise NullPathError("out must be a contiguous float32 device tensor of 3 * total pixels")
+    t = None if tuning is None else C.byref(tuning if isinstance(tuning, Tuning) else Tuning(**tuning))
+    code = lib().np_render_batch(arr, n, spp, sample_offset, out.data_ptr(),
+                                 None if records is None else records.data_ptr(),
+                                 None if counters is None else counters.data_ptr(), t,
+                                 _stream_ptr(stream))
+    check(code, "np_render_batch")
+
+
+def autotune_batch(frames, spp=1, subsample=8, max_trials=64):
+    """np_autotune_batch: returns (best Tuning, [(Tuning dict, ms), ...])."""
+    import ctypes as C
+
+    from ._lib import Tuning
+    n = len(frames)
+    arr = (RenderParams * n)(*frames)
+    best = Tuning()
+    trials = (Tuning * max_trials)()
+    ms = (C.c_double * max_trials)()
+    nt = C.c_int(0)
+    code = lib().np_autotune_batch(arr, n, spp, subsample, C.byref(best), trials, ms, max_trials,
+                                   C.byref(nt))
+    check(code, "np_autotune_batch")
+    return best, [(trials[i].as_dict(), ms[i]) for i in range(nt.value)]

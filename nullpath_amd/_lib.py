@@ -59,6 +59,16 @@ class AdjointParams(C.Structure):
         return q
 
 
+class Tuning(C.Structure):
+    """np_tuning (include/nullpath.h)."""
+
+    _fields_ = [("block_size", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("refill_chunk", C.c_int32), ("fp64_radius", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TraceCounters(C.Structure):
     """np_trace_counters (include/nullpath.h)."""
 
@@ -116,6 +126,12 @@ def _declare(L: C.CDLL) -> None:
     L.np_precompute_geodesics.restype = i32
     L.np_free_lookup_table.argtypes = [C.POINTER(LookupTable)]
     L.np_free_lookup_table.restype = None
+    prp, pt = C.POINTER(RenderParams), C.POINTER(Tuning)
+    L.np_render_batch.argtypes = [prp, i32, i32, i32, vp, vp, vp, pt, vp]
+    L.np_render_batch.restype = i32
+    L.np_autotune_batch.argtypes = [prp, i32, i32, i32, pt, pt, C.POINTER(C.c_double), i32,
+                                    C.POINTER(C.c_int)]
+    L.np_autotune_batch.restype = i32
     pa = C.POINTER(AdjointParams)
     L.np_adjoint_workspace_bytes.argtypes = [pa]
     L.np_adjoint_workspace_bytes.restype = i64

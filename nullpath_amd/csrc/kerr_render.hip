@@ -1077,41 +1077,70 @@ struct StepF32C {
 };
 
 // ---------------------------------------------------------------------------
-// Persistent renderer: work item = pixel, lane-level refill, one integrator
-// attempt per loop iteration.
+// Persistent renderer: work item = pixel (or a chunk of consecutive pixels),
+// lane-level refill, one integrator attempt per loop iteration.
+//
+// Single-frame mode renders rows [y0, y0+rows) of the frame F0.  Batch mode
+// renders whole frames from a device table; a wave works on one frame at a
+// time (so the frame constants stay wave-uniform and are read with scalar
+// loads), refilling its lanes from that frame's work counter, and moves to
+// the next frame with work left once all of its lanes are idle.  Waves start
+// staggered over the frames, so the drain at a frame switch is rare.
 // ---------------------------------------------------------------------------
-template <class S, bool ADAPTIVE, bool INV>
+template <class S, bool ADAPTIVE, bool INV, bool BATCH>
 __device__ __forceinline__ void
-kerr_render_body(const FrameConst& F, const int y0, const int rows, const int spp,
-                 const int sample_offset, float* __restrict__ out, const int accumulate,
-                 float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
-                 np_ray_record* __restrict__ records, unsigned long long* __restrict__ work) {
-    const int W = F.width;
-    const int y_end = min(y0 + rows, F.height);
-    const uint64_t npix = (y_end > y0) ? (uint64_t)(y_end - y0) * (uint64_t)W : 0ull;
-
+kerr_body(const FrameConst& F0, const BatchFrame* __restrict__ frames, const int nframes,
+          const int y0, const int rows, const int spp, const int sample_offset,
+          float* __restrict__ out_base, const int accumulate, float* __restrict__ inv_max3,
+          np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records_base,
+          unsigned long long* __restrict__ work, const int chunk) {
+    int fid = 0;
+    if (BATCH) {
+        const int wave_global = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+        fid = __builtin_amdgcn_readfirstlane(wave_global % nframes);
+    }
     typename S::Ray R;
     bool has = false, done = false;
-    uint64_t pix = 0;
+    uint64_t pix = 0, chunk_end = 0;
     int sample = 0;
     float acc[3] = {0.0f, 0.0f, 0.0f};
     float inv_lane[3] = {0.0f, 0.0f, 0.0f};
     // per-lane counters, flushed to the u64 totals before they can wrap
     uint32_t c_rhs = 0, c_att = 0, c_steps = 0, c_rays = 0, c_cap = 0, c_esc = 0, c_cap_steps = 0,
              c_disk = 0, c_rhs64 = 0, c_att64 = 0;
-    if (spp <= 0 || F.max_steps <= 0) done = true;
+    if (spp <= 0) done = true;
 
     while (true) {
+        const FrameConst& F = BATCH ? frames[fid].F : F0;
+        const int W = F.width;
+        const int ybase = BATCH ? 0 : y0;
+        uint64_t npix;
+        if (BATCH) {
+            npix = frames[fid].npix;
+        } else {
+            const int y_end = min(y0 + rows, F.height);
+            npix = (y_end > y0) ? (uint64_t)(y_end - y0) * (uint64_t)W : 0ull;
+        }
+        if (F.max_steps <= 0) done = true;
         const bool need = !has && !done;
         if (__ballot(need)) {
-            const uint64_t idx = wave_fetch(need, work);
+            const bool in_chunk = need && (pix + 1 < chunk_end);
+            const bool fetch = need && !in_chunk;
+            const uint64_t unit = wave_fetch(fetch, work + (BATCH ? fid : 0));
             if (need) {
-                if (idx < npix) {
-                    pix = idx;
+                uint64_t p;
+                if (in_chunk) {
+                    p = pix + 1;
+                } else {
+                    p = unit * (uint64_t)chunk;
+                    chunk_end = min(p + (uint64_t)chunk, npix);
+                }
+                if (p < npix) {
+                    pix = p;
                     sample = 0;
                     acc[0] = acc[1] = acc[2] = 0.0f;
                     const int x = (int)(pix % (uint64_t)W);
-                    const int yy = y0 + (int)(pix / (uint64_t)W);
+                    const int yy = ybase + (int)(pix / (uint64_t)W);
                     float n_r, n_th, n_ph, cx, cy;
                     camera_dir(F, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
                     S::start(F, R, n_r, n_th, n_ph, cx, cy);
@@ -1121,7 +1150,26 @@ kerr_render_body(const FrameConst& F, const int y0, const int rows, const int sp
                 }
             }
         }
-        if (__ballot(has) == 0) break;
+        if (__ballot(has) == 0) {
+            if (!BATCH) break;
+            // every lane is idle: move to the next frame that still has work
+            int next = -1;
+            for (int k = 1; k <= nframes; ++k) {
+                const int f = (fid + k) % nframes;
+                const unsigned long long taken =
+                    __hip_atomic_load(work + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (taken * (unsigned long long)chunk < frames[f].npix) {
+                    next = f;
+                    break;
+                }
+            }
+            if (next < 0) break;
+            fid = __builtin_amdgcn_readfirstlane(next);
+            done = (spp <= 0);
+            pix = 0;
+            chunk_end = 0;
+            continue;
+        }
         if (counters && __ballot(c_rhs > 0x40000000u || c_steps > 0x40000000u)) {
             wave_add_u64(&counters->steps, c_steps);
             wave_add_u64(&counters->attempts, c_att);
@@ -1158,7 +1206,7 @@ kerr_render_body(const FrameConst& F, const int y0, const int rows, const int sp
                 c_esc += (cls == NP_RAY_ESCAPED);
                 c_cap_steps += (cls == NP_RAY_STEPCAP);
                 c_disk += (R.orders > 0);
-                if (records) {
+                if (records_base) {
                     np_ray_record rec;
                     rec.cls = cls;
                     rec.orders = R.orders;
@@ -1166,17 +1214,18 @@ kerr_render_body(const FrameConst& F, const int y0, const int rows, const int sp
                     rec.attempts = R.attempts_total;
                     rec.min_r = R.min_r;
                     rec.r_end = S::r(R);
-                    records[pix * (uint64_t)spp + (uint64_t)sample] = rec;
+                    const uint64_t r0 = BATCH ? frames[fid].rec_off : 0ull;
+                    records_base[r0 + pix * (uint64_t)spp + (uint64_t)sample] = rec;
                 }
                 sample += 1;
                 const int x = (int)(pix % (uint64_t)W);
-                const int yy = y0 + (int)(pix / (uint64_t)W);
+                const int yy = ybase + (int)(pix / (uint64_t)W);
                 if (sample < spp) {
                     float n_r, n_th, n_ph, cx, cy;
                     camera_dir(F, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
                     S::start(F, R, n_r, n_th, n_ph, cx, cy);
                 } else {
-                    float* o = out + 3 * pix;
+                    float* o = out_base + (BATCH ? frames[fid].out_off : 0ull) + 3 * pix;
                     if (accumulate) {
                         o[0] += acc[0];
                         o[1] += acc[1];
@@ -1214,8 +1263,15 @@ kerr_render_body(const FrameConst& F, const int y0, const int rows, const int sp
     const FrameConst F, const int y0, const int rows, const int spp, const int sample_offset,    \
         float* __restrict__ out, const int accumulate, float* __restrict__ inv_max3,            \
         np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records,          \
-        unsigned long long* __restrict__ work
-#define NP_KERNEL_PASS F, y0, rows, spp, sample_offset, out, accumulate, inv_max3, counters, records, work
+        unsigned long long* __restrict__ work, const int chunk
+#define NP_KERNEL_PASS F, nullptr, 0, y0, rows, spp, sample_offset, out, accumulate, inv_max3, \
+        counters, records, work, chunk
+#define NP_BATCH_ARGS                                                                           \
+    const BatchFrame* __restrict__ frames, const int nframes, const int spp,                    \
+        const int sample_offset, float* __restrict__ out, np_trace_counters* __restrict__ counters, \
+        np_ray_record* __restrict__ records, unsigned long long* __restrict__ work, const int chunk
+#define NP_BATCH_PASS frames[0].F, frames, nframes, 0, 0, spp, sample_offset, out, 0, nullptr, \
+        counters, records, work, chunk
 
 // Kernel entry points.  Variant 0 leaves the register budget to the
 // compiler; variant 1 (FP32C only) asks for 3 waves per SIMD, which caps
@@ -1223,17 +1279,30 @@ kerr_render_body(const FrameConst& F, const int y0, const int rows, const int sp
 // branch to scratch.
 template <class S, bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
-    kerr_render_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
+    kerr_body<S, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
 }
 
 template <bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 kerr_render_f32c_w3(NP_KERNEL_ARGS) {
-    kerr_render_body<StepF32C, ADAPTIVE, INV>(NP_KERNEL_PASS);
+    kerr_body<StepF32C, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
+}
+
+template <class S, bool ADAPTIVE>
+__global__ void __launch_bounds__(256) kerr_batch(NP_BATCH_ARGS) {
+    kerr_body<S, ADAPTIVE, false, true>(NP_BATCH_PASS);
+}
+
+template <bool ADAPTIVE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+kerr_batch_f32c_w3(NP_BATCH_ARGS) {
+    kerr_body<StepF32C, ADAPTIVE, false, true>(NP_BATCH_PASS);
 }
 
 #undef NP_KERNEL_ARGS
 #undef NP_KERNEL_PASS
+#undef NP_BATCH_ARGS
+#undef NP_BATCH_PASS
 
 template <class S, bool A>
 const void* kernel_ptr(bool inv) {
@@ -1255,23 +1324,52 @@ const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
     return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
 }
 
+const void* select_batch_kernel(int precision, bool adaptive, int variant) {
+    if (precision == NP_PREC_FP32C) {
+        if (variant == 1)
+            return adaptive ? reinterpret_cast<const void*>(&kerr_batch_f32c_w3<true>)
+                            : reinterpret_cast<const void*>(&kerr_batch_f32c_w3<false>);
+        return adaptive ? reinterpret_cast<const void*>(&kerr_batch<StepF32C, true>)
+                        : reinterpret_cast<const void*>(&kerr_batch<StepF32C, false>);
+    }
+    return adaptive ? reinterpret_cast<const void*>(&kerr_batch<StepF64, true>)
+                    : reinterpret_cast<const void*>(&kerr_batch<StepF64, false>);
+}
+
 }  // namespace
 
 int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int rows, int spp,
                 int sample_offset, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
-                unsigned long long* d_work, hipStream_t stream, int grid) {
+                unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
     const void* fn = select_kernel(precision, adaptive, F.check_inv != 0, F.variant);
     FrameConst Fc = F;
     void* args[] = {&Fc, &y0, &rows, &spp, &sample_offset, &d_out, &accumulate, &d_inv,
-                    &d_counters, &d_records, &d_work};
-    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, stream);
+                    &d_counters, &d_records, &d_work, &chunk};
+    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
     return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
 }
 
-int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int* blocks_per_cu) {
+int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu) {
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, select_kernel(precision, adaptive, inv, variant), 256, 0);
+        blocks_per_cu, select_kernel(precision, adaptive, inv, variant), block, 0);
+    return e == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int launch_kerr_batch(const BatchFrame* d_frames, int nframes, int precision, bool adaptive,
+                      int variant, int spp, int sample_offset, float* d_out,
+                      np_trace_counters* d_counters, np_ray_record* d_records,
+                      unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
+    const void* fn = select_batch_kernel(precision, adaptive, variant);
+    void* args[] = {&d_frames, &nframes, &spp, &sample_offset, &d_out, &d_counters, &d_records,
+                    &d_work, &chunk};
+    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+    return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
+}
+
+int occupancy_kerr_batch(int precision, bool adaptive, int variant, int block, int* blocks_per_cu) {
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, select_batch_kernel(precision, adaptive, variant), block, 0);
     return e == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

@@ -72,12 +72,10 @@ int take_slot(DeviceState* d, hipStream_t stream, unsigned long long** slot) {
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
-int persistent_grid(DeviceState* d, int precision, bool adaptive, bool inv, int variant,
-                    uint64_t items) {
-    int per_cu = 0;
-    if (occupancy_kerr(precision, adaptive, inv, variant, &per_cu) != NP_OK || per_cu <= 0) per_cu = 1;
+int persistent_grid(DeviceState* d, int per_cu, int block, uint64_t items) {
+    if (per_cu <= 0) per_cu = 1;
     uint64_t grid = (uint64_t)per_cu * (uint64_t)d->cus;
-    uint64_t need = (items + 255) / 256;
+    uint64_t need = (items + (uint64_t)block - 1) / (uint64_t)block;
     if (need < grid) grid = need;
     if (grid < 1) grid = 1;
     return (int)grid;
@@ -100,9 +98,197 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     err = take_slot(d, stream, &slot);
     if (err != NP_OK) return err;
     const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
-    const int grid = persistent_grid(d, p->precision, adaptive, p->check_inv != 0, F.variant, npix);
+    const int block = 256;
+    int per_cu = 0;
+    if (occupancy_kerr(p->precision, adaptive, p->check_inv != 0, F.variant, block, &per_cu) != NP_OK)
+        per_cu = 1;
+    const int grid = persistent_grid(d, per_cu, block, npix);
     return launch_kerr(F, p->precision, adaptive, y0, rows, spp, sample_offset, d_rgb, accumulate,
-                       d_inv, d_counters, d_records, slot, stream, grid);
+                       d_inv, d_counters, d_records, slot, stream, grid, block, 1);
+}
+
+__global__ void zero_u64_n(unsigned long long* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+// The frames of a batch, checked and laid out: one precision and one
+// integrator for all frames, no invariant monitor.
+int batch_frames(const np_render_params* frames, int nframes, int spp, const np_tuning* t,
+                 std::vector<BatchFrame>& out, uint64_t* total_px) {
+    if (!frames || nframes <= 0 || spp < 0) return NP_EINVAL;
+    out.resize((size_t)nframes);
+    uint64_t off = 0;
+    for (int i = 0; i < nframes; ++i) {
+        np_render_params q = frames[i];
+        if (q.precision != frames[0].precision || q.integrator != frames[0].integrator) return NP_EINVAL;
+        if (q.precision != NP_PREC_FP64 && q.precision != NP_PREC_FP32C) return NP_EINVAL;
+        if (q.check_inv) return NP_EINVAL;
+        if (t) {
+            q.waves_per_simd = t->waves_per_simd;
+            if (t->fp64_radius > 0.0) q.fp64_radius = t->fp64_radius;
+        }
+        int err = make_frame(&q, &out[i].F);
+        if (err != NP_OK) return err;
+        out[i].npix = (uint64_t)q.width * (uint64_t)q.height;
+        out[i].out_off = 3 * off;
+        out[i].rec_off = off * (uint64_t)spp;
+        off += out[i].npix;
+    }
+    *total_px = off;
+    return NP_OK;
+}
+
+int batch_launch(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                 float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
+                 const np_tuning* tuning, hipStream_t stream) {
+    if (!d_rgb) return NP_EINVAL;
+    np_tuning t = tuning ? *tuning : np_tuning{256, 3, 1, 0.0};
+    if (t.block_size != 64 && t.block_size != 128 && t.block_size != 256) return NP_EINVAL;
+    if (t.refill_chunk < 1 || t.refill_chunk > 1024) return NP_EINVAL;
+    std::vector<BatchFrame> host;
+    uint64_t total_px = 0;
+    int err = batch_frames(frames, nframes, spp, &t, host, &total_px);
+    if (err != NP_OK) return err;
+    DeviceState* d = device_state(&err);
+    if (!d) return err;
+    if (total_px == 0 || spp == 0) return NP_OK;
+    const size_t table = host.size() * sizeof(BatchFrame);
+    const size_t bytes = table + (size_t)nframes * sizeof(unsigned long long);
+    void* buf = nullptr;
+    bool stream_ordered = true;
+    if (hipMallocAsync(&buf, bytes, stream) != hipSuccess) {
+        (void)hipGetLastError();
+        stream_ordered = false;
+        if (hipMalloc(&buf, bytes) != hipSuccess) return NP_EALLOC;
+    }
+    BatchFrame* d_frames = (BatchFrame*)buf;
+    unsigned long long* d_work = (unsigned long long*)((char*)buf + table);
+    err = NP_OK;
+    if (hipMemcpyAsync(d_frames, host.data(), table, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)   // host table released on return
+        err = NP_ELAUNCH;
+    if (err == NP_OK) {
+        hipLaunchKernelGGL(zero_u64_n, dim3(1), dim3(256), 0, stream, d_work, nframes);
+        if (hipGetLastError() != hipSuccess) err = NP_ELAUNCH;
+    }
+    if (err == NP_OK) {
+        const bool adaptive = frames[0].integrator == NP_INTEGRATOR_ADAPTIVE;
+        const int variant = host[0].F.variant;
+        int per_cu = 0;
+        if (occupancy_kerr_batch(frames[0].precision, adaptive, variant, t.block_size, &per_cu) != NP_OK)
+            per_cu = 1;
+        const int grid = persistent_grid(d, per_cu, t.block_size, total_px);
+        err = launch_kerr_batch(d_frames, nframes, frames[0].precision, adaptive, variant, spp,
+                                sample_offset, d_rgb, d_counters, d_records, d_work, stream, grid,
+                                t.block_size, t.refill_chunk);
+    }
+    if (stream_ordered) {
+        if (hipFreeAsync(buf, stream) != hipSuccess && err == NP_OK) err = NP_ELAUNCH;
+    } else {
+        // without stream-ordered allocation the table must outlive the kernel
+        if (hipStreamSynchronize(stream) != hipSuccess && err == NP_OK) err = NP_ELAUNCH;
+        (void)hipFree(buf);
+    }
+    return err;
+}
+
+// Autotuner: times every candidate tuning on a subsampled copy of the batch
+// (each frame's resolution divided by `subsample`, same field of view, so
+// the mix of short sky rays and long photon-ring rays is kept) and returns
+// the fastest.  Results are cached per batch signature for the process.
+std::mutex g_tune_mu;
+std::vector<std::pair<std::vector<double>, np_tuning>> g_tune_cache;
+
+std::vector<double> batch_signature(const np_render_params* f, int n, int spp, int sub) {
+    std::vector<double> k = {(double)n, (double)spp, (double)sub};
+    for (int i = 0; i < n; ++i) {
+        const np_render_params& q = f[i];
+        const double v[] = {(double)q.width, (double)q.height, q.fov_y_deg, q.theta_cam, q.spin,
+                            q.mass, q.r_cam_Rs, (double)q.integrator, (double)q.precision, q.rtol,
+                            q.disk_rin, q.disk_rout, (double)q.disk_max_orders, (double)q.crossing};
+        k.insert(k.end(), v, v + sizeof(v) / sizeof(v[0]));
+    }
+    return k;
+}
+
+int autotune_batch(const np_render_params* frames, int nframes, int spp, int subsample,
+                   np_tuning* best, np_tuning* trials, double* trial_ms, int max_trials,
+                   int* n_trials) {
+    if (!frames || nframes <= 0 || spp <= 0 || subsample < 1 || !best) return NP_EINVAL;
+    if (n_trials) *n_trials = 0;
+    const std::vector<double> sig = batch_signature(frames, nframes, spp, subsample);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto& e : g_tune_cache)
+            if (e.first == sig) {
+                *best = e.second;
+                return NP_OK;
+            }
+    }
+    std::vector<np_render_params> sub(frames, frames + nframes);
+    double rout_max = 0.0;
+    for (auto& q : sub) {
+        q.width = std::max(8, q.width / subsample);
+        q.height = std::max(8, q.height / subsample);
+        rout_max = std::max(rout_max, q.disk_rout);
+    }
+    const bool fp32c = frames[0].precision == NP_PREC_FP32C;
+    std::vector<np_tuning> cand;
+    const int blocks[] = {128, 256};
+    const int chunks[] = {1, 4};
+    const int variants32[] = {3, 2};
+    const double radii32[] = {1.25, 1.5, 2.0};
+    for (int b : blocks)
+        for (int c : chunks) {
+            if (fp32c) {
+                for (int v : variants32)
+                    for (double r : radii32) cand.push_back(np_tuning{b, v, c, r * rout_max});
+            } else {
+                cand.push_back(np_tuning{b, 0, c, 0.0});
+            }
+        }
+    uint64_t total = 0;
+    for (const auto& q : sub) total += (uint64_t)q.width * (uint64_t)q.height;
+    float* d_rgb = nullptr;
+    if (hipMalloc(&d_rgb, total * 3 * sizeof(float)) != hipSuccess) return NP_EALLOC;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int err = NP_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) err = NP_ELAUNCH;
+    // one untimed launch loads the kernels
+    if (err == NP_OK)
+        err = batch_launch(sub.data(), nframes, spp, 0, d_rgb, nullptr, nullptr, &cand[0], nullptr);
+    if (err == NP_OK && hipDeviceSynchronize() != hipSuccess) err = NP_ELAUNCH;
+    double best_ms = 1e300;
+    int n = 0;
+    for (const auto& t : cand) {
+        if (err != NP_OK) break;
+        if (hipEventRecord(e0, nullptr) != hipSuccess) { err = NP_ELAUNCH; break; }
+        err = batch_launch(sub.data(), nframes, spp, 0, d_rgb, nullptr, nullptr, &t, nullptr);
+        if (err != NP_OK) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            err = NP_ELAUNCH;
+            break;
+        }
+        if (n < max_trials) {
+            if (trials) trials[n] = t;
+            if (trial_ms) trial_ms[n] = ms;
+        }
+        ++n;
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = t;
+        }
+    }
+    if (n_trials) *n_trials = std::min(n, max_trials);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_rgb);
+    if (err != NP_OK) return err;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache.emplace_back(sig, *best);
+    return NP_OK;
 }
 
 }  // namespace
@@ -173,6 +359,20 @@ int np_render_accumulate(const np_render_params* p, int spp_done, int samples_pe
     (void)hipFree(d_acc);
     (void)hipFree(d_inv);
     return err;
+}
+
+int np_render_batch(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                    float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
+                    const np_tuning* tuning, void* stream) {
+    return np::batch_launch(frames, nframes, spp, sample_offset, d_rgb, d_records, d_counters,
+                            tuning, (hipStream_t)stream);
+}
+
+int np_autotune_batch(const np_render_params* frames, int nframes, int spp, int subsample,
+                      np_tuning* best, np_tuning* trials, double* trial_ms, int max_trials,
+                      int* n_trials) {
+    return np::autotune_batch(frames, nframes, spp, subsample, best, trials, trial_ms, max_trials,
+                              n_trials);
 }
 
 }  // extern "C"

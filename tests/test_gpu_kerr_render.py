@@ -130,3 +130,48 @@ def test_fp32c_kernel_variants_agree(device):
     assert np.array_equal(i3, i2)
     assert np.array_equal(r3, r2)
     assert c3 == c2
+
+
+def _batch_frames(precision):
+    spins = [0.0, 0.5, 0.9, 0.998]
+    sizes = [(40, 24), (32, 32), (48, 20), (24, 36)]
+    thetas = [20.0, 45.0, 70.0, 88.0]
+    return [npa.kerr_disk_params(width=w, height=h, spin=a, theta_deg=t, fov=30.0, seed=3,
+                                 precision=precision, crossing=1, r_cam_Rs=25.0)
+            for (w, h), a, t in zip(sizes, spins, thetas)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("tuning", [None, {"block_size": 128, "waves_per_simd": 2,
+                                           "refill_chunk": 4, "fp64_radius": 0.0}])
+def test_batch_equals_per_frame_renders(device, precision, tuning):
+    """One persistent launch over frames of different size, spin and
+    inclination gives, frame by frame, exactly the single-frame render."""
+    frames = _batch_frames(precision)
+    total = sum(f.width * f.height for f in frames)
+    out = torch.zeros(3 * total, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(10, dtype=torch.int64, device="cuda")
+    print(f"[gpu] np_render_batch precision={precision} tuning={tuning}", flush=True)
+    npa.render_batch(frames, out, spp=2, counters=cnt, tuning=tuning)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    off = 0
+    for f in frames:
+        ref, c, _ = _run_gpu(f, 2)
+        n = f.width * f.height
+        np.testing.assert_array_equal(got[3 * off:3 * (off + n)].reshape(-1, 3), ref)
+        off += n
+    assert npa.counters_dict(cnt)["rays"] == 2 * total
+
+
+@pytest.mark.gpu
+def test_autotune_batch_reports_candidates(device):
+    frames = _batch_frames(1)
+    best, trials = npa.autotune_batch(frames, spp=1, subsample=2)
+    assert len(trials) == 24
+    assert all(ms > 0 for _, ms in trials)
+    fastest = min(trials, key=lambda t: t[1])[0]
+    assert best.as_dict() == fastest
+    best2, trials2 = npa.autotune_batch(frames, spp=1, subsample=2)
+    assert best2.as_dict() == best.as_dict() and trials2 == []
