@@ -116,7 +116,7 @@ int np_render_batch(const np_render_params* frames, int nframes, int spp, int sa
 
 /*
  * Time every candidate tuning (block 128/256 x chunk 1/4, and for FP32C the
- * kernel variant 3/2 x FP64 radius 1.25/1.5/2.0 x the largest disk_rout) on
+ * kernel variant 3/2 x FP64 radius 0.25/0.5/1.5 x the largest disk_rout) on
  * the batch with each frame's resolution divided by `subsample`, and write
  * the fastest to *best.  Optional trials / trial_ms (capacity max_trials)
  * receive every candidate and its kernel time; *n_trials their count.  The
