@@ -290,6 +290,25 @@ void orc_calcimusym(double a, double mneg, double mpos, double mu0, double muplu
     *imum = *imum / fac;
 }
 
+/* The RF term of calcimuasymf (geokerr_original.f:3175-3204): the only
+ * output GEOMU keeps (RFFMU2) for GEOPHITIME. */
+double orc_calcimuasymf_rf(double mneg, double mpos, double muf) {
+    double m1 = mneg / mpos;
+    double sf = sqrt((mpos - muf * muf) / (mpos - mneg));
+    double ak = sqrt(1.0 - m1);
+    double qf = (1.0 - sf * ak) * (1.0 + sf * ak);
+    return orc_rf(1.0 - sf * sf, qf, 1.0);
+}
+
+/* The RF term of calcimusymf (geokerr_original.f:3244-3274). */
+double orc_calcimusymf_rf(double mneg, double mpos, double muf, double muplus) {
+    double m1 = -mneg / (mpos - mneg);
+    double sf = sin(acos(muf / muplus));
+    double ak = sqrt(1.0 - m1);
+    double qf = (1.0 - sf * ak) * (1.0 + sf * ak);
+    return orc_rf(1.0 - sf * sf, qf, 1.0);
+}
+
 /* SNCNDN (geokerr_original.f:3277-3340), Numerical Recipes. */
 void orc_sncndn(double uu, double emmc, double* sn, double* cn, double* dn) {
     const double CA = 3e-8;
@@ -435,10 +454,19 @@ void orc_zroots(const double complex* a, int m, double complex* roots, int polis
     }
 }
 
-/* ---------------- GEOMU (geokerr_original.f:771-1186) ---------------- */
-
-void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
-               double su, double sm, orc_geomu_out* o) {
+/* ---------------- GEOMU (geokerr_original.f:771-1186) ----------------
+ *
+ * Arguments the Fortran passes by reference are pointers here: u0 and uf
+ * (clamped to the turning point, or u0 moved out of a forbidden region with
+ * a WARNING), tpm (left untouched when q2 = 0) and tpr (zeroed for equal
+ * roots).  GEOKERR's variables shared between calls live in *v: with
+ * firstpt = 0 the quartic roots, ncase, h1, the u0 piece iu0/rffu0 of the
+ * Table-1 rows 1, 2, 7, 8 and the mu0 pieces i1mu/i3mu/rffmu1/rffmu3 are
+ * read from it instead of recomputed.  pht = 1 also computes rffmu2
+ * (calcimu*f) for GEOPHITIME. */
+void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a, double l,
+                  double q2, int32_t* tpmp, int32_t* tprp, double su, double sm, int pht,
+                  int firstpt, double* mufp, double* iup, int32_t* warnedp) {
     const double one = 1.0, two = 2.0, third = 0.3333333333333333;
     double pi = acos(-one);
     double uplus = one / (one + sqrt(one - a * a));
@@ -447,10 +475,11 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
     double cc = a * a - q2 - l2;
     double dd = two * ((a - l) * (a - l) + q2);
     double ee = -a * a * q2;
-    double iu = 0.0, iu0 = 0.0, iu1 = 0.0, rffu0 = 0.0, rffu1 = 0.0, muf = 0.0;
-    double u1 = 0.0, u2 = 0.0, u3 = 0.0, u4 = 0.0, i1mu = 0.0, i3mu = 0.0, rffmu1 = 0.0, rffmu3 = 0.0;
-    int tpm = 0, ncase = 0, warned = 0, p[5];
-    o->h1 = 0.0;
+    double u0 = *u0p, uf = *ufp;
+    int tpm = *tpmp, tpr = *tprp;
+    double iu = 0.0, iu1 = 0.0, muf = 0.0;
+    int ncase = v->ncase, warned = 0, p[5];
+    double u1 = v->u1, u2 = v->u2, u3 = v->u3, u4 = v->u4;
 
     if (ee == 0.0 && dd != 0.0) {
         p[0] = -1; p[1] = -1; p[2] = -1; p[3] = 0; p[4] = 0;
@@ -467,18 +496,22 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
                 if (u0 <= u2) {
                     if (uf > u2) uf = u2;
                     ncase = 1;
-                    if (u0 != u2) iu0 = orc_ellcubicreal(p, -u1, one, u2, -one, u3, -one, 0.0, 0.0, &rffu0, u0, u2);
-                    else iu0 = 0.0;
-                    if (uf != u2) iu1 = orc_ellcubicreal(p, -u1, one, u2, -one, u3, -one, 0.0, 0.0, &rffu1, uf, u2);
-                    iu = su * (iu0 - neg1pow(tpr) * iu1) / sqrt(dd);
+                    if (firstpt && u0 != u2)
+                        v->iu0 = orc_ellcubicreal(p, -u1, one, u2, -one, u3, -one, 0.0, 0.0, &v->rffu0, u0, u2);
+                    else if (u0 == u2)
+                        v->iu0 = 0.0;
+                    if (uf != u2) iu1 = orc_ellcubicreal(p, -u1, one, u2, -one, u3, -one, 0.0, 0.0, &v->rffu1, uf, u2);
+                    iu = su * (v->iu0 - neg1pow(tpr) * iu1) / sqrt(dd);
                     break;
                 } else if (u0 >= u3) {
                     if (uf < u3) uf = u3;
                     ncase = 2;
-                    if (u0 != u3) iu0 = -orc_ellcubicreal(p, -u1, one, -u2, one, -u3, one, 0.0, 0.0, &rffu0, u3, u0);
-                    else iu0 = 0.0;
-                    if (uf != u3) iu1 = -orc_ellcubicreal(p, -u1, one, -u2, one, -u3, one, 0.0, 0.0, &rffu1, u3, uf);
-                    iu = su * (iu0 - neg1pow(tpr) * iu1) / sqrt(dd);
+                    if (firstpt && u0 != u3)
+                        v->iu0 = -orc_ellcubicreal(p, -u1, one, -u2, one, -u3, one, 0.0, 0.0, &v->rffu0, u3, u0);
+                    else if (u0 == u3)
+                        v->iu0 = 0.0;
+                    if (uf != u3) iu1 = -orc_ellcubicreal(p, -u1, one, -u2, one, -u3, one, 0.0, 0.0, &v->rffu1, u3, uf);
+                    iu = su * (v->iu0 - neg1pow(tpr) * iu1) / sqrt(dd);
                     break;
                 } else {
                     /* forbidden region: the reference prints a warning and moves u0 */
@@ -519,8 +552,8 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             u1 = (aa + bb) - cc / dd / 3.0;
             double f = -one / dd / u1;
             double g = f / u1;
-            if (uf > u0) iu = su * orc_ellcubiccomplex(p, -u1, one, 0.0, 0.0, f, g, one, &rffu0, u0, uf) / sqrt(dd);
-            else if (uf < u0) iu = -su * orc_ellcubiccomplex(p, -u1, one, 0.0, 0.0, f, g, one, &rffu0, uf, u0) / sqrt(dd);
+            if (uf > u0) iu = su * orc_ellcubiccomplex(p, -u1, one, 0.0, 0.0, f, g, one, &v->rffu0, u0, uf) / sqrt(dd);
+            else if (uf < u0) iu = -su * orc_ellcubiccomplex(p, -u1, one, 0.0, 0.0, f, g, one, &v->rffu0, uf, u0) / sqrt(dd);
             else iu = 0.0;
         }
         if (q2 == 0.0) {
@@ -535,8 +568,9 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             double a1 = sm;
             double muplus = sqrt(q2 / ql2);
             if (mu0 > muplus) muplus = mu0;
-            i1mu = acos(mu0 / muplus) / sqrt(ql2);
-            i3mu = pi / sqrt(ql2);
+            v->i1mu = acos(mu0 / muplus) / sqrt(ql2);
+            v->i3mu = pi / sqrt(ql2);
+            double i1mu = v->i1mu, i3mu = v->i3mu;
             if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0);
             else tpm = f_int((iu + i1mu) / i3mu);
             double a2 = sm * neg1pow(tpm);
@@ -550,26 +584,28 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
         muf = 0.0;
     } else {
         p[0] = -1; p[1] = -1; p[2] = -1; p[3] = -1; p[4] = 0;
-        double complex c[5], root[4];
-        c[0] = one; c[1] = 0.0; c[2] = cc; c[3] = dd; c[4] = ee;
-        orc_zroots(c, 4, root, 1);
-        int nreal = 0;
-        for (int i = 0; i < 4; ++i)
-            if (cimag(root[i]) == 0.0) nreal++;
-        if (nreal == 2) {
-            ncase = 5;
-            u1 = creal(root[0]);
-            u4 = (cimag(root[1]) == 0.0) ? creal(root[1]) : creal(root[3]);
-        } else if (nreal == 0) {
-            ncase = 6;
-        } else {
-            u1 = creal(root[0]); u2 = creal(root[1]); u3 = creal(root[2]); u4 = creal(root[3]);
-            for (;;) {
-                if (u2 > uplus && u3 > uplus) { ncase = 5; break; }
-                else if (u0 <= u2) { ncase = 7; break; }
-                else if (u0 >= u3) { ncase = 8; break; }
-                warned = 1;
-                u0 = (su == 1.0) ? 0.0 : uplus;
+        if (firstpt) {
+            double complex c[5], root[4];
+            c[0] = one; c[1] = 0.0; c[2] = cc; c[3] = dd; c[4] = ee;
+            orc_zroots(c, 4, root, 1);
+            int nreal = 0;
+            for (int i = 0; i < 4; ++i)
+                if (cimag(root[i]) == 0.0) nreal++;
+            if (nreal == 2) {
+                ncase = 5;
+                u1 = creal(root[0]);
+                u4 = (cimag(root[1]) == 0.0) ? creal(root[1]) : creal(root[3]);
+            } else if (nreal == 0) {
+                ncase = 6;
+            } else {
+                u1 = creal(root[0]); u2 = creal(root[1]); u3 = creal(root[2]); u4 = creal(root[3]);
+                for (;;) {
+                    if (u2 > uplus && u3 > uplus) { ncase = 5; break; }
+                    else if (u0 <= u2) { ncase = 7; break; }
+                    else if (u0 >= u3) { ncase = 8; break; }
+                    warned = 1;
+                    u0 = (su == 1.0) ? 0.0 : uplus;
+                }
             }
         }
         if (ncase == 5) {
@@ -577,12 +613,14 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             double f = -qs * one / fabs(ee) / u1 / u4;
             double g = (u4 + u1) / u1 / u4 * f;
             if (uf > u0)
-                iu = su * orc_ellquarticcomplex(p, -u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, &rffu0, u0, uf) / sqrt(fabs(ee));
+                iu = su * orc_ellquarticcomplex(p, -u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, &v->rffu0, u0, uf) / sqrt(fabs(ee));
             else if (uf < u0)
-                iu = -su * orc_ellquarticcomplex(p, -u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, &rffu0, uf, u0) / sqrt(fabs(ee));
+                iu = -su * orc_ellquarticcomplex(p, -u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, &v->rffu0, uf, u0) / sqrt(fabs(ee));
             else
                 iu = 0.0;
         } else if (ncase == 6) {
+            /* The sextic for h1 (Eq. 21) sits outside the Fortran's FIRSTPT
+             * test, so it is solved on every call. */
             double complex coefs[7], hroots[6];
             double se = sqrt(ee);
             coefs[0] = one; coefs[1] = -cc / se; coefs[2] = -one;
@@ -593,41 +631,45 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             /* first real root of the sextic (the Fortran loop has no bound) */
             for (int i = 0; i < 6 && h1 == 0.0; ++i)
                 if (cimag(hroots[i]) == 0.0) h1 = creal(hroots[i]);
-            o->h1 = h1;
+            v->h1 = h1;
             double h2 = one / h1;
             double g1 = dd / ee / (h2 - h1);
             double g2 = -g1;
             double f1 = one / sqrt(ee);
             double f2 = f1;
-            if (uf > u0) iu = su * orc_elldoublecomplex(p, f1, g1, h1, f2, g2, h2, 0.0, 0.0, &rffu0, u0, uf) / sqrt(fabs(ee));
-            else if (uf < u0) iu = -su * orc_elldoublecomplex(p, f1, g1, h1, f2, g2, h2, 0.0, 0.0, &rffu0, uf, u0) / sqrt(fabs(ee));
+            if (uf > u0) iu = su * orc_elldoublecomplex(p, f1, g1, h1, f2, g2, h2, 0.0, 0.0, &v->rffu0, u0, uf) / sqrt(fabs(ee));
+            else if (uf < u0) iu = -su * orc_elldoublecomplex(p, f1, g1, h1, f2, g2, h2, 0.0, 0.0, &v->rffu0, uf, u0) / sqrt(fabs(ee));
             else iu = 0.0;
         } else {
             if (fabs(u3 - u2) > 1e-12) {
                 iu1 = 0.0;
                 if (ncase == 7) {
                     if (uf > u2) uf = u2;
-                    if (u0 != u2) iu0 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &rffu0, u0, u2);
-                    else iu0 = 0.0;
-                    if (uf != u2) iu1 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &rffu1, uf, u2);
-                    iu = su * (iu0 - neg1pow(tpr) * iu1) / sqrt(fabs(ee));
+                    if (firstpt && u0 != u2)
+                        v->iu0 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &v->rffu0, u0, u2);
+                    else if (u0 == u2)
+                        v->iu0 = 0.0;
+                    if (uf != u2) iu1 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &v->rffu1, uf, u2);
+                    iu = su * (v->iu0 - neg1pow(tpr) * iu1) / sqrt(fabs(ee));
                 } else if (ncase == 8) {
                     if (uf < u3) uf = u3;
-                    if (u0 != u3) iu0 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &rffu0, u3, u0);
-                    else iu0 = 0.0;
-                    if (uf != u3) iu1 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &rffu1, u3, uf);
-                    iu = su * (iu0 - neg1pow(tpr) * iu1) / sqrt(fabs(ee));
+                    if (firstpt && u0 != u3)
+                        v->iu0 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &v->rffu0, u3, u0);
+                    else if (u0 == u3)
+                        v->iu0 = 0.0;
+                    if (uf != u3) iu1 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &v->rffu1, u3, uf);
+                    iu = su * (v->iu0 - neg1pow(tpr) * iu1) / sqrt(fabs(ee));
                 }
             } else {
                 tpr = 0;
                 if (ncase == 7) {
                     if (uf > u2) uf = u2;
-                    iu0 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &rffu0, u0, uf);
-                    iu = su * iu0 / sqrt(fabs(ee));
+                    v->iu0 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &v->rffu0, u0, uf);
+                    iu = su * v->iu0 / sqrt(fabs(ee));
                 } else if (ncase == 8) {
                     if (uf < u3) uf = u3;
-                    iu0 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &rffu0, u0, uf);
-                    iu = su * iu0 / sqrt(fabs(ee));
+                    v->iu0 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &v->rffu0, u0, uf);
+                    iu = su * v->iu0 / sqrt(fabs(ee));
                 } else {
                     ncase = 0;
                 }
@@ -644,7 +686,8 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
         if (mneg < 0.0) {
             if (muplus < mu0) { muplus = mu0; mpos = muplus * muplus; }
             double muminus = -muplus;
-            orc_calcimusym(a, mneg, mpos, mu0, muplus, &i1mu, &i3mu, &rffmu1, &rffmu3);
+            if (firstpt) orc_calcimusym(a, mneg, mpos, mu0, muplus, &v->i1mu, &v->i3mu, &v->rffmu1, &v->rffmu3);
+            double i1mu = v->i1mu, i3mu = v->i3mu;
             double a1 = sm;
             if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0);
             else tpm = f_int((iu + i1mu) / i3mu);
@@ -656,12 +699,14 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             double sn, cn, dn;
             orc_sncndn(uarg, m1, &sn, &cn, &dn);
             muf = muminus * cn;
+            if (pht) v->rffmu2 = orc_calcimusymf_rf(mneg, mpos, muf, muplus);
         } else {
             muplus = sgn1(mu0) * muplus;
             double muminus = sgn1(mu0) * sqrt(mneg);
             if (fabs(muminus) > fabs(mu0)) { muminus = mu0; mneg = muminus * muminus; }
             if (fabs(mu0) > fabs(muplus)) { muplus = mu0; mpos = muplus * muplus; }
-            orc_calcimuasym(a, mneg, mpos, mu0, muplus, &i1mu, &i3mu, &rffmu1, &rffmu3);
+            if (firstpt) orc_calcimuasym(a, mneg, mpos, mu0, muplus, &v->i1mu, &v->i3mu, &v->rffmu1, &v->rffmu3);
+            double i1mu = v->i1mu, i3mu = v->i3mu;
             double a1 = sm;
             if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int(fabs((1 + sgn1((iu - i1mu) / i3mu)) / 2.0));
             else tpm = f_int((iu + i1mu) / i3mu);
@@ -673,18 +718,38 @@ void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, 
             double sn, cn, dn;
             orc_sncndn(uarg, m1, &sn, &cn, &dn);
             muf = muminus / dn;
+            if (pht) v->rffmu2 = orc_calcimuasymf_rf(mneg, mpos, muf);
         }
     }
+    v->ncase = ncase;
+    v->u1 = u1; v->u2 = u2; v->u3 = u3; v->u4 = u4;
+    *u0p = u0;
+    *ufp = uf;
+    *tpmp = tpm;
+    *tprp = tpr;
+    *mufp = muf;
+    *iup = iu;
+    if (warnedp) *warnedp = warned;
+}
+
+void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
+               double su, double sm, orc_geomu_out* o) {
+    orc_gk_vars v;
+    memset(&v, 0, sizeof v);
+    int32_t tpm = 0, tpr32 = tpr, warned = 0;
+    double muf = 0.0, iu = 0.0;
+    orc_gk_geomu(&v, &u0, &uf, mu0, a, l, q2, &tpm, &tpr32, su, sm, 0, 1, &muf, &iu, &warned);
     o->muf = muf;
     o->iu = iu;
     o->tpm = tpm;
-    o->ncase = ncase;
-    o->tpr = tpr;
-    o->u1 = u1; o->u2 = u2; o->u3 = u3; o->u4 = u4;
+    o->ncase = v.ncase;
+    o->tpr = tpr32;
+    o->u1 = v.u1; o->u2 = v.u2; o->u3 = v.u3; o->u4 = v.u4;
     o->u0_used = u0;
     o->uf_used = uf;
-    o->i1mu = i1mu;
-    o->i3mu = i3mu;
+    o->i1mu = v.i1mu;
+    o->i3mu = v.i3mu;
+    o->h1 = v.h1;
     o->warned = warned;
 }
 

@@ -122,6 +122,59 @@ typedef struct orc_geomu_out {
 
 void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
                double su, double sm, orc_geomu_out* o);
+
+/* GEOKERR's variables shared by its GEOMU and GEOPHITIME calls along one
+ * geodesic (geokerr_original.f:398-412): written when FIRSTPT is true (or on
+ * every call, where the Fortran does so), read otherwise. */
+typedef struct orc_gk_vars {
+    int32_t ncase, pad_;
+    double u1, u2, u3, u4, h1;
+    double iu0, rffu0, rffu1;
+    double i1mu, i3mu, rffmu1, rffmu2, rffmu3;
+    double rdc, rjc, tu01, tu02, tu03, tu04, tmu1, tmu3, phimu1, phimu3;
+} orc_gk_vars;
+
+/* GEOMU with the Fortran's by-reference arguments (u0, uf, tpm, tpr) and
+ * its FIRSTPT/PHT switches. */
+void orc_gk_geomu(orc_gk_vars* v, double* u0, double* uf, double mu0, double a, double l,
+                  double q2, int32_t* tpm, int32_t* tpr, double su, double sm, int pht,
+                  int firstpt, double* muf, double* iu, int32_t* warned);
+double orc_calcimuasymf_rf(double mneg, double mpos, double muf);
+double orc_calcimusymf_rf(double mneg, double mpos, double muf, double muplus);
+
+/* GEOPHITIME (geokerr_original.f:1648-2360): u and mu parts of t and phi
+ * and the affine parameter between (u0, mu0) and (uf, muf). */
+void orc_gk_geophitime(orc_gk_vars* v, double u0, double uf, double mu0, double muf, double a,
+                       double l, double q2, int32_t tpm, int32_t tpr, double su, double sm,
+                       double iu, int firstpt, double* phimu, double* tmu, double* phiu,
+                       double* tu, double* lambda);
+
+/* GEOKERR driver (geokerr_original.f:357-604) in the configuration
+ * geokerr_eval_lambda and geokerr_estimate_periods use: USEGEOR = .false.,
+ * PHIT = .true., MUFILL with NPTS = KEXT = 0, ALPHA = BETA = 0.  uf_is_u0
+ * reproduces geokerr_estimate_periods passing one variable as both U0 and
+ * UF.  Arrays hold nup entries; returns how many leading entries the
+ * driver wrote (the reference leaves the rest unset), after the driver's
+ * own NUP update.  *u0 and *uf come back as the driver modified them. */
+int orc_gk_geokerr(double* u0, double* uf, double uout, double mu0, double a, double l, double q2,
+                   int32_t tpm, int32_t tpr, double su, double sm, int nup, double offset,
+                   int uf_is_u0, double* ufi, double* mufi, double* dti, double* dphi,
+                   int32_t* tpmi, int32_t* tpri, double* lami, int* nup_out);
+
+/* geokerr_ref_api (geokerr_eval_lambda.f90). */
+void orc_geokerr_eval_lambda(double u0, double mu0, double uf, double a, double lz, double q2,
+                             double t0, double lam0, double dlam, int s, double* u, double* mu,
+                             double* t, double* phi, double* affine, int* tpmi, int* tpri,
+                             int* info);
+void orc_geokerr_estimate_periods(double u0, double mu0, double a, double lz, double q2,
+                                  double* tr, double* ttheta, int* info);
+double orc_geokerr_phi_omega(double u0, double mu0, double a, double lz, double q2);
+/* n independent eval_lambda calls (inputs SoA, outputs n x s), OpenMP. */
+void orc_geokerr_eval_lambda_batch(const double* u0, const double* mu0, const double* uf,
+                                   const double* a, const double* lz, const double* q2,
+                                   const double* t0, double lam0, double dlam, int s, int n,
+                                   double* u, double* mu, double* t, double* phi, double* affine,
+                                   int* tpmi, int* tpri, int* info, int nthreads);
 void orc_sncndn(double uu, double emmc, double* sn, double* cn, double* dn);
 /* coefficients and roots are C99 double _Complex arrays, constant term first */
 void orc_zroots(const double _Complex* a, int m, double _Complex* roots, int polish);

@@ -458,3 +458,83 @@ def ell(kind, p, *args):
     *coef, y, x = args
     val = f(pp, *[float(c) for c in coef], C.byref(rff), float(y), float(x))
     return val, rff.value
+
+
+# ---- GEOKERR GEOPHITIME, driver and geokerr_ref_api (geokerr_oracle.c) ----
+_gk_declared = False
+
+
+def _gklib():
+    global _gk_declared
+    L = _glib()
+    if not _gk_declared:
+        d, i32 = C.c_double, C.c_int
+        pd, pi32 = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.orc_gk_geokerr.argtypes = [pd, pd, d, d, d, d, d, C.c_int32, C.c_int32, d, d, i32, d, i32,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+        L.orc_gk_geokerr.restype = i32
+        L.orc_geokerr_eval_lambda.argtypes = [d] * 9 + [i32] + [C.c_void_p] * 5 + [C.POINTER(C.c_int)] * 3
+        L.orc_geokerr_eval_lambda.restype = None
+        L.orc_geokerr_estimate_periods.argtypes = [d] * 5 + [pd, pd, C.POINTER(C.c_int)]
+        L.orc_geokerr_estimate_periods.restype = None
+        L.orc_geokerr_phi_omega.argtypes = [d] * 5
+        L.orc_geokerr_phi_omega.restype = d
+        L.orc_geokerr_eval_lambda_batch.argtypes = [C.c_void_p] * 7 + [d, d, i32, i32] + [C.c_void_p] * 8 + [i32]
+        L.orc_geokerr_eval_lambda_batch.restype = None
+        _gk_declared = True
+    return L
+
+
+def geokerr_driver(u0, uf, mu0, a, l, q2, nup, su=None, sm=None, tpr=1, offset=0.5, uf_is_u0=False):
+    """GEOKERR's sample arrays (USEGEOR off, PHIT on) for one geodesic:
+    returns dict with n (entries written), nup_out, u0/uf as modified, and
+    ufi, mufi, dti, dphi, tpmi, tpri, lami (length nup)."""
+    if su is None:
+        su = 1.0 if uf - u0 >= 0 else -1.0
+    if sm is None:
+        sm = 1.0 if mu0 <= 0 else -1.0
+    out = {k: np.zeros(nup) for k in ("ufi", "mufi", "dti", "dphi", "lami")}
+    out["tpmi"] = np.zeros(nup, np.int32)
+    out["tpri"] = np.zeros(nup, np.int32)
+    U0, UF, nn = C.c_double(u0), C.c_double(uf), C.c_int(0)
+    n = _gklib().orc_gk_geokerr(C.byref(U0), C.byref(UF), u0, mu0, a, l, q2, 0, tpr, su, sm, nup, offset,
+                                1 if uf_is_u0 else 0, out["ufi"].ctypes.data, out["mufi"].ctypes.data,
+                                out["dti"].ctypes.data, out["dphi"].ctypes.data, out["tpmi"].ctypes.data,
+                                out["tpri"].ctypes.data, out["lami"].ctypes.data, C.byref(nn))
+    out.update(n=n, nup_out=nn.value, u0=U0.value, uf=UF.value, su=su, sm=sm)
+    return out
+
+
+def geokerr_eval_lambda(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, s):
+    arrs = [np.zeros(max(s, 0)) for _ in range(5)]
+    tpmi, tpri, info = C.c_int(), C.c_int(), C.c_int()
+    _gklib().orc_geokerr_eval_lambda(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, s,
+                                     *[x.ctypes.data for x in arrs], C.byref(tpmi), C.byref(tpri),
+                                     C.byref(info))
+    u, mu, t, phi, aff = arrs
+    return dict(u=u, mu=mu, t=t, phi=phi, affine=aff, tpmi=tpmi.value, tpri=tpri.value, info=info.value)
+
+
+def geokerr_estimate_periods(u0, mu0, a, lz, q2):
+    tr, tth, info = C.c_double(), C.c_double(), C.c_int()
+    _gklib().orc_geokerr_estimate_periods(u0, mu0, a, lz, q2, C.byref(tr), C.byref(tth), C.byref(info))
+    return tr.value, tth.value, info.value
+
+
+def geokerr_phi_omega(u0, mu0, a, lz, q2):
+    return _gklib().orc_geokerr_phi_omega(u0, mu0, a, lz, q2)
+
+
+def geokerr_eval_lambda_batch(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, s, nthreads=0):
+    ins = [_arr(v) for v in (u0, mu0, uf, a, lz, q2, t0)]
+    n = ins[0].size
+    outs = [np.zeros((n, s)) for _ in range(5)]
+    tpmi = np.zeros(n, np.int32)
+    tpri = np.zeros(n, np.int32)
+    info = np.zeros(n, np.int32)
+    _gklib().orc_geokerr_eval_lambda_batch(*[x.ctypes.data for x in ins], lam0, dlam, s, n,
+                                           *[x.ctypes.data for x in outs], tpmi.ctypes.data,
+                                           tpri.ctypes.data, info.ctypes.data, nthreads)
+    u, mu, t, phi, aff = outs
+    return dict(u=u, mu=mu, t=t, phi=phi, affine=aff, tpmi=tpmi, tpri=tpri, info=info)

@@ -181,13 +181,12 @@ def test_zroots_vs_numpy(oracle):
         assert np.all(np.diff(r.real) >= 0)
 
 
-# The [.., -4] branches (the 1/u^2 time integrals of GEOPHITIME, e.g.
-# geokerr_original.f:1759-1761) are restated as the Fortran reads but do not
-# reproduce direct quadrature (relative gaps 0.1-1 at the test arguments,
-# a4 = 0 included); GEOMU never takes them.  Parity unpinned.
-MINUS4 = pytest.param(-4, marks=pytest.mark.xfail(
-    reason="[-1,-1,-1,(-1,)-4] reduction restated from the Fortran disagrees with quadrature; "
-           "unused by GEOMU; parity unpinned", strict=False))
+# The [.., -4] branches of ellcubic*/ellquartic* drop the third-kind term
+# whose coefficient is the linear coefficient of the polynomial under the
+# root; U(u) = 1 + cc u^2 + dd u^3 + ee u^4 has none, so they are exact in
+# the only form GEOPHITIME uses (fourth factor u, 1/u^2 integrand).  They are
+# pinned in that form by test_minus4_reductions_on_U_polynomials; on the
+# general polynomials of the tests below they would not match quadrature.
 
 
 def _prod_integrand(factors):
@@ -199,7 +198,7 @@ def _prod_integrand(factors):
     return f
 
 
-@pytest.mark.parametrize("p4", [0, -2, MINUS4])
+@pytest.mark.parametrize("p4", [0, -2])
 def test_ellcubicreal_branches(oracle, p4):
     mpmath.mp.dps = 25
     # roots u1 < 0 < y < x < u2 < u3, fourth factor a4 + b4 t positive on [y, x]
@@ -226,7 +225,7 @@ def _ell_with_rff(kind, oracle, p, rff_in, *args):
     return f(pp, *[float(c) for c in coef], C.byref(rff), float(y), float(x)), rff.value
 
 
-@pytest.mark.parametrize("p5", [0, -2, MINUS4])
+@pytest.mark.parametrize("p5", [0, -2])
 def test_ellquarticreal_branches(oracle, p5):
     mpmath.mp.dps = 25
     u1, u2, u3, u4 = -0.8, 0.5, 0.7, 1.4
@@ -244,7 +243,7 @@ def _quad_factor(f, g, h, p):
     return lambda t: (f + g * t + h * t * t) ** (mpmath.mpf(p) / 2)
 
 
-@pytest.mark.parametrize("p4", [0, -2, MINUS4])
+@pytest.mark.parametrize("p4", [0, -2])
 def test_ellcubiccomplex_branches(oracle, p4):
     mpmath.mp.dps = 25
     u1 = -0.6
@@ -260,7 +259,7 @@ def test_ellcubiccomplex_branches(oracle, p4):
     assert abs(val - ref) <= 1e-12 * abs(ref), (val, ref)
 
 
-@pytest.mark.parametrize("p5", [0, -2, MINUS4])
+@pytest.mark.parametrize("p5", [0, -2])
 def test_ellquarticcomplex_branches(oracle, p5):
     mpmath.mp.dps = 25
     u1, u4 = -0.6, 1.3
@@ -290,3 +289,63 @@ def test_elldoublecomplex_branches(oracle, p5):
     integrand = lambda t: q1(t) * q2(t) * (a5 + b5 * t) ** (mpmath.mpf(p5) / 2)
     ref = float(mpmath.quad(integrand, [y, x]))
     assert abs(val - ref) <= 1e-12 * abs(ref), (val, ref)
+
+
+def _u_roots(a, l, q2):
+    cc, dd, ee = a * a - q2 - l * l, 2 * ((a - l) ** 2 + q2), -a * a * q2
+    coefs = [ee, dd, cc, 0.0, 1.0] if ee != 0.0 else [dd, cc, 0.0, 1.0]
+    r = np.roots(coefs)
+    return cc, dd, ee, r
+
+
+@pytest.mark.parametrize("kind", ["cubicreal", "cubiccomplex", "quarticreal", "quarticcomplex"])
+def test_minus4_reductions_on_U_polynomials(oracle, kind):
+    """int dt / (t^2 sqrt(P(t))) with P the factored U(u) of a geodesic, the
+    way GEOPHITIME calls the -4 branch (geokerr_original.f:1759-1761,
+    1878-1880, 1989-1991, 2186-2188), against mpmath quadrature."""
+    mpmath.mp.dps = 30
+    if kind == "cubicreal":          # a = 0, three real roots (row 1)
+        cc, dd, ee, r = _u_roots(0.0, 6.0, 10.0)
+        u1, u2, u3 = np.sort(r.real)
+        y, x = 0.02, u2
+        args = (-u1, 1, u2, -1, u3, -1, 0.0, 1.0)
+        P = lambda t: dd * (t - u1) * (u2 - t) * (u3 - t)
+        p = [-1, -1, -1, -4, 0]
+        scale = math.sqrt(dd)
+    elif kind == "cubiccomplex":     # a = 0, one real root (row 3)
+        cc, dd, ee, r = _u_roots(0.0, 2.0, 3.0)
+        u1 = float(r[np.abs(r.imag) < 1e-12].real[0])
+        f = -1 / dd / u1
+        g = f / u1
+        y, x = 0.02, 0.9
+        args = (-u1, 1, 0.0, 1.0, f, g, 1.0)
+        P = lambda t: dd * (t - u1) * (f + g * t + t * t)
+        p = [-1, -1, -1, -4, 0]
+        scale = math.sqrt(dd)
+    elif kind == "quarticreal":      # four real roots (row 7)
+        cc, dd, ee, r = _u_roots(0.9, 6.0, 10.0)
+        u1, u2, u3, u4 = np.sort(r.real)
+        y, x = 0.02, u2
+        args = (-u1, 1, u2, -1, u3, -1, u4, -1, 0.0, 1.0)
+        P = lambda t: abs(ee) * (t - u1) * (u2 - t) * (u3 - t) * (u4 - t)
+        p = [-1, -1, -1, -1, -4]
+        scale = math.sqrt(abs(ee))
+    else:                            # two real roots (row 5)
+        cc, dd, ee, r = _u_roots(0.9, 2.0, 3.0)
+        real = np.sort(r[np.abs(r.imag) < 1e-12].real)
+        u1, u4 = real[0], real[-1]
+        qs = 1.0
+        f = -qs / abs(ee) / u1 / u4
+        g = (u4 + u1) / u1 / u4 * f
+        y, x = 0.02, 0.6
+        args = (-u1, 1, qs * u4, -qs, 0.0, 1.0, f, g, 1.0)
+        P = lambda t: abs(ee) * (t - u1) * (u4 - t) * (f + g * t + t * t)
+        p = [-1, -1, -1, -1, -4]
+        scale = math.sqrt(abs(ee))
+    name = "ell" + kind
+    p0 = list(p)
+    p0[3 if kind.startswith("cubic") else 4] = 0
+    _, rff0 = oracle.ell(name, p0, *args, y, x)
+    val, _ = _ell_with_rff(name, oracle, p, rff0, *args, y, x)
+    ref = float(mpmath.quad(lambda t: 1 / (t * t * mpmath.sqrt(P(t))), [y, x])) * scale
+    assert abs(val - ref) <= 1e-11 * abs(ref), (kind, val, ref)
