@@ -22,7 +22,8 @@ typedef enum np_status {
     NP_EINVAL = -1,    /* bad argument (sizes, pointers, parameters) */
     NP_EALLOC = -2,    /* device or pinned-host allocation failed */
     NP_ELAUNCH = -3,   /* kernel launch or execution failed */
-    NP_ENODEV = -4     /* no HIP device */
+    NP_ENODEV = -4,    /* no HIP device */
+    NP_EDISABLED = -5  /* entry point switched off in this build (see its comment) */
 } np_status;
 
 /* Human-readable text for a status code (static storage). */
@@ -169,6 +170,52 @@ int np_geomu_camera_batch(const double* d_alpha, const double* d_beta, const dou
                           const double* d_mu0, const double* d_uf, double u0, int64_t n,
                           double* d_muf, int32_t* d_tpm, int32_t* d_ncase, double* d_iu,
                           void* stream);
+
+/* Samples of the GEOKERR driver behind geokerr_estimate_periods
+ * (geokerr_eval_lambda.f90:233). */
+#define NP_GEOKERR_PERIOD_SAMPLES 4096
+/* geokerr_ref_api info for a failed GPU call: 100 - np_status. */
+#define NP_GEOKERR_INFO_DEVICE 100
+
+/*
+ * Device workspace (256-byte aligned) for the GEOKERR driver over n
+ * geodesics of nup samples each: a per-geodesic record (roots, case, the
+ * cached u0 / mu0 integrals) and the driver's sample arrays, 48 bytes per
+ * sample.  Returns -1 for nup <= 0 or n < 0.
+ */
+int64_t np_geokerr_workspace_bytes(int32_t nup, int64_t n);
+
+/*
+ * n independent geokerr_eval_lambda calls (geokerr_eval_lambda.f90:62-198;
+ * host form in geokerr_ref_api.h) on device arrays: inputs u0, mu0, uf, a,
+ * lz, q2, t0, lam0, dlam have n entries; outputs u, mu, t, phi, affine are
+ * n x s (row i = geodesic i), tpmi, tpri and info (optional) have n.
+ * workspace: np_geokerr_workspace_bytes(s, n) bytes.  Replaces the
+ * per-call Fortran (GEOKERR, geokerr_original.f:357-604) and the port
+ * solver geokerr_complete_stream_cuda (geokerr_port/cuda/
+ * geokerr_complete.cu:446-490).
+ */
+int np_geokerr_eval_lambda_batch(const double* d_u0, const double* d_mu0, const double* d_uf,
+                                 const double* d_a, const double* d_lz, const double* d_q2,
+                                 const double* d_t0, const double* d_lam0, const double* d_dlam,
+                                 int32_t s, int64_t n, double* d_u, double* d_mu, double* d_t,
+                                 double* d_phi, double* d_affine, int32_t* d_tpmi,
+                                 int32_t* d_tpri, int32_t* d_info, void* d_workspace,
+                                 void* stream);
+
+/*
+ * n geokerr_estimate_periods calls (geokerr_eval_lambda.f90:200-295);
+ * workspace: np_geokerr_workspace_bytes(NP_GEOKERR_PERIOD_SAMPLES, n).
+ */
+int np_geokerr_estimate_periods_batch(const double* d_u0, const double* d_mu0, const double* d_a,
+                                      const double* d_lz, const double* d_q2, int64_t n,
+                                      double* d_tr, double* d_ttheta, int32_t* d_info,
+                                      void* d_workspace, void* stream);
+
+/* n geokerr_phi_omega values (geokerr_eval_lambda.f90:297-306). */
+int np_geokerr_phi_omega_batch(const double* d_u0, const double* d_mu0, const double* d_a,
+                               const double* d_lz, const double* d_q2, int64_t n, double* d_out,
+                               void* stream);
 
 /* ---------------------------------------------------------------------
  * Schwarzschild equatorial sweeps (schwarzschild_blackhole.cu)

@@ -150,6 +150,90 @@ def geomu_batch(u0, uf, mu0, spin, L, Q2, su, sm, tpr=None, stream=None):
     return muf, tpm, ncase, iu
 
 
+GEOKERR_PERIOD_SAMPLES = 4096
+
+
+def _geokerr_workspace(nup, n, device):
+    import torch
+    nbytes = lib().np_geokerr_workspace_bytes(int(nup), int(n))
+    if nbytes < 0:
+        raise NullPathError("np_geokerr_workspace_bytes: invalid sizes")
+    # torch's caching allocator returns blocks aligned to at least 512 bytes
+    return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+
+
+def geokerr_eval_lambda_batch(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, s, stream=None, workspace=None):
+    """n geokerr_eval_lambda calls on float64 device tensors of n entries
+    (np_geokerr_eval_lambda_batch).  Returns a dict of device tensors:
+    u, mu, t, phi, affine (n x s) and tpmi, tpri, info (n, int32)."""
+    import torch
+    n = u0.numel()
+    dev = u0.device
+    out = {k: torch.empty((n, s), dtype=torch.float64, device=dev) for k in ("u", "mu", "t", "phi", "affine")}
+    for k in ("tpmi", "tpri", "info"):
+        out[k] = torch.empty(n, dtype=torch.int32, device=dev)
+    ws = _geokerr_workspace(s, n, dev) if workspace is None else workspace
+    code = lib().np_geokerr_eval_lambda_batch(
+        u0.data_ptr(), mu0.data_ptr(), uf.data_ptr(), a.data_ptr(), lz.data_ptr(), q2.data_ptr(),
+        t0.data_ptr(), lam0.data_ptr(), dlam.data_ptr(), int(s), n, out["u"].data_ptr(), out["mu"].data_ptr(),
+        out["t"].data_ptr(), out["phi"].data_ptr(), out["affine"].data_ptr(), out["tpmi"].data_ptr(),
+        out["tpri"].data_ptr(), out["info"].data_ptr(), ws.data_ptr(), _stream_ptr(stream))
+    check(code, "np_geokerr_eval_lambda_batch")
+    return out
+
+
+def geokerr_estimate_periods_batch(u0, mu0, a, lz, q2, stream=None):
+    """n geokerr_estimate_periods calls (np_geokerr_estimate_periods_batch):
+    returns (Tr, Ttheta, info) device tensors."""
+    import torch
+    n = u0.numel()
+    tr = torch.empty(n, dtype=torch.float64, device=u0.device)
+    tth = torch.empty_like(tr)
+    info = torch.empty(n, dtype=torch.int32, device=u0.device)
+    ws = _geokerr_workspace(GEOKERR_PERIOD_SAMPLES, n, u0.device)
+    code = lib().np_geokerr_estimate_periods_batch(u0.data_ptr(), mu0.data_ptr(), a.data_ptr(), lz.data_ptr(),
+                                                   q2.data_ptr(), n, tr.data_ptr(), tth.data_ptr(),
+                                                   info.data_ptr(), ws.data_ptr(), _stream_ptr(stream))
+    check(code, "np_geokerr_estimate_periods_batch")
+    return tr, tth, info
+
+
+def geokerr_phi_omega_batch(u0, mu0, a, lz, q2, stream=None):
+    """np_geokerr_phi_omega_batch on float64 device tensors."""
+    import torch
+    out = torch.empty_like(u0)
+    code = lib().np_geokerr_phi_omega_batch(u0.data_ptr(), mu0.data_ptr(), a.data_ptr(), lz.data_ptr(),
+                                            q2.data_ptr(), u0.numel(), out.data_ptr(), _stream_ptr(stream))
+    check(code, "np_geokerr_phi_omega_batch")
+    return out
+
+
+def geokerr_eval_lambda(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, s):
+    """The drop-in geokerr_eval_lambda C entry point (host arrays)."""
+    import ctypes as C
+    import numpy as np
+    arrs = [np.zeros(max(int(s), 0)) for _ in range(5)]
+    tpmi, tpri, info = C.c_int(), C.c_int(), C.c_int()
+    pd = C.POINTER(C.c_double)
+    lib().geokerr_eval_lambda(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, int(s),
+                              *[x.ctypes.data_as(pd) for x in arrs], C.byref(tpmi), C.byref(tpri), C.byref(info))
+    u, mu, t, phi, aff = arrs
+    return dict(u=u, mu=mu, t=t, phi=phi, affine=aff, tpmi=tpmi.value, tpri=tpri.value, info=info.value)
+
+
+def geokerr_estimate_periods(u0, mu0, a, lz, q2):
+    """The drop-in geokerr_estimate_periods C entry point."""
+    import ctypes as C
+    tr, tth, info = C.c_double(), C.c_double(), C.c_int()
+    lib().geokerr_estimate_periods(u0, mu0, a, lz, q2, C.byref(tr), C.byref(tth), C.byref(info))
+    return tr.value, tth.value, info.value
+
+
+def geokerr_phi_omega(u0, mu0, a, lz, q2):
+    """The drop-in geokerr_phi_omega C entry point."""
+    return lib().geokerr_phi_omega(u0, mu0, a, lz, q2)
+
+
 def schw_sweep(b, M=10.0, r0=None, r_capture=None, ps_band=None, max_steps=2_000_000,
                stream=None, total_steps=None):
     """FP64 plain-method Schwarzschild sweep (np_schw_sweep) over a float64

@@ -139,6 +139,21 @@ def _declare(L: C.CDLL) -> None:
     L.np_adjoint_forward.restype = i32
     L.np_adjoint_backward.argtypes = [pa, vp, vp, vp, vp]
     L.np_adjoint_backward.restype = i32
+    L.np_geokerr_workspace_bytes.argtypes = [i32, i64]
+    L.np_geokerr_workspace_bytes.restype = i64
+    L.np_geokerr_eval_lambda_batch.argtypes = [vp] * 9 + [i32, i64] + [vp] * 10
+    L.np_geokerr_eval_lambda_batch.restype = i32
+    L.np_geokerr_estimate_periods_batch.argtypes = [vp] * 5 + [i64] + [vp] * 5
+    L.np_geokerr_estimate_periods_batch.restype = i32
+    L.np_geokerr_phi_omega_batch.argtypes = [vp] * 5 + [i64, vp, vp]
+    L.np_geokerr_phi_omega_batch.restype = i32
+    pd, pi = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.geokerr_eval_lambda.argtypes = [d] * 9 + [C.c_int] + [pd] * 5 + [pi] * 3
+    L.geokerr_eval_lambda.restype = None
+    L.geokerr_estimate_periods.argtypes = [d] * 5 + [pd, pd, pi]
+    L.geokerr_estimate_periods.restype = None
+    L.geokerr_phi_omega.argtypes = [d] * 5
+    L.geokerr_phi_omega.restype = d
 
 
 class LookupTable(C.Structure):

@@ -22,7 +22,8 @@ COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", f"-I{INCLUDE}"
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
-LIB_SOURCES = ["np_frame.cpp", "kerr_render.hip", "geomu.hip", "schw.hip", "adjoint.hip", "np_api.hip"]
+LIB_SOURCES = ["np_frame.cpp", "kerr_render.hip", "geomu.hip", "geokerr.hip", "schw.hip", "adjoint.hip",
+               "np_api.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -313,6 +313,7 @@ const char* np_status_string(int code) {
         case NP_EALLOC: return "allocation failed";
         case NP_ELAUNCH: return "kernel launch or execution failed";
         case NP_ENODEV: return "no HIP device";
+        case NP_EDISABLED: return "entry point disabled in this build";
         default: return "unknown status";
     }
 }

@@ -28,8 +28,10 @@
 #endif
 
 static double sgn1(double x) { return copysign(1.0, x); }
-static double neg1pow(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
-static int f_int(double x) { return (int)x; }
+static double neg1pow(unsigned n) { return (n & 1u) ? -1.0 : 1.0; }
+/* Fortran INT() as the reference's x86-64 build executes it: INT_MIN for
+ * NaN and out-of-range values. */
+static int f_int(double x) { return (x > -2147483649.0 && x < 2147483648.0) ? (int)x : (int)0x80000000u; }
 
 /* Eq. (47): t_u from the four u integrals (geokerr_original.f:1796-1798). */
 static double eq47(double a, double l, double uplus, double umi, double t1, double t2, double t3,
@@ -312,7 +314,7 @@ void orc_gk_geophitime(orc_gk_vars* v, double u0, double uf, double mu0, double 
         /* mu part for q2 = 0 (geokerr_original.f:1925-1941) */
         double s1 = sgn1(mu0);
         double b1 = s1 * sm;
-        double b2 = s1 * sm * neg1pow(tpm + 1);
+        double b2 = s1 * sm * neg1pow((unsigned)tpm + 1u);
         if (fabs(l) < fabs(a)) {
             double muplus = s1 * sqrt(one - l2 / a / a);
             double sq = sqrt(one - muplus * muplus);
@@ -496,7 +498,7 @@ void orc_gk_geophitime(orc_gk_vars* v, double u0, double uf, double mu0, double 
         if (mpos > 1.0) mpos = 1.0;
         double muplus = sqrt(mpos);
         double c1 = sm;
-        double c2 = sm * neg1pow(tpm + 1);
+        double c2 = sm * neg1pow((unsigned)tpm + 1u);
         double c3 = 2.0 * f_int((2.0 * (double)tpm - sm + 1.0) / 4.0);
         double phimu2 = 0.0, tmu2 = 0.0;
         if (mneg < 0.0) {

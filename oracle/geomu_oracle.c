@@ -26,8 +26,14 @@
 #endif
 
 static double sgn1(double x) { return copysign(1.0, x); }
-static double neg1pow(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
-static int f_int(double x) { return (int)x; } /* C cast truncates toward zero like Fortran INT */
+/* (-1)**n by parity, unsigned so that n + 1 at INT_MAX wraps like the
+ * reference's 32-bit integers. */
+static double neg1pow(unsigned n) { return (n & 1u) ? -1.0 : 1.0; }
+static int add_i32(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+/* Fortran INT() as the reference's x86-64 build executes it (cvttsd2si):
+ * truncation toward zero, INT_MIN for NaN and for values outside int (a
+ * plain C cast of those is undefined). */
+static int f_int(double x) { return (x > -2147483649.0 && x < 2147483648.0) ? (int)x : (int)0x80000000u; }
 
 /* ---------------- Carlson reductions (RF, RC, RD, RJ callers) ---------------- */
 
@@ -476,6 +482,12 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
     double dd = two * ((a - l) * (a - l) + q2);
     double ee = -a * a * q2;
     double u0 = *u0p, uf = *ufp;
+    /* geokerr_estimate_periods hands GEOKERR one variable as both U0 and UF,
+     * so GEOMU's first call sees u0 and uf aliased: every edit of one is an
+     * edit of the other. */
+    const int alias = (u0p == ufp);
+#define SET_UF(x) do { uf = (x); if (alias) u0 = uf; } while (0)
+#define SET_U0(x) do { u0 = (x); if (alias) uf = u0; } while (0)
     int tpm = *tpmp, tpr = *tprp;
     double iu = 0.0, iu1 = 0.0, muf = 0.0;
     int ncase = v->ncase, warned = 0, p[5];
@@ -494,7 +506,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             iu1 = 0.0;
             for (;;) {
                 if (u0 <= u2) {
-                    if (uf > u2) uf = u2;
+                    if (uf > u2) SET_UF(u2);
                     ncase = 1;
                     if (firstpt && u0 != u2)
                         v->iu0 = orc_ellcubicreal(p, -u1, one, u2, -one, u3, -one, 0.0, 0.0, &v->rffu0, u0, u2);
@@ -504,7 +516,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
                     iu = su * (v->iu0 - neg1pow(tpr) * iu1) / sqrt(dd);
                     break;
                 } else if (u0 >= u3) {
-                    if (uf < u3) uf = u3;
+                    if (uf < u3) SET_UF(u3);
                     ncase = 2;
                     if (firstpt && u0 != u3)
                         v->iu0 = -orc_ellcubicreal(p, -u1, one, -u2, one, -u3, one, 0.0, 0.0, &v->rffu0, u3, u0);
@@ -516,7 +528,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
                 } else {
                     /* forbidden region: the reference prints a warning and moves u0 */
                     warned = 1;
-                    u0 = (su == 1.0) ? u3 : u2;
+                    SET_U0((su == 1.0) ? u3 : u2);
                 }
             }
         } else if (fabs(dis) < 1e-16) {
@@ -527,7 +539,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             if (u0 <= u2) {
                 ncase = 1;
                 if (uf > u2) {
-                    uf = u2;
+                    SET_UF(u2);
                     iu = su * 1e300;
                 } else {
                     double farg = (sqrt(uf - u1) + sqrt(u2 - u1)) / fabs(sqrt(uf - u1) - sqrt(u2 - u1));
@@ -537,7 +549,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             } else if (u0 >= u2) {
                 ncase = 2;
                 if (uf < u2) {
-                    uf = u2;
+                    SET_UF(u2);
                     iu = su * 1e300;
                 } else {
                     double farg = (sqrt(uf - u1) + sqrt(u2 - u1)) / fabs(sqrt(uf - u1) - sqrt(u2 - u1));
@@ -571,7 +583,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             v->i1mu = acos(mu0 / muplus) / sqrt(ql2);
             v->i3mu = pi / sqrt(ql2);
             double i1mu = v->i1mu, i3mu = v->i3mu;
-            if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0);
+            if (sm == 1.0) tpm = add_i32(f_int((iu - i1mu) / i3mu), f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0));
             else tpm = f_int((iu + i1mu) / i3mu);
             double a2 = sm * neg1pow(tpm);
             double a3 = two * f_int((two * (double)tpm + 3.0 - sm) / 4.0) - one;
@@ -604,7 +616,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
                     else if (u0 <= u2) { ncase = 7; break; }
                     else if (u0 >= u3) { ncase = 8; break; }
                     warned = 1;
-                    u0 = (su == 1.0) ? 0.0 : uplus;
+                    SET_U0((su == 1.0) ? 0.0 : uplus);
                 }
             }
         }
@@ -644,7 +656,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             if (fabs(u3 - u2) > 1e-12) {
                 iu1 = 0.0;
                 if (ncase == 7) {
-                    if (uf > u2) uf = u2;
+                    if (uf > u2) SET_UF(u2);
                     if (firstpt && u0 != u2)
                         v->iu0 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &v->rffu0, u0, u2);
                     else if (u0 == u2)
@@ -652,7 +664,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
                     if (uf != u2) iu1 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &v->rffu1, uf, u2);
                     iu = su * (v->iu0 - neg1pow(tpr) * iu1) / sqrt(fabs(ee));
                 } else if (ncase == 8) {
-                    if (uf < u3) uf = u3;
+                    if (uf < u3) SET_UF(u3);
                     if (firstpt && u0 != u3)
                         v->iu0 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &v->rffu0, u3, u0);
                     else if (u0 == u3)
@@ -663,11 +675,11 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             } else {
                 tpr = 0;
                 if (ncase == 7) {
-                    if (uf > u2) uf = u2;
+                    if (uf > u2) SET_UF(u2);
                     v->iu0 = orc_ellquarticreal(p, -u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, &v->rffu0, u0, uf);
                     iu = su * v->iu0 / sqrt(fabs(ee));
                 } else if (ncase == 8) {
-                    if (uf < u3) uf = u3;
+                    if (uf < u3) SET_UF(u3);
                     v->iu0 = -orc_ellquarticreal(p, -u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, &v->rffu0, u0, uf);
                     iu = su * v->iu0 / sqrt(fabs(ee));
                 } else {
@@ -689,7 +701,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             if (firstpt) orc_calcimusym(a, mneg, mpos, mu0, muplus, &v->i1mu, &v->i3mu, &v->rffmu1, &v->rffmu3);
             double i1mu = v->i1mu, i3mu = v->i3mu;
             double a1 = sm;
-            if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0);
+            if (sm == 1.0) tpm = add_i32(f_int((iu - i1mu) / i3mu), f_int((1 + sgn1((iu - i1mu) / i3mu)) / 2.0));
             else tpm = f_int((iu + i1mu) / i3mu);
             double a2 = sm * neg1pow(tpm);
             double a3 = two * f_int((two * (double)tpm + 3.0 - sm) / 4.0) - one;
@@ -708,7 +720,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             if (firstpt) orc_calcimuasym(a, mneg, mpos, mu0, muplus, &v->i1mu, &v->i3mu, &v->rffmu1, &v->rffmu3);
             double i1mu = v->i1mu, i3mu = v->i3mu;
             double a1 = sm;
-            if (sm == 1.0) tpm = f_int((iu - i1mu) / i3mu) + f_int(fabs((1 + sgn1((iu - i1mu) / i3mu)) / 2.0));
+            if (sm == 1.0) tpm = add_i32(f_int((iu - i1mu) / i3mu), f_int(fabs((1 + sgn1((iu - i1mu) / i3mu)) / 2.0)));
             else tpm = f_int((iu + i1mu) / i3mu);
             double a2 = sm * neg1pow(tpm);
             double a3 = two * f_int((two * (double)tpm + 3.0 - sm) / 4.0) - one;
@@ -721,6 +733,8 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             if (pht) v->rffmu2 = orc_calcimuasymf_rf(mneg, mpos, muf);
         }
     }
+#undef SET_UF
+#undef SET_U0
     v->ncase = ncase;
     v->u1 = u1; v->u2 = u2; v->u3 = u3; v->u4 = u4;
     *u0p = u0;

@@ -1,0 +1,26 @@
+"""Host-side behaviour of the geokerr_ref_api drop-ins in libnullpath.so
+that needs no GPU: argument checks and geokerr_phi_omega (a host closed
+form) against the oracle."""
+import numpy as np
+
+import nullpath_amd as npa
+
+
+def test_eval_lambda_rejects_nonpositive_samples():
+    # returns before touching the device, as the reference does for S <= 0
+    r = npa.geokerr_eval_lambda(0.02, 0.3, 0.5, 0.9, 6.0, 10.0, 0.0, 0.0, 0.1, 0)
+    assert r["info"] == 1
+    assert npa.lib().np_status_string(-5) == b"entry point disabled in this build"
+
+
+def test_phi_omega_host_matches_oracle(oracle):
+    rng = np.random.default_rng(17)
+    for _ in range(500):
+        args = (10 ** rng.uniform(-4, -0.4), rng.uniform(-0.99, 0.99), rng.uniform(-0.99, 0.99),
+                rng.uniform(-8, 8), rng.uniform(0, 30))
+        got = npa.geokerr_phi_omega(*args)
+        want = oracle.geokerr_phi_omega(*args)
+        assert abs(got - want) <= 1e-13 * max(1.0, abs(want)), (args, got, want)
+    # the floors: u0 = 0 puts r at 1e12, mu0 = 1 floors 1 - mu0^2 at 1e-30
+    assert npa.geokerr_phi_omega(0.0, 0.2, 0.5, 2.0, 1.0) == oracle.geokerr_phi_omega(0.0, 0.2, 0.5, 2.0, 1.0)
+    assert npa.geokerr_phi_omega(0.1, 1.0, 0.5, 2.0, 1.0) == oracle.geokerr_phi_omega(0.1, 1.0, 0.5, 2.0, 1.0)
