@@ -398,6 +398,71 @@ def bench_geomu(n: int) -> dict:
     }
 
 
+def geokerr_cases(n: int, seed: int = 2026):
+    """Mixed geodesics for the GEOKERR driver: camera-like starts u0 ~
+    U[1e-3, 0.02] (r = 50..1000 M), toward uf ~ U(0.1, 0.9) (through the
+    radial turning point when the ray scatters), a ~ U[0, 0.99],
+    L ~ U[-7, 7], Q2 ~ U[0.5, 40], mu0 inside the allowed range."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u0 = rng.uniform(1e-3, 0.02, n)
+    uf = rng.uniform(0.1, 0.9, n)
+    a = rng.uniform(0.0, 0.99, n)
+    lz = rng.uniform(-7, 7, n)
+    q2 = rng.uniform(0.5, 40, n)
+    mu0 = rng.uniform(-0.6, 0.6, n)
+    th = q2 - (q2 + lz * lz - a * a) * mu0 ** 2 - a * a * mu0 ** 4
+    mu0[th < 0] = 0.0
+    return u0, mu0, uf, a, lz, q2
+
+
+def bench_geokerr(n: int, s: int, cpu_geodesics: int) -> dict:
+    """geokerr_eval_lambda batch (np_geokerr_eval_lambda_batch): n geodesics
+    x s driver samples resampled onto lambda = 0.1 k; HIP events around the
+    three kernels on the launch stream; the oracle on cpu_geodesics of the
+    same inputs on the host cores."""
+    import numpy as np
+    import torch
+
+    import nullpath_amd as npa
+    u0, mu0, uf, a, lz, q2 = geokerr_cases(n)
+    t = lambda x: torch.tensor(x, dtype=torch.float64, device="cuda")
+    args = [t(u0), t(mu0), t(uf), t(a), t(lz), t(q2), t(np.zeros(n)), t(np.zeros(n)), t(np.full(n, 0.1))]
+    ws = npa._geokerr_workspace(s, n, "cuda")
+    npa.geokerr_eval_lambda_batch(*[x[:64] for x in args], s, workspace=ws)   # warm-up
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record(stream)
+    for _ in range(reps):
+        out = npa.geokerr_eval_lambda_batch(*args, s, stream=stream, workspace=ws)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    res = {
+        "workload": f"geokerr_eval_lambda {n} geodesics x {s} samples (GEOKERR driver, NUP = S), "
+                    "u0~U[1e-3,0.02], uf~U(0.1,0.9), a~U[0,0.99], L~U[-7,7], Q2~U[0.5,40], FP64",
+        "samples_per_s": n * s / (ms / 1e3), "batch_ms": ms,
+        "rays_with_u_turning_point": int((out["tpri"] >= 0).sum().item()),
+        "nan_rows": int(torch.isnan(out["t"]).any(dim=1).sum().item()),
+    }
+    if cpu_geodesics > 0:
+        import os
+        import time
+
+        from oracle import np_oracle as O
+        m = min(cpu_geodesics, n)
+        threads = min(16, len(os.sched_getaffinity(0)))
+        t0 = time.perf_counter()
+        O.geokerr_eval_lambda_batch(u0[:m], mu0[:m], uf[:m], a[:m], lz[:m], q2[:m], np.zeros(m), 0.0, 0.1, s,
+                                    nthreads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m * s / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} of the {n} geodesics, {s} samples each"}
+    return res
+
+
 def cpu_baseline(args, fp64: dict) -> dict:
     """The FP64 oracle (a plain-C restatement of the reference integrator)
     on a bounded random sample of the same frame, on the host cores."""
@@ -506,6 +571,7 @@ def main() -> int:
         line["config"]["uneven_batch"] = bench_batch()
         line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000)
         line["config"]["geomu_batch"] = bench_geomu(1 << 20)
+        line["config"]["geokerr_eval_lambda"] = bench_geokerr(256, 4096, 64 if args.cpu_pixels > 0 else 0)
 
     if args.cpu_pixels > 0 and "fp64" in res:
         line["cpu_baseline"] = cpu_baseline(args, res["fp64"])
