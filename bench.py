@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--adjoint-size", type=int, default=512)
     ap.add_argument("--side-workloads", type=int, default=1,
                     help="also time the Schwarzschild 1e5 sweep and the GEOMU 2^20 batch")
+    ap.add_argument("--geokerr-only", type=int, default=0,
+                    help="only time geokerr_eval_lambda batches of this many geodesics x 4096 "
+                         "samples (for profiling that path) and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (a reference "
                          "kernel of known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) "
@@ -508,6 +511,9 @@ def main() -> int:
         return 2
     if args.copy_calibration_mib > 0:
         return copy_calibration(args.copy_calibration_mib)
+    if args.geokerr_only > 0:
+        print(json.dumps(bench_geokerr(args.geokerr_only, 4096, 0)))
+        return 0
     precs = args.precisions.split(",")
     res = {prec: render_case(prec, args.crossing, args.width, args.height, args.spp, args.steps,
                              args.warmup) for prec in precs}

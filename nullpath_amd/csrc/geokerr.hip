@@ -12,11 +12,13 @@
 //                        FIRSTPT GEOPHITIME pieces of sample 1 -> GkRecord;
 //   2. gk_sample_kernel  one lane per (geodesic, sample): GEOMU + GEOPHITIME
 //                        with FIRSTPT = false from the record, NUP x n lanes;
-//   3. gk_resample_kernel / gk_periods_kernel  one wave per geodesic: the
-//                        eval_lambda linear resampling onto the lambda grid
-//                        (binary search when the samples' lambda is sorted,
-//                        the reference's sequential scan otherwise) and the
-//                        turning-point indices, or estimate_periods' search.
+//   3. gk_resample_kernel  one 256-thread block per geodesic, the samples'
+//                        lambda staged in LDS: eval_lambda's linear
+//                        resampling onto the lambda grid (binary search when
+//                        the samples' lambda is sorted, the reference's
+//                        sequential scan otherwise) and the turning-point
+//                        indices; gk_periods_kernel (one wave per geodesic)
+//                        for estimate_periods' search.
 // Each sample reproduces the sequential driver's value: the record holds
 // exactly the state the Fortran carries from sample to sample.
 #include <hip/hip_runtime.h>
@@ -196,30 +198,28 @@ __device__ inline int first_grid_at_or_after(double lam0, double dlam, int s, do
     return -1;
 }
 
+// Samples of lambda staged in LDS per geodesic by the resampling kernel
+// (32 KB); longer sample arrays are searched in place in HBM.
+constexpr int kLdsLam = 4096;
+
 // geokerr_eval_lambda's resampling and turning-point indices
-// (geokerr_eval_lambda.f90:128-195), one wave per geodesic.
-__global__ void __launch_bounds__(256)
-gk_resample_kernel(const GkRecord* __restrict__ rec, GkSamples smp, int32_t nup, int64_t n,
-                   const double* __restrict__ t0a, const double* __restrict__ lam0a,
-                   const double* __restrict__ dlama, int32_t s, double* __restrict__ u, double* __restrict__ mu,
-                   double* __restrict__ t, double* __restrict__ phi, double* __restrict__ affine,
-                   int32_t* __restrict__ tpmi_o, int32_t* __restrict__ tpri_o, int32_t* __restrict__ info) {
-    const int lane = threadIdx.x & 63;
-    const int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    if (g >= n) return;
-    const GkRecord& R = rec[g];
-    const int nw = R.nwritten;
-    const size_t o = (size_t)g * (size_t)nup;
-    const double* lami = smp.lami + o;
+// (geokerr_eval_lambda.f90:128-195) for one geodesic per 256-thread block,
+// with lam the geodesic's nw driver samples of lambda (LDS or HBM).
+__device__ __forceinline__ void resample_geodesic(const double* lam, const GkRecord& R, int nw,
+                                                  const GkSamples& smp, size_t o, double t0, double lam0,
+                                                  double dlam, int32_t s, size_t so, double* __restrict__ u,
+                                                  double* __restrict__ mu, double* __restrict__ t,
+                                                  double* __restrict__ phi, double* __restrict__ affine,
+                                                  int32_t* tp_out) {
+    const int tid = threadIdx.x, lane = tid & 63;
     const double *ufi = smp.ufi + o, *mufi = smp.mufi + o, *dti = smp.dti + o, *dph = smp.dphi + o;
-    const size_t so = (size_t)g * (size_t)s;
-    const double t0 = t0a[g], lam0 = lam0a[g], dlam = dlama[g];
-    const double base = (nw > 0) ? lami[0] : 0.0;
-    // sorted check: lami[j] <= lami[j+1] for all j (NaN fails it)
-    bool sorted = true;
-    for (int j = lane; j + 1 < nw; j += 64)
-        if (!(lami[j] <= lami[j + 1])) sorted = false;
-    sorted = (ballot(!sorted) == 0);
+    const double base = (nw > 0) ? lam[0] : 0.0;
+    // sorted check: lam[j] <= lam[j+1] for all j (NaN fails it)
+    int unsorted = 0;
+    for (int j = tid; j + 1 < nw; j += blockDim.x)
+        if (!(lam[j] <= lam[j + 1])) unsorted = 1;
+    const bool sorted = !__syncthreads_or(unsorted);
+    const double* lami = lam;
 
     auto emit = [&](int i, int j) {
         const double lamk = lam0 + dlam * (double)i;
@@ -253,7 +253,7 @@ gk_resample_kernel(const GkRecord* __restrict__ rec, GkSamples smp, int32_t nup,
     if (nw <= 0 || sorted) {
         // j = first index whose successor reaches the key (n-1 if none); the
         // scan's running j is the key of the largest grid value seen so far
-        for (int i = lane; i < s; i += 64) {
+        for (int i = tid; i < s; i += blockDim.x) {
             int j = 0;
             if (nw > 1) {
                 const double lamk = lam0 + dlam * (double)((dlam >= 0.0) ? i : 0);
@@ -268,7 +268,7 @@ gk_resample_kernel(const GkRecord* __restrict__ rec, GkSamples smp, int32_t nup,
             }
             emit(i, j);
         }
-    } else if (lane == 0) {
+    } else if (tid == 0) {
         int j = 0;
         for (int i = 0; i < s; ++i) {
             const double lamk = lam0 + dlam * (double)i;
@@ -277,27 +277,57 @@ gk_resample_kernel(const GkRecord* __restrict__ rec, GkSamples smp, int32_t nup,
         }
     }
 
-    // first turning-point indices on the grid
-    int tp[2] = {-1, -1};
-    const int32_t* tpa[2] = {smp.tpmi + o, smp.tpri + o};
-    for (int which = 0; which < 2; ++which) {
-        for (int jb = 0; jb < nw && tp[which] < 0; jb += 64) {
-            const int jj = jb + lane;
-            uint64_t m = ballot(jj < nw && tpa[which][jj] > 0);
-            while (m && tp[which] < 0) {
-                const int jl = first_lane(m);
-                m &= m - 1;
-                const double L = lami[jb + jl] - base;
-                tp[which] = first_grid_at_or_after(lam0, dlam, s, L, lane);
-                if (tp[which] < 0 && sorted) {
-                    // later samples have larger lambda: none of them lands either
-                    jb = nw;
-                    break;
+    // first turning-point indices on the grid (wave 0)
+    if (tid < 64) {
+        int tp[2] = {-1, -1};
+        const int32_t* tpa[2] = {smp.tpmi + o, smp.tpri + o};
+        for (int which = 0; which < 2; ++which) {
+            for (int jb = 0; jb < nw && tp[which] < 0; jb += 64) {
+                const int jj = jb + lane;
+                uint64_t m = ballot(jj < nw && tpa[which][jj] > 0);
+                while (m && tp[which] < 0) {
+                    const int jl = first_lane(m);
+                    m &= m - 1;
+                    const double L = lami[jb + jl] - base;
+                    tp[which] = first_grid_at_or_after(lam0, dlam, s, L, lane);
+                    if (tp[which] < 0 && sorted) {
+                        // later samples have larger lambda: none of them lands either
+                        jb = nw;
+                        break;
+                    }
                 }
             }
         }
+        if (lane == 0) {
+            tp_out[0] = tp[0];
+            tp_out[1] = tp[1];
+        }
     }
-    if (lane == 0) {
+}
+
+__global__ void __launch_bounds__(256)
+gk_resample_kernel(const GkRecord* __restrict__ rec, GkSamples smp, int32_t nup, int64_t n,
+                   const double* __restrict__ t0a, const double* __restrict__ lam0a,
+                   const double* __restrict__ dlama, int32_t s, double* __restrict__ u, double* __restrict__ mu,
+                   double* __restrict__ t, double* __restrict__ phi, double* __restrict__ affine,
+                   int32_t* __restrict__ tpmi_o, int32_t* __restrict__ tpri_o, int32_t* __restrict__ info) {
+    __shared__ double s_lam[kLdsLam];
+    const int64_t g = blockIdx.x;
+    if (g >= n) return;
+    const GkRecord& R = rec[g];
+    const int nw = R.nwritten;
+    const size_t o = (size_t)g * (size_t)nup;
+    const size_t so = (size_t)g * (size_t)s;
+    const double t0 = t0a[g], lam0 = lam0a[g], dlam = dlama[g];
+    int32_t tp[2];
+    if (nw <= kLdsLam) {
+        for (int j = threadIdx.x; j < nw; j += blockDim.x) s_lam[j] = smp.lami[o + j];
+        __syncthreads();
+        resample_geodesic(s_lam, R, nw, smp, o, t0, lam0, dlam, s, so, u, mu, t, phi, affine, tp);
+    } else {
+        resample_geodesic(smp.lami + o, R, nw, smp, o, t0, lam0, dlam, s, so, u, mu, t, phi, affine, tp);
+    }
+    if (threadIdx.x == 0) {
         tpmi_o[g] = tp[0];
         tpri_o[g] = tp[1];
         if (info) info[g] = 0;
@@ -409,7 +439,7 @@ int np_geokerr_eval_lambda_batch(const double* d_u0, const double* d_mu0, const 
     GkSamples smp;
     int rc = run_driver(in, 0, s, n, d_workspace, st, &rec, &smp);
     if (rc != NP_OK) return rc;
-    hipLaunchKernelGGL(gk_resample_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, rec, smp, s, n, d_t0,
+    hipLaunchKernelGGL(gk_resample_kernel, dim3((unsigned)n), dim3(256), 0, st, rec, smp, s, n, d_t0,
                        d_lam0, d_dlam, s, d_u, d_mu, d_t, d_phi, d_affine, d_tpmi, d_tpri, d_info);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
