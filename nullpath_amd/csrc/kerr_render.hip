@@ -657,18 +657,18 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 
 // theta enters as th + thd with th a float and thd a small float offset (the
 // low word of the float-float state plus the stage increment): sin and cos
-// of th are corrected by the rotation through thd (Taylor to thd^4).  With
-// theta rounded to one float instead, cos(theta) near the equator carries a
-// relative error of tan(theta) * 6e-8, which is what limits dp_theta.
-__device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, float th, float thd,
-                                               float pr, float pth, float E, float L) {
+// of th (s0, c0) are corrected by the rotation through thd (Taylor to
+// thd^4).  With theta rounded to one float instead, cos(theta) near the
+// equator carries a relative error of tan(theta) * 6e-8, which is what
+// limits dp_theta.  The stages of one RK4 run share th (only thd moves), so
+// callers evaluate sincos(th) once per run and pass it in.
+__device__ __forceinline__ DerivF kerr_rhs_f32_sc(const FrameConst& F, float r, float s0, float c0,
+                                                  float thd, float pr, float pth, float E, float L) {
     DerivF d;
     if (r <= F.rpf_guard) {
         d.r = d.th = d.ph = d.pr = d.pth = 0.0f;
         return d;
     }
-    float s0, c0;
-    sincosf(th, &s0, &c0);
     const float d2 = thd * thd;
     const float cd = 1.0f - 0.5f * d2 * (1.0f - d2 * (1.0f / 12.0f));
     const float sd = thd * (1.0f - d2 * (1.0f / 6.0f));
@@ -691,6 +691,13 @@ __device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, flo
     d.pr = ((r - M) * Q * iD - P * X * (iD * iD)) * iS;
     d.pth = c * s * w * (L * is2 + a * E) * iS;
     return d;
+}
+
+__device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, float th, float thd,
+                                               float pr, float pth, float E, float L) {
+    float s0, c0;
+    sincosf(th, &s0, &c0);
+    return kerr_rhs_f32_sc(F, r, s0, c0, thd, pr, pth, E, L);
 }
 
 // The same Carter-form field in FP64 (the FP64 steps of the FP32C path:
@@ -821,12 +828,14 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
                                        float L, float h, int& rhs_count) {
     const float hh = 0.5f * h;
     const float yr = ff_f(y.r), ypr = ff_f(y.pr), ypth = ff_f(y.pth);
-    const DerivF k2 = kerr_rhs_f32(F, yr + hh * k1.r, y.th.hi, y.th.lo + hh * k1.th, ypr + hh * k1.pr,
-                                   ypth + hh * k1.pth, E, L);
-    const DerivF k3 = kerr_rhs_f32(F, yr + hh * k2.r, y.th.hi, y.th.lo + hh * k2.th, ypr + hh * k2.pr,
-                                   ypth + hh * k2.pth, E, L);
-    const DerivF k4 = kerr_rhs_f32(F, yr + h * k3.r, y.th.hi, y.th.lo + h * k3.th, ypr + h * k3.pr,
-                                   ypth + h * k3.pth, E, L);
+    float s0, c0;
+    sincosf(y.th.hi, &s0, &c0);
+    const DerivF k2 = kerr_rhs_f32_sc(F, yr + hh * k1.r, s0, c0, y.th.lo + hh * k1.th, ypr + hh * k1.pr,
+                                      ypth + hh * k1.pth, E, L);
+    const DerivF k3 = kerr_rhs_f32_sc(F, yr + hh * k2.r, s0, c0, y.th.lo + hh * k2.th, ypr + hh * k2.pr,
+                                      ypth + hh * k2.pth, E, L);
+    const DerivF k4 = kerr_rhs_f32_sc(F, yr + h * k3.r, s0, c0, y.th.lo + h * k3.th, ypr + h * k3.pr,
+                                      ypth + h * k3.pth, E, L);
     rhs_count += 3;
     const float h6 = h * (1.0f / 6.0f);
     FF5 o;
@@ -839,19 +848,24 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
 }
 
 // FP32 counterpart of step_doubling_f64: same stage sequence, float RHS,
-// float stage sums, float-float updates of the state.
+// float stage sums, float-float updates of the state.  Stages 0-5 (the full
+// step's k2..k4 and the first half step's k2..k4) share the base theta of y,
+// stages 6-9 that of half1: two sincos per attempt.
 __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5& y, const DerivF& k1,
                                                   const float E, const float L, const float h,
                                                   FF5& full, FF5& half1, FF5& half2) {
     const float h2 = 0.5f * h;
     DerivF S = k1, kp = k1;
-    float br = ff_f(y.r), bth = y.th.hi, bthl = y.th.lo, bpr = ff_f(y.pr), bpth = ff_f(y.pth);
-#pragma clang loop unroll(disable)
+    float br = ff_f(y.r), bthl = y.th.lo, bpr = ff_f(y.pr), bpth = ff_f(y.pth);
+    float bs, bc;
+    sincosf(y.th.hi, &bs, &bc);
+    // unrolled: the stage coefficients below fold to constants
+#pragma unroll
     for (int j = 0; j < 10; ++j) {
         const float c = (j <= 1) ? 0.5f * h : (j == 2) ? h : (j == 5 || j == 9) ? h2 : 0.5f * h2;
         const float cc = (j == 6) ? 0.0f : c;
-        const DerivF k = kerr_rhs_f32(F, br + cc * kp.r, bth, bthl + cc * kp.th, bpr + cc * kp.pr,
-                                      bpth + cc * kp.pth, E, L);
+        const DerivF k = kerr_rhs_f32_sc(F, br + cc * kp.r, bs, bc, bthl + cc * kp.th, bpr + cc * kp.pr,
+                                         bpth + cc * kp.pth, E, L);
         if (j == 6) {
             S = k;
         } else {
@@ -874,7 +888,8 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
                 kp = k1;
             } else if (j == 5) {
                 half1 = o;
-                br = ff_f(o.r); bth = o.th.hi; bthl = o.th.lo; bpr = ff_f(o.pr); bpth = ff_f(o.pth);
+                br = ff_f(o.r); bthl = o.th.lo; bpr = ff_f(o.pr); bpth = ff_f(o.pth);
+                sincosf(o.th.hi, &bs, &bc);
             } else {
                 half2 = o;
             }
