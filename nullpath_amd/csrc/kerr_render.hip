@@ -850,15 +850,18 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
 // FP32 counterpart of step_doubling_f64: same stage sequence, float RHS,
 // float stage sums, float-float updates of the state.  Stages 0-5 (the full
 // step's k2..k4 and the first half step's k2..k4) share the base theta of y,
-// stages 6-9 that of half1: two sincos per attempt.
+// whose sin and cos (s0, c0) the caller passes in; stages 6-9 share that of
+// half1.  The full step is returned as its float increments inc = full - y
+// (all the error norm needs) and as float-float r and theta (for the domain
+// check); the half steps as float-float states.
 __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5& y, const DerivF& k1,
-                                                  const float E, const float L, const float h,
-                                                  FF5& full, FF5& half1, FF5& half2) {
+                                                  const float s0, const float c0, const float E,
+                                                  const float L, const float h, DerivF& inc, ff& full_r,
+                                                  ff& full_th, FF5& half1, FF5& half2) {
     const float h2 = 0.5f * h;
     DerivF S = k1, kp = k1;
     float br = ff_f(y.r), bthl = y.th.lo, bpr = ff_f(y.pr), bpth = ff_f(y.pth);
-    float bs, bc;
-    sincosf(y.th.hi, &bs, &bc);
+    float bs = s0, bc = c0;
     // unrolled: the stage coefficients below fold to constants
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
@@ -873,8 +876,15 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
             S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
         }
         kp = k;
-        if (j == 2 || j == 5 || j == 9) {
-            const float h6 = (j == 2 ? h : h2) * (1.0f / 6.0f);
+        if (j == 2) {
+            const float h6 = h * (1.0f / 6.0f);
+            inc = DerivF{h6 * S.r, h6 * S.th, h6 * S.ph, h6 * S.pr, h6 * S.pth};
+            full_r = ff_add(y.r, inc.r);
+            full_th = ff_add(y.th, inc.th);
+            S = k1;
+            kp = k1;
+        } else if (j == 5 || j == 9) {
+            const float h6 = h2 * (1.0f / 6.0f);
             const FF5& b = (j == 9) ? half1 : y;
             FF5 o;
             o.r = ff_add(b.r, h6 * S.r);
@@ -882,11 +892,7 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
             o.ph = ff_add(b.ph, h6 * S.ph);
             o.pr = ff_add(b.pr, h6 * S.pr);
             o.pth = ff_add(b.pth, h6 * S.pth);
-            if (j == 2) {
-                full = o;
-                S = k1;
-                kp = k1;
-            } else if (j == 5) {
+            if (j == 5) {
                 half1 = o;
                 br = ff_f(o.r); bthl = o.th.lo; bpr = ff_f(o.pr); bpth = ff_f(o.pth);
                 sincosf(o.th.hi, &bs, &bc);
@@ -904,16 +910,23 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
 // plus h |k1|).  Differences at that floor carry no truncation information;
 // counting them would reject sound far-field steps of components that pass
 // near zero (p_r at periapsis, p_theta at a theta turning point).
+// Both differences are increment-sized: full - y0 is the float increment
+// inc itself and full - half2 = inc - (half2 - y0), each within two ulps of
+// the increment, well under the floor.
 constexpr float kNoise = 16.0f * 5.9604645e-8f;
-__device__ __forceinline__ float errf(ff x, ff y, ff y0, float hk1, float rt, float at) {
-    const float sc = at + rt * fmaxf(fabsf(x.hi), fabsf(y.hi));
-    const float floor_ = kNoise * (fabsf(ff_diff(x, y0)) + fabsf(hk1));
-    return fmaxf(fabsf(ff_diff(x, y)) - floor_, 0.0f) / fmaxf(sc, 1e-30f);
+__device__ __forceinline__ float errf(float inc, ff y2, ff y0, float hk1, float rt, float at) {
+    const float d2 = ff_diff(y2, y0);
+    const float sc = at + rt * fmaxf(fabsf(y0.hi + inc), fabsf(y2.hi));
+    const float floor_ = kNoise * (fabsf(inc) + fabsf(hk1));
+    return fmaxf(fabsf(inc - d2) - floor_, 0.0f) * rcpf(fmaxf(sc, 1e-30f));
 }
 
-__device__ __forceinline__ bool okf(const FrameConst& F, const FF5& y) {
-    const double r = ff_to_d(y.r), th = ff_to_d(y.th);
-    return r > F.r_plus + 1e-12 && th > 1e-9 && th < NP_PI_D - 1e-9;
+// Domain check of a float attempt's result on the high words of r and theta
+// (thresholds of FrameConst); NaN fails.  Results in the sliver between
+// these thresholds and the FP64 bounds go to the FP64 re-run like any other
+// failed float attempt.
+__device__ __forceinline__ bool okf(const FrameConst& F, float r_hi, float th_hi) {
+    return r_hi > F.r_okf && th_hi > F.th_okf_lo && th_hi < F.th_okf_hi;
 }
 
 struct StepF32C {
@@ -1057,22 +1070,27 @@ struct StepF32C {
         }
         if (R.mode != 0) return attempt64(F, R, rhs64);
         const float h = R.h;
+        float s0, c0;
+        sincosf(R.y.th.hi, &s0, &c0);
         if (!R.k1_valid) {
-            R.k1 = kerr_rhs_f32(F, ff_f(R.y.r), R.y.th.hi, R.y.th.lo, ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
+            R.k1 = kerr_rhs_f32_sc(F, ff_f(R.y.r), s0, c0, R.y.th.lo, ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
             rhs_count += 1;
             R.k1_valid = true;
         }
-        FF5 full, half1, half2;
-        step_doubling_f32(F, R.y, R.k1, R.E, R.L, h, full, half1, half2);
+        DerivF inc;
+        ff full_r, full_th;
+        FF5 half1, half2;
+        step_doubling_f32(F, R.y, R.k1, s0, c0, R.E, R.L, h, inc, full_r, full_th, half1, half2);
         rhs_count += 10;
-        if (okf(F, full) && okf(F, half1) && okf(F, half2)) {
+        if (okf(F, full_r.hi, full_th.hi) && okf(F, half1.r.hi, half1.th.hi) &&
+            okf(F, half2.r.hi, half2.th.hi)) {
             const float rt = F.rtolf, at = F.atolf;
             const DerivF& k = R.k1;
-            const float e = fmaxf(fmaxf(errf(full.r, half2.r, R.y.r, h * k.r, rt, at),
-                                        errf(full.th, half2.th, R.y.th, h * k.th, rt, at)),
-                                  fmaxf(errf(full.ph, half2.ph, R.y.ph, h * k.ph, rt, at),
-                                        fmaxf(errf(full.pr, half2.pr, R.y.pr, h * k.pr, rt, at),
-                                              errf(full.pth, half2.pth, R.y.pth, h * k.pth, rt, at))));
+            const float e = fmaxf(fmaxf(errf(inc.r, half2.r, R.y.r, h * k.r, rt, at),
+                                        errf(inc.th, half2.th, R.y.th, h * k.th, rt, at)),
+                                  fmaxf(errf(inc.ph, half2.ph, R.y.ph, h * k.ph, rt, at),
+                                        fmaxf(errf(inc.pr, half2.pr, R.y.pr, h * k.pr, rt, at),
+                                              errf(inc.pth, half2.pth, R.y.pth, h * k.pth, rt, at))));
             if (e <= 1.0f) {
                 const float le = __log2f(fmaxf(1e-12f, e));
                 const float fac = fminf(2.0f, 0.9f * exp2f(-0.2f * le));

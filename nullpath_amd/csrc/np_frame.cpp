@@ -114,6 +114,22 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->rpf_guard = (float)(f->r_plus + 1e-12);
     f->rtolf = (float)p->rtol;
     f->atolf = (float)p->atol;
+    // A float-float value differs from its high word by at most 2^-24 of it
+    // (6e-8 relative in r; 1.9e-7 absolute in theta < pi): the thresholds
+    // sit 1e-6 (relative, absolute) inside the FP64 bounds and are rounded
+    // outward-safe, so passing them implies passing the FP64 check.
+    const auto up = [](double x) {
+        float v = (float)x;
+        return ((double)v < x) ? std::nextafter(v, HUGE_VALF) : v;
+    };
+    const auto down = [](double x) {
+        float v = (float)x;
+        return ((double)v > x) ? std::nextafter(v, -HUGE_VALF) : v;
+    };
+    const double pi = 3.141592653589793238462643383279502884;
+    f->r_okf = up((f->r_plus + 1e-12) * (1.0 + 1e-6));
+    f->th_okf_lo = up(1e-9 + 1e-6);
+    f->th_okf_hi = down(pi - 1e-9 - 1e-6);
 
     double g[5];
     cov_blocks(M, a, f->r_cam, f->th_cam, g);

@@ -28,6 +28,10 @@ struct FrameConst {
     double g_tt, g_tphi, g_phiphi, g_rr, g_thth;
     // FP32 copies for the compensated stepper
     float Mf, af, a2f, rpf, rmf, rpf_guard, rtolf, atolf;
+    // domain check of a float attempt on the high words of r and theta:
+    // r_hi > r_okf and th_okf_lo < th_hi < th_okf_hi imply the FP64 check
+    // r > r_+ + 1e-12, 1e-9 < theta < pi - 1e-9 of the full value
+    float r_okf, th_okf_lo, th_okf_hi;
     // float pinhole camera (render.cu:137-139)
     float aspect, tan_half, Rs, mass, disk_gain, emiss_q, bg_sigma_rad, bg_peak;
     uint32_t salt;        // jitter salt (0 = reference hash keys)
