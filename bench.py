@@ -31,11 +31,13 @@ sys.path.insert(0, ROOT)
 METRIC = "Kerr rays/s @1080p a=0.9, FP32-comp vs FP64 at ≤1e-5 err; adjoint grads/s"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
-# HBM bytes per ray of the FP32C renderer, measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE on the 480x270x1 frame and scaled by a copy kernel of known size
-# (profiles/kerr_render_f32c_480x270x1_hbm_traffic.txt): 13.8 MB read + 35.0 MB
-# written for 129,600 rays.
-FP32C_HBM_BYTES_PER_RAY = 376.0
+# HBM bytes of one FP32C render launch on the 1920x1080x4 frame (with ray
+# records), measured with rocprofv3 FETCH_SIZE and WRITE_SIZE and scaled by
+# what the counters read for a 256 MiB device copy (FETCH x2.00, WRITE x1.00;
+# profiles/kerr_render_f32c_1920x1080x4_hbm_traffic.txt): 578.3 MB read +
+# 790.9 MB written.  The algorithmic bytes are the 24.9 MB image and 265.4 MB
+# of ray records; the rest is register-spill scratch of the persistent lanes.
+FP32C_HBM_BYTES_1080P4 = (282358.5625 * 2.0 + 772341.625 * 1.0) * 1024.0
 
 # Algorithmic operation counts per unit (add, sub, mul, div, sqrt, rcp = 1;
 # fma = 2; library sin/cos/pow counted apart), derived from the kernel source:
@@ -44,14 +46,15 @@ FP32C_HBM_BYTES_PER_RAY = 376.0
 #     updates (3 x 196) plus the error norm and step factor (29).
 #   fp32c float steps: the Carter-form right-hand side with the split-theta
 #     sin/cos correction is 70 flops; per attempt 10 stage inputs and stage
-#     sums (10 x 18), 3 float-float updates (3 x 56) and the noise-floored
-#     error norm and step factor (85).
+#     sums (10 x 18), the full step's increments with float-float r and
+#     theta (26), the two half steps' float-float updates (2 x 56) and the
+#     noise-floored error norm and step factor (90).
 #   fp32c FP64 steps: the FP64 Carter right-hand side with its polynomial
 #     sincos is 105 flops; per attempt 12 stage inputs and sums (12 x 18),
 #     3 updates (3 x 10) and the error norm (40).
 FLOPS = {
     "fp64": {"rhs": 200, "attempt": 3 * 196 + 29},
-    "fp32c": {"rhs": 70, "attempt": 10 * 18 + 3 * 56 + 85,
+    "fp32c": {"rhs": 70, "attempt": 10 * 18 + 26 + 2 * 56 + 90,
               "rhs64": 105, "attempt64": 12 * 18 + 3 * 10 + 40},
 }
 
@@ -522,6 +525,7 @@ def main() -> int:
     value = R["rays"] / R["wall"]
     kernel = "kerr_render_f32c_w3<adaptive>" if head == "fp32c" else "kerr_render<StepF64, adaptive>"
     achieved = (R["flops32"] + 2.0 * R["flops64"]) / R["mean_s"] / 1e12
+    head_1080p4 = head == "fp32c" and (args.width, args.height, args.spp) == (1920, 1080, 4)
     line = {
         "metric": METRIC,
         "value": value,
@@ -548,8 +552,11 @@ def main() -> int:
             "peak": PEAK_TFLOPS["fp32"],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS["fp32"],
-            "traffic": (FP32C_HBM_BYTES_PER_RAY * R["rays"] / args.steps) if head == "fp32c" else None,
-            "traffic_note": "HBM bytes per launch: measured bytes per ray x rays (rocprofv3 --pmc)",
+            "traffic": FP32C_HBM_BYTES_1080P4 if head_1080p4 else None,
+            "traffic_note": ("HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE on this workload "
+                             "(copy-calibrated); mostly register-spill scratch of the persistent lanes"
+                             if head_1080p4 else "measured only for the 1920x1080x4 FP32C launch"),
+            "bytes_algorithmic": 12 * R["npix"] + 32 * R["rays"] // args.steps,
             "kernel": kernel,
             "flops_per_launch": {"fp32": R["flops32"] / args.steps, "fp64": R["flops64"] / args.steps},
             "note": "FP64 flops count twice (FP32-equivalent) since the FP64 rate is half",

@@ -105,7 +105,10 @@ __device__ __forceinline__ void driver_sample(GkVars& v, const GkRecord& R, int 
 
 // mode 0: geokerr_eval_lambda (uf given, SU = sign(uf - u0));
 // mode 1: geokerr_estimate_periods (one variable as U0 and UF, SU = 1).
-__global__ void __launch_bounds__(256)
+// The setup and sample kernels are capped at 256 registers (2 waves per
+// SIMD): left alone the compiler takes 256 VGPRs plus AGPRs, 1 wave per
+// SIMD, and these latency-bound FP64 chains run 4 % slower.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 gk_setup_kernel(GkInputs in, int mode, int32_t nup, int64_t n, GkRecord* __restrict__ rec) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (g >= n) return;
@@ -161,7 +164,7 @@ gk_setup_kernel(GkInputs in, int mode, int32_t nup, int64_t n, GkRecord* __restr
     rec[g] = R;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 gk_sample_kernel(const GkRecord* __restrict__ rec, int32_t nup, int64_t n, GkSamples s) {
     const int64_t total = (int64_t)nup * n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;

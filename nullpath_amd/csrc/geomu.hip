@@ -51,7 +51,10 @@ __device__ __forceinline__ void camera_constants(double alpha, double beta, doub
     sm = (mu0 < 1.0 && beta >= 0.0) ? 1.0 : -1.0;
 }
 
-__global__ void __launch_bounds__(256)
+// Both GEOMU kernels are capped at 256 registers (2 waves per SIMD): left
+// alone the compiler takes 256 VGPRs plus AGPRs, 1 wave per SIMD, and the
+// 2^20-ray batch runs 1.4x slower.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 geomu_camera_kernel(const double* __restrict__ alpha, const double* __restrict__ beta,
                     const double* __restrict__ spin, const double* __restrict__ mu0,
                     const double* __restrict__ uf, const double u0, const int64_t n,
@@ -69,7 +72,7 @@ geomu_camera_kernel(const double* __restrict__ alpha, const double* __restrict__
     }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 geomu_kernel(const double* __restrict__ u0, const double* __restrict__ uf,
              const double* __restrict__ mu0, const double* __restrict__ spin,
              const double* __restrict__ L, const double* __restrict__ Q2,
