@@ -234,6 +234,20 @@ def geokerr_phi_omega(u0, mu0, a, lz, q2):
     return lib().geokerr_phi_omega(u0, mu0, a, lz, q2)
 
 
+def geokerr_port_stream(u0, uf, mu0, a, lz, q2, lam0, dlam, s, symbol="geokerr_complete_stream_cuda"):
+    """The port-solver C entry points (include/geokerr_port.h):
+    geokerr_complete_stream_cuda or geokerr_stream_cuda.  Returns
+    (code, dict of u, mu, t, phi, affine host arrays); code 0 on success,
+    -1 / -2 / -3 as the reference."""
+    import ctypes as C
+    import numpy as np
+    arrs = [np.zeros(max(int(s), 0)) for _ in range(5)]
+    pd = C.POINTER(C.c_double)
+    code = getattr(lib(), symbol)(u0, uf, mu0, a, lz, q2, lam0, dlam, int(s),
+                                  *[x.ctypes.data_as(pd) for x in arrs])
+    return code, dict(zip(("u", "mu", "t", "phi", "affine"), arrs))
+
+
 def schw_sweep(b, M=10.0, r0=None, r_capture=None, ps_band=None, max_steps=2_000_000,
                stream=None, total_steps=None):
     """FP64 plain-method Schwarzschild sweep (np_schw_sweep) over a float64

@@ -154,6 +154,9 @@ def _declare(L: C.CDLL) -> None:
     L.geokerr_estimate_periods.restype = None
     L.geokerr_phi_omega.argtypes = [d] * 5
     L.geokerr_phi_omega.restype = d
+    for name in ("geokerr_complete_stream_cuda", "geokerr_stream_cuda"):
+        getattr(L, name).argtypes = [d] * 8 + [C.c_int] + [pd] * 5
+        getattr(L, name).restype = C.c_int
 
 
 class LookupTable(C.Structure):

@@ -493,6 +493,52 @@ struct OneCall {
 
 int gk_status(hipError_t e) { return e == hipSuccess ? NP_OK : NP_ELAUNCH; }
 
+// One geokerr_eval_lambda on the GPU for host arrays (S > 0): args = {u0,
+// mu0, uf, a, Lz, Q2, t0, lam0, dlam}; hosts = {u, mu, t, phi, affine}
+// (null entries are not copied back); ints_out = {tpmi, tpri, info}, left
+// as passed unless the call succeeds.  Returns an np status.
+int eval_lambda_one(const double args[9], int S, double* const hosts[5], int32_t ints_out[3]) {
+    using namespace np::gk;
+    double in[9];
+    for (int k = 0; k < 9; ++k) in[k] = args[k];
+    const size_t ws = align_up(workspace_bytes(S, 1));
+    const size_t outd = align_up(5 * (size_t)S * sizeof(double));
+    const size_t total = ws + align_up(sizeof(in)) + outd + align_up(3 * sizeof(int32_t));
+    OneCall m;
+    int rc = (hipMalloc((void**)&m.base, total) == hipSuccess) ? NP_OK : NP_EALLOC;
+    double* d_in = (double*)(m.base + ws);
+    double* d_out = (double*)(m.base + ws + align_up(sizeof(in)));
+    int32_t* d_ints = (int32_t*)(m.base + ws + align_up(sizeof(in)) + outd);
+    if (rc == NP_OK) rc = gk_status(hipMemcpy(d_in, in, sizeof(in), hipMemcpyHostToDevice));
+    if (rc == NP_OK)
+        rc = np_geokerr_eval_lambda_batch(d_in + 0, d_in + 1, d_in + 2, d_in + 3, d_in + 4, d_in + 5, d_in + 6,
+                                          d_in + 7, d_in + 8, S, 1, d_out, d_out + S, d_out + 2 * (size_t)S,
+                                          d_out + 3 * (size_t)S, d_out + 4 * (size_t)S, d_ints, d_ints + 1,
+                                          d_ints + 2, m.base, nullptr);
+    if (rc == NP_OK) rc = gk_status(hipStreamSynchronize(nullptr));
+    for (int k = 0; k < 5 && rc == NP_OK; ++k)
+        if (hosts[k]) rc = gk_status(hipMemcpy(hosts[k], d_out + k * (size_t)S, S * sizeof(double), hipMemcpyDeviceToHost));
+    int32_t ints[3];
+    if (rc == NP_OK) rc = gk_status(hipMemcpy(ints, d_ints, sizeof(ints), hipMemcpyDeviceToHost));
+    if (rc == NP_OK)
+        for (int k = 0; k < 3; ++k) ints_out[k] = ints[k];
+    return rc;
+}
+
+// The port-solver entry points: geokerr_eval_lambda with t0 = 0, return
+// codes 0 / -1 (bad arguments) / -2 (device allocation) / -3 (launch or
+// kernel failure).
+int port_stream(double u0, double uf, double mu0, double a, double L, double Q2, double lam0, double dlam, int S,
+                double* h_u, double* h_mu, double* h_t, double* h_phi, double* h_affine) {
+    if (S <= 0 || !h_u || !h_mu || !h_t || !h_phi || !h_affine) return -1;
+    if (!np::gk::kDriverEnabled) return -3;
+    const double in[9] = {u0, mu0, uf, a, L, Q2, 0.0, lam0, dlam};
+    double* const hosts[5] = {h_u, h_mu, h_t, h_phi, h_affine};
+    int32_t ints[3] = {-1, -1, 0};
+    const int rc = eval_lambda_one(in, S, hosts, ints);
+    return rc == NP_OK ? 0 : rc == NP_EALLOC ? -2 : -3;
+}
+
 }  // namespace
 
 void geokerr_eval_lambda(double u0, double mu0, double uf, double a, double Lz, double Q2, double t0, double lam0,
@@ -509,29 +555,23 @@ void geokerr_eval_lambda(double u0, double mu0, double uf, double a, double Lz, 
         return;
     }
     const double in[9] = {u0, mu0, uf, a, Lz, Q2, t0, lam0, dlam};
-    const size_t ws = align_up(workspace_bytes(S, 1));
-    const size_t outd = align_up(5 * (size_t)S * sizeof(double));
-    const size_t total = ws + align_up(sizeof(in)) + outd + align_up(3 * sizeof(int32_t));
-    OneCall m;
-    int rc = (hipMalloc((void**)&m.base, total) == hipSuccess) ? NP_OK : NP_EALLOC;
-    double* d_in = (double*)(m.base + ws);
-    double* d_out = (double*)(m.base + ws + align_up(sizeof(in)));
-    int32_t* d_ints = (int32_t*)(m.base + ws + align_up(sizeof(in)) + outd);
-    if (rc == NP_OK) rc = gk_status(hipMemcpy(d_in, in, sizeof(in), hipMemcpyHostToDevice));
-    if (rc == NP_OK)
-        rc = np_geokerr_eval_lambda_batch(d_in + 0, d_in + 1, d_in + 2, d_in + 3, d_in + 4, d_in + 5, d_in + 6,
-                                          d_in + 7, d_in + 8, S, 1, d_out, d_out + S, d_out + 2 * (size_t)S,
-                                          d_out + 3 * (size_t)S, d_out + 4 * (size_t)S, d_ints, d_ints + 1,
-                                          d_ints + 2, m.base, nullptr);
-    if (rc == NP_OK) rc = gk_status(hipStreamSynchronize(nullptr));
-    double* hosts[5] = {u, mu, t, phi, affine};
-    for (int k = 0; k < 5 && rc == NP_OK; ++k)
-        if (hosts[k]) rc = gk_status(hipMemcpy(hosts[k], d_out + k * (size_t)S, S * sizeof(double), hipMemcpyDeviceToHost));
+    double* const hosts[5] = {u, mu, t, phi, affine};
     int32_t ints[3] = {-1, -1, 0};
-    if (rc == NP_OK) rc = gk_status(hipMemcpy(ints, d_ints, sizeof(ints), hipMemcpyDeviceToHost));
+    const int rc = eval_lambda_one(in, S, hosts, ints);
     if (tpmi) *tpmi = ints[0];
     if (tpri) *tpri = ints[1];
     if (info) *info = (rc == NP_OK) ? ints[2] : NP_GEOKERR_INFO_DEVICE - rc;
+}
+
+int geokerr_complete_stream_cuda(double u0, double uf, double mu0, double a, double L, double Q2, double lam0,
+                                 double dlam, int S, double* h_u, double* h_mu, double* h_t, double* h_phi,
+                                 double* h_affine) {
+    return port_stream(u0, uf, mu0, a, L, Q2, lam0, dlam, S, h_u, h_mu, h_t, h_phi, h_affine);
+}
+
+int geokerr_stream_cuda(double u0, double uf, double mu0, double a, double L, double Q2, double lam0, double dlam,
+                        int S, double* h_u, double* h_mu, double* h_t, double* h_phi, double* h_affine) {
+    return port_stream(u0, uf, mu0, a, L, Q2, lam0, dlam, S, h_u, h_mu, h_t, h_phi, h_affine);
 }
 
 void geokerr_estimate_periods(double u0, double mu0, double a, double Lz, double Q2, double* Tr, double* Ttheta,

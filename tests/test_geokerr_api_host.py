@@ -1,6 +1,8 @@
-"""Host-side behaviour of the geokerr_ref_api drop-ins in libnullpath.so
-that needs no GPU: argument checks and geokerr_phi_omega (a host closed
-form) against the oracle."""
+"""Host-side behaviour of the geokerr_ref_api and port-solver drop-ins in
+libnullpath.so that needs no GPU: argument checks and geokerr_phi_omega (a
+host closed form) against the oracle."""
+import ctypes as C
+
 import numpy as np
 
 import nullpath_amd as npa
@@ -11,6 +13,19 @@ def test_eval_lambda_rejects_nonpositive_samples():
     r = npa.geokerr_eval_lambda(0.02, 0.3, 0.5, 0.9, 6.0, 10.0, 0.0, 0.0, 0.1, 0)
     assert r["info"] == 1
     assert npa.lib().np_status_string(-5) == b"entry point disabled in this build"
+
+
+def test_port_solver_rejects_bad_arguments():
+    # -1 before any device call, as geokerr_complete.cu:451-453 returns
+    pd = C.POINTER(C.c_double)
+    buf = C.cast((C.c_double * 4)(), pd)
+    null = C.cast(None, pd)
+    for sym in ("geokerr_complete_stream_cuda", "geokerr_stream_cuda"):
+        code, _ = npa.geokerr_port_stream(0.02, 0.5, 0.3, 0.9, 6.0, 10.0, 0.0, 0.1, 0, symbol=sym)
+        assert code == -1
+        fn = getattr(npa.lib(), sym)
+        assert fn(0.02, 0.5, 0.3, 0.9, 6.0, 10.0, 0.0, 0.1, 4, buf, null, buf, buf, buf) == -1
+        assert fn(0.02, 0.5, 0.3, 0.9, 6.0, 10.0, 0.0, 0.1, -3, buf, buf, buf, buf, buf) == -1
 
 
 def test_phi_omega_host_matches_oracle(oracle):

@@ -72,6 +72,25 @@ def test_drop_in_eval_lambda_matches_fortran_fixture(device):
 
 
 @pytest.mark.gpu
+def test_port_solver_symbols_match_fortran_fixture(device):
+    # the harness's port solver (gkval_adapters.c:661-695) passes the same
+    # inputs without t0 and compares the five fields with the Fortran's
+    f = np.load(GOLDEN, allow_pickle=False)
+    u0, mu0, uf, a, lz, q2, t0 = (float(v) for v in f["inputs"])
+    assert t0 == 0.0
+    ref = npa.geokerr_eval_lambda(u0, mu0, uf, a, lz, q2, 0.0, float(f["lam0"]), float(f["dlam"]), int(f["s"]))
+    for sym in ("geokerr_complete_stream_cuda", "geokerr_stream_cuda"):
+        print(f"[gpu] {sym} on the stored Fortran case, S=4096", flush=True)
+        code, r = npa.geokerr_port_stream(u0, uf, mu0, a, lz, q2, float(f["lam0"]), float(f["dlam"]),
+                                          int(f["s"]), symbol=sym)
+        assert code == 0
+        for k in ("u", "mu", "t", "phi", "affine"):
+            np.testing.assert_array_equal(r[k], ref[k])
+            e = _close(r[k], f[k])
+            assert e.max() <= 1e-11, (sym, k, e.max(), int(e.argmax()))
+
+
+@pytest.mark.gpu
 def test_eval_lambda_batch_matches_oracle(oracle, device):
     n, s = 96, 512
     u0, mu0, uf, a, lz, q2, t0 = _cases(n, 5)
