@@ -9,12 +9,12 @@
  *   geokerr_eval_lambda, geokerr_estimate_periods, geokerr_phi_omega
  *                                 geokerr_eval_lambda.f90:62-306
  *
- * The restatement is faithful to the Fortran, including where it is wrong:
- * the [-1,-1,-1,-4] and [-1,-1,-1,-1,-4] Carlson reductions it takes the
- * first term of Eq. (47) and the affine parameter (Eq. 49) from disagree
- * with quadrature, so t and lambda follow the reference formula, not the
- * integral.  Where the Fortran reads a variable it never set, the choice
- * made here is stated at the spot.
+ * The restatement follows the Fortran formula by formula.  The
+ * [-1,-1,-1,-4] and [-1,-1,-1,-1,-4] Carlson reductions it takes the first
+ * term of Eq. (47) and the affine parameter (Eq. 49) from are exact for
+ * U(u), which has no linear term (pinned against quadrature on U's own
+ * roots in the tests).  Where the Fortran reads a variable it never set,
+ * the choice made here is stated at the spot.
  *
  * TEST INFRASTRUCTURE ONLY (see np_oracle.h).
  */

@@ -504,7 +504,11 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
             u2 = -two * sqrt(qq) * cos((theta - two * pi) / 3.0) - cc / dd / 3.0;
             u3 = -two * sqrt(qq) * cos((theta + two * pi) / 3.0) - cc / dd / 3.0;
             iu1 = 0.0;
-            for (;;) {
+            /* The Fortran retries the case test after its forbidden-region
+             * edit without bound; where the edit does not leave the region
+             * (Q2 < 0) it cycles forever.  At most 3 tries here, as in the
+             * device code; the case then stays as it was. */
+            for (int guard = 0; guard < 3; ++guard) {
                 if (u0 <= u2) {
                     if (uf > u2) SET_UF(u2);
                     ncase = 1;
@@ -611,7 +615,7 @@ void orc_gk_geomu(orc_gk_vars* v, double* u0p, double* ufp, double mu0, double a
                 ncase = 6;
             } else {
                 u1 = creal(root[0]); u2 = creal(root[1]); u3 = creal(root[2]); u4 = creal(root[3]);
-                for (;;) {
+                for (int guard = 0; guard < 3; ++guard) {
                     if (u2 > uplus && u3 > uplus) { ncase = 5; break; }
                     else if (u0 <= u2) { ncase = 7; break; }
                     else if (u0 >= u3) { ncase = 8; break; }

@@ -123,6 +123,30 @@ typedef struct orc_geomu_out {
 void orc_geomu(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
                double su, double sm, orc_geomu_out* o);
 
+/* ---- GEOR (geokerr_original.f:2672-3134), FIRSTPT on, PHT off ---- */
+typedef struct orc_geor_out {
+    double uf;        /* final inverse radius (-1 where the reference returns it) */
+    double imu;       /* the mu integral from mu0 to muf after tpm turning points */
+    double iu0;       /* the u0 piece of the u integral */
+    double u1, u2, u3, u4;  /* real roots of U(u) where the case has them */
+    double h1;        /* Eq. (21) root, ncase 6 */
+    double u0_used;   /* after the reference's forbidden-region edit */
+    int32_t ncase;    /* Table 1 case 1..8 (0 = no solution) */
+    int32_t tpr;      /* radial turning points (as passed where not written) */
+    int32_t warned;   /* 1 when the reference would print a WARNING and modify u0 */
+    int32_t pad_;
+} orc_geor_out;
+
+double orc_calcimuasymf(double a, double mneg, double mpos, double muf, double muplus, double* rff);
+double orc_calcimusymf(double a, double mneg, double mpos, double muf, double muplus, double imum,
+                       double* rff);
+void orc_geor(double u0, double uf_in, double mu0, double muf, double a, double l, double q2, int32_t tpm,
+              int32_t tpr_in, double su, double sm, orc_geor_out* o);
+void orc_geor_batch(const double* u0, const double* uf_in, const double* mu0, const double* muf,
+                    const double* a, const double* l, const double* q2, const int32_t* tpm,
+                    const double* su, const double* sm, int n, double* uf, double* imu, int32_t* tpr,
+                    int32_t* ncase, int nthreads);
+
 /* GEOKERR's variables shared by its GEOMU and GEOPHITIME calls along one
  * geodesic (geokerr_original.f:398-412): written when FIRSTPT is true (or on
  * every call, where the Fortran does so), read otherwise. */

@@ -373,10 +373,25 @@ class GeomuOut(C.Structure):
                 ("warned", C.c_int32)]
 
 
+class GeorOut(C.Structure):
+    _fields_ = [("uf", C.c_double), ("imu", C.c_double), ("iu0", C.c_double), ("u1", C.c_double),
+                ("u2", C.c_double), ("u3", C.c_double), ("u4", C.c_double), ("h1", C.c_double),
+                ("u0_used", C.c_double), ("ncase", C.c_int32), ("tpr", C.c_int32),
+                ("warned", C.c_int32), ("pad_", C.c_int32)]
+
+
 def _declare_geomu(L):
     d, i32 = C.c_double, C.c_int
     L.orc_geomu.argtypes = [d, d, d, d, d, d, i32, d, d, C.POINTER(GeomuOut)]
     L.orc_geomu.restype = None
+    L.orc_geor.argtypes = [d] * 7 + [C.c_int32, C.c_int32, d, d, C.POINTER(GeorOut)]
+    L.orc_geor.restype = None
+    L.orc_geor_batch.argtypes = [C.c_void_p] * 10 + [i32] + [C.c_void_p] * 4 + [i32]
+    L.orc_geor_batch.restype = None
+    L.orc_calcimusymf.argtypes = [d] * 6 + [C.POINTER(d)]
+    L.orc_calcimusymf.restype = d
+    L.orc_calcimuasymf.argtypes = [d] * 5 + [C.POINTER(d)]
+    L.orc_calcimuasymf.restype = d
     L.orc_sncndn.argtypes = [d, d, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
     L.orc_sncndn.restype = None
     L.orc_zroots.argtypes = [C.c_void_p, i32, C.c_void_p, i32]
@@ -414,6 +429,30 @@ def geomu(u0, uf, mu0, a, l, q2, tpr=0, su=1.0, sm=1.0) -> dict:
     o = GeomuOut()
     _glib().orc_geomu(u0, uf, mu0, a, l, q2, tpr, su, sm, C.byref(o))
     return {k: getattr(o, k) for k, _ in GeomuOut._fields_}
+
+
+def geor(u0, mu0, muf, a, l, q2, tpm, su=1.0, sm=1.0, uf_in=-1.0, tpr_in=0) -> dict:
+    """GEOR (geokerr_original.f:2672-3134) with FIRSTPT: uf at muf after
+    tpm mu turning points; uf_in is the UF the caller passes in (read only
+    by the equal-root cubic branch)."""
+    o = GeorOut()
+    _glib().orc_geor(u0, uf_in, mu0, muf, a, l, q2, int(tpm), int(tpr_in), su, sm, C.byref(o))
+    return {k: getattr(o, k) for k, _ in GeorOut._fields_ if k != "pad_"}
+
+
+def geor_batch(u0, mu0, muf, a, l, q2, tpm, su, sm, uf_in=None, nthreads=0):
+    """orc_geor_batch over arrays of n: returns (uf, imu, tpr, ncase)."""
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (u0, mu0, muf, a, l, q2, su, sm)]
+    n = arr[0].size
+    tpm = np.ascontiguousarray(tpm, dtype=np.int32)
+    ufi = None if uf_in is None else np.ascontiguousarray(uf_in, dtype=np.float64)
+    uf, imu = np.zeros(n), np.zeros(n)
+    tpr, ncase = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    _glib().orc_geor_batch(arr[0].ctypes.data, None if ufi is None else ufi.ctypes.data, arr[1].ctypes.data,
+                           arr[2].ctypes.data, arr[3].ctypes.data, arr[4].ctypes.data, arr[5].ctypes.data,
+                           tpm.ctypes.data, arr[6].ctypes.data, arr[7].ctypes.data, n, uf.ctypes.data,
+                           imu.ctypes.data, tpr.ctypes.data, ncase.ctypes.data, nthreads)
+    return uf, imu, tpr, ncase
 
 
 def sncndn(u, emmc):
