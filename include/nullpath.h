@@ -171,6 +171,24 @@ int np_geomu_camera_batch(const double* d_alpha, const double* d_beta, const dou
                           double* d_muf, int32_t* d_tpm, int32_t* d_ncase, double* d_iu,
                           void* stream);
 
+/*
+ * GEOR for n rays (geokerr_original.f:2672-3134, FIRSTPT = .true., PHT =
+ * .false.): the inverse radius uf at which a photon from (u0[i], mu0[i])
+ * with constants of motion L[i], Q2[i], spin[i] and initial signs su[i],
+ * sm[i] reaches mu = muf[i] after tpm[i] mu turning points (with muf = 0,
+ * an equatorial crossing: the disk-crossing radius).  Outputs uf (-1 where the
+ * reference has no solution: equatorial orbits, a muf outside the allowed
+ * band, or more Mino time than remains before the horizon or infinity) and,
+ * optional, imu = the Mino-time integral, tpr = radial turning points passed
+ * (0 where the reference does not set it), ncase = Table 1 case (0 = no
+ * solution).  uf_in (optional, NULL = -1) is the UF value the reference's
+ * equal-root cubic branch reads.  Device arrays of n.
+ */
+int np_geor_batch(const double* d_u0, const double* d_mu0, const double* d_muf, const double* d_spin,
+                  const double* d_L, const double* d_Q2, const double* d_su, const double* d_sm,
+                  const int32_t* d_tpm, const double* d_uf_in, int64_t n, double* d_uf, double* d_imu,
+                  int32_t* d_tpr, int32_t* d_ncase, void* stream);
+
 /* Samples of the GEOKERR driver behind geokerr_estimate_periods
  * (geokerr_eval_lambda.f90:233). */
 #define NP_GEOKERR_PERIOD_SAMPLES 4096

@@ -150,6 +150,23 @@ def geomu_batch(u0, uf, mu0, spin, L, Q2, su, sm, tpr=None, stream=None):
     return muf, tpm, ncase, iu
 
 
+def geor_batch(u0, mu0, muf, spin, L, Q2, su, sm, tpm, uf_in=None, stream=None):
+    """GEOR on constants of motion (np_geor_batch): float64 device tensors
+    of n (tpm int32); returns (uf, imu, tpr, ncase) device tensors."""
+    import torch
+    n = u0.numel()
+    uf = torch.empty(n, dtype=torch.float64, device=u0.device)
+    imu = torch.empty_like(uf)
+    tpr = torch.empty(n, dtype=torch.int32, device=u0.device)
+    ncase = torch.empty_like(tpr)
+    code = lib().np_geor_batch(u0.data_ptr(), mu0.data_ptr(), muf.data_ptr(), spin.data_ptr(), L.data_ptr(),
+                               Q2.data_ptr(), su.data_ptr(), sm.data_ptr(), tpm.data_ptr(), _dptr(uf_in), n,
+                               uf.data_ptr(), imu.data_ptr(), tpr.data_ptr(), ncase.data_ptr(),
+                               _stream_ptr(stream))
+    check(code, "np_geor_batch")
+    return uf, imu, tpr, ncase
+
+
 GEOKERR_PERIOD_SAMPLES = 4096
 
 

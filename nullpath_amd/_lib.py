@@ -115,6 +115,8 @@ def _declare(L: C.CDLL) -> None:
     L.np_carlson_batch.restype = i32
     L.np_geomu_batch.argtypes = [vp] * 9 + [i64] + [vp] * 5
     L.np_geomu_batch.restype = i32
+    L.np_geor_batch.argtypes = [vp] * 10 + [i64] + [vp] * 5
+    L.np_geor_batch.restype = i32
     L.np_geomu_camera_batch.argtypes = [vp] * 5 + [d, i64] + [vp] * 5
     L.np_geomu_camera_batch.restype = i32
     f = C.c_float

@@ -107,6 +107,25 @@ carlson_kernel(const int kind, const double* __restrict__ x, const double* __res
     }
 }
 
+// GEOR per ray (one lane per ray); capped at 2 waves per SIMD like the GEOMU
+// kernels (same FP64 numerics, same register appetite).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+geor_kernel(const double* __restrict__ u0, const double* __restrict__ mu0, const double* __restrict__ muf,
+            const double* __restrict__ spin, const double* __restrict__ L, const double* __restrict__ Q2,
+            const double* __restrict__ su, const double* __restrict__ sm, const int32_t* __restrict__ tpm,
+            const double* __restrict__ uf_in, const int64_t n, double* __restrict__ uf, double* __restrict__ imu,
+            int32_t* __restrict__ tpr, int32_t* __restrict__ ncase) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const gk::GeorResult r = gk::geor(u0[i], uf_in ? uf_in[i] : -1.0, mu0[i], muf[i], spin[i], L[i], Q2[i],
+                                          tpm[i], 0, su[i], sm[i]);
+        uf[i] = r.uf;
+        if (imu) imu[i] = r.imu;
+        if (tpr) tpr[i] = r.tpr;
+        if (ncase) ncase[i] = r.ncase;
+    }
+}
+
 int grid_for(int64_t n) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
@@ -146,6 +165,19 @@ int np_geomu_batch(const double* d_u0, const double* d_uf, const double* d_mu0,
     hipLaunchKernelGGL(np::geomu_kernel, dim3(np::grid_for(n)), dim3(256), 0, (hipStream_t)stream,
                        d_u0, d_uf, d_mu0, d_spin, d_L, d_Q2, d_su, d_sm, d_tpr, n, d_muf, d_tpm,
                        d_ncase, d_iu);
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int np_geor_batch(const double* d_u0, const double* d_mu0, const double* d_muf, const double* d_spin,
+                  const double* d_L, const double* d_Q2, const double* d_su, const double* d_sm,
+                  const int32_t* d_tpm, const double* d_uf_in, int64_t n, double* d_uf, double* d_imu,
+                  int32_t* d_tpr, int32_t* d_ncase, void* stream) {
+    if (n < 0 || (n > 0 && (!d_u0 || !d_mu0 || !d_muf || !d_spin || !d_L || !d_Q2 || !d_su || !d_sm ||
+                            !d_tpm || !d_uf)))
+        return NP_EINVAL;
+    if (n == 0) return NP_OK;
+    hipLaunchKernelGGL(np::geor_kernel, dim3(np::grid_for(n)), dim3(256), 0, (hipStream_t)stream, d_u0, d_mu0,
+                       d_muf, d_spin, d_L, d_Q2, d_su, d_sm, d_tpm, d_uf_in, n, d_uf, d_imu, d_tpr, d_ncase);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

@@ -28,6 +28,14 @@ def test_port_solver_rejects_bad_arguments():
         assert fn(0.02, 0.5, 0.3, 0.9, 6.0, 10.0, 0.0, 0.1, -3, buf, buf, buf, buf, buf) == -1
 
 
+def test_geor_batch_argument_checks():
+    # rejected on the host before any device call; n = 0 is a no-op
+    L = npa.lib()
+    assert L.np_geor_batch(*([None] * 10), 1, *([None] * 5)) == -1
+    assert L.np_geor_batch(*([None] * 10), -1, *([None] * 5)) == -1
+    assert L.np_geor_batch(*([None] * 10), 0, *([None] * 5)) == 0
+
+
 def test_phi_omega_host_matches_oracle(oracle):
     rng = np.random.default_rng(17)
     for _ in range(500):
