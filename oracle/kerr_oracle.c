@@ -461,6 +461,8 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
         orc_init_static_d(M, a, r_cam, p->theta_cam, p->phi_cam, n[0], n[1], n[2], state);
 
     const double pt0 = state[PT_], pph0 = state[PPH_];
+    info->p0[0] = state[PT_]; info->p0[1] = state[PR_]; info->p0[2] = state[PTH_]; info->p0[3] = state[PPH_];
+    info->ncross = 0;
     double min_r = state[R_];
     int captured = 0, orders = 0;
     float disk_rgb[3] = {0, 0, 0};
@@ -521,6 +523,7 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
                     r_cross = orc_hermite_crossing(h_acc, prev[R_], prev[TH_], d0[R_], d0[TH_], state[R_],
                                                    state[TH_], d1[R_], d1[TH_], alpha, r_cross);
                 }
+                if (info->ncross < 4) info->xr[info->ncross++] = r_cross;
                 if (r_cross >= rin && r_cross <= rout) {
                     double cross[8];
                     memcpy(cross, state, sizeof(cross));

@@ -64,6 +64,9 @@ typedef struct orc_ray_info {
     double  min_r;
     float   inv[3];       /* |H|, dE/E, dL/L at crossings/ends (render.cu:233-244,327-345) */
     float   rgb[3];       /* this sample's colour */
+    int32_t ncross, pad_; /* equatorial crossings detected (in the disk or not), at most 4 kept */
+    double  xr[4];        /* their radii, as the crossing rule computes them */
+    double  p0[4];        /* covariant p_t, p_r, p_theta, p_phi at the camera */
 } orc_ray_info;
 
 /* Camera-space unit direction of sample `sample_id` of pixel (x, y)

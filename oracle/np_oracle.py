@@ -62,13 +62,15 @@ class RayInfo(C.Structure):
         ("cls", C.c_int32), ("orders", C.c_int32), ("steps", C.c_int64),
         ("attempts", C.c_int64), ("rhs_evals", C.c_int64), ("min_r", C.c_double),
         ("inv", C.c_float * 3), ("rgb", C.c_float * 3),
+        ("ncross", C.c_int32), ("pad_", C.c_int32), ("xr", C.c_double * 4), ("p0", C.c_double * 4),
     ]
 
 
 RAYINFO_DTYPE = np.dtype([
     ("cls", np.int32), ("orders", np.int32), ("steps", np.int64), ("attempts", np.int64),
     ("rhs_evals", np.int64), ("min_r", np.float64), ("inv", np.float32, 3),
-    ("rgb", np.float32, 3),
+    ("rgb", np.float32, 3), ("ncross", np.int32), ("pad_", np.int32), ("xr", np.float64, 4),
+    ("p0", np.float64, 4),
 ], align=True)
 
 
