@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--precisions", default="fp64,fp32c")
     ap.add_argument("--crossing", default="hermite", choices=["hermite", "linear"])
+    ap.add_argument("--waves-per-simd", type=int, default=0, choices=[0, 2, 3, 4],
+                    help="FP32C kernel variant (0 = the library default, 3 waves per SIMD)")
     ap.add_argument("--linear-check", default="960x540x1",
                     help="WxHxSPP of the linear-crossing comparison; 'none' skips it")
     ap.add_argument("--adjoint", type=int, default=1,
@@ -102,7 +104,7 @@ def copy_calibration(mib: int) -> int:
 
 
 def render_case(prec: str, crossing: str, width: int, height: int, spp: int, steps: int,
-                warmup: int) -> dict:
+                warmup: int, waves_per_simd: int = 0) -> dict:
     import torch
 
     import nullpath_amd as npa
@@ -114,7 +116,8 @@ def render_case(prec: str, crossing: str, width: int, height: int, spp: int, ste
     npix = width * height
     rays = npix * spp
     p = npa.kerr_disk_params(width=width, height=height, spin=0.9, theta_deg=80.0, fov=55.0,
-                             rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid, crossing=cr)
+                             rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid, crossing=cr,
+                             waves_per_simd=waves_per_simd)
     out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
     cnt = torch.zeros(10, dtype=torch.int64, device=dev)
     rec = torch.zeros(rays * 32, dtype=torch.uint8, device=dev)
@@ -519,13 +522,20 @@ def main() -> int:
         return 0
     precs = args.precisions.split(",")
     res = {prec: render_case(prec, args.crossing, args.width, args.height, args.spp, args.steps,
-                             args.warmup) for prec in precs}
+                             args.warmup, args.waves_per_simd) for prec in precs}
     head = "fp32c" if "fp32c" in res else precs[0]
     R = res[head]
     value = R["rays"] / R["wall"]
-    kernel = "kerr_render_f32c_w3<adaptive>" if head == "fp32c" else "kerr_render<StepF64, adaptive>"
+    waves = args.waves_per_simd or 3
+    if head != "fp32c":
+        kernel = "kerr_render<StepF64, adaptive>"
+    elif waves == 2:
+        kernel = "kerr_render<StepF32C, adaptive>"
+    else:
+        kernel = f"kerr_render_f32c_capped<adaptive, {waves} waves>"
     achieved = (R["flops32"] + 2.0 * R["flops64"]) / R["mean_s"] / 1e12
-    head_1080p4 = head == "fp32c" and (args.width, args.height, args.spp) == (1920, 1080, 4)
+    # the traffic constant was measured on the default kernel variant
+    head_1080p4 = head == "fp32c" and waves == 3 and (args.width, args.height, args.spp) == (1920, 1080, 4)
     line = {
         "metric": METRIC,
         "value": value,

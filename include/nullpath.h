@@ -90,7 +90,7 @@ int np_render_accumulate(const np_render_params* p, int spp_done, int samples_pe
 /*
  * Launch-shape choices of the batch renderer, picked by np_autotune_batch.
  * block_size: 64, 128 or 256 threads; waves_per_simd: FP32C kernel variant
- * (3 = 168-VGPR cap, 2 = compiler budget); refill_chunk: consecutive pixels
+ * (4 = 128-VGPR cap, 3 = 168-VGPR cap, 2 = compiler budget); refill_chunk: consecutive pixels
  * a lane takes per refill; fp64_radius: FP32C radius (absolute units) of
  * FP64 steps, 0 = each frame's own.
  */
@@ -109,7 +109,7 @@ typedef struct np_tuning {
  * (frame i starts at 3 * sum_{j<i} width_j * height_j floats; colour sums
  * over spp samples); d_records (optional) likewise with spp records per
  * pixel; d_counters (optional) sums over all frames.  tuning NULL = block
- * 256, 3-wave FP32C variant, chunk 1, frame radii.
+ * 256, the frames' own FP32C variant (3 waves by default), chunk 1, frame radii.
  */
 int np_render_batch(const np_render_params* frames, int nframes, int spp, int sample_offset,
                     float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
@@ -117,7 +117,7 @@ int np_render_batch(const np_render_params* frames, int nframes, int spp, int sa
 
 /*
  * Time every candidate tuning (block 128/256 x chunk 1/4, and for FP32C the
- * kernel variant 3/2 x FP64 radius 0.25/0.5/1.5 x the largest disk_rout) on
+ * kernel variant 4/3/2 x FP64 radius 0.25/0.5/1.5 x the largest disk_rout) on
  * the batch with each frame's resolution divided by `subsample`, and write
  * the fastest to *best.  Optional trials / trial_ms (capacity max_trials)
  * receive every candidate and its kernel time; *n_trials their count.  The

@@ -1307,17 +1307,20 @@ kerr_body(const FrameConst& F0, const BatchFrame* __restrict__ frames, const int
         counters, records, work, chunk
 
 // Kernel entry points.  Variant 0 leaves the register budget to the
-// compiler; variant 1 (FP32C only) asks for 3 waves per SIMD, which caps
-// the kernel at 168 VGPRs and moves the excess of the rarely taken FP64
-// branch to scratch.
+// compiler; the FP32C variants W = 3 and W = 4 ask for W waves per SIMD,
+// which caps the kernel at 168 or 128 VGPRs and moves the excess to
+// scratch.  At 168 that is the rarely taken FP64 branch; at 128 the step
+// loop spills too, which on the 1080p frame is 1.7 % faster but writes
+// about 1 TB of scratch back to HBM per frame (against 0.8 GB), so 3 is the
+// default.
 template <class S, bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
     kerr_body<S, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
 }
 
-template <bool ADAPTIVE, bool INV>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-kerr_render_f32c_w3(NP_KERNEL_ARGS) {
+template <bool ADAPTIVE, bool INV, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
+kerr_render_f32c_capped(NP_KERNEL_ARGS) {
     kerr_body<StepF32C, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
 }
 
@@ -1326,9 +1329,9 @@ __global__ void __launch_bounds__(256) kerr_batch(NP_BATCH_ARGS) {
     kerr_body<S, ADAPTIVE, false, true>(NP_BATCH_PASS);
 }
 
-template <bool ADAPTIVE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-kerr_batch_f32c_w3(NP_BATCH_ARGS) {
+template <bool ADAPTIVE, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
+kerr_batch_f32c_capped(NP_BATCH_ARGS) {
     kerr_body<StepF32C, ADAPTIVE, false, true>(NP_BATCH_PASS);
 }
 
@@ -1343,28 +1346,36 @@ const void* kernel_ptr(bool inv) {
                : reinterpret_cast<const void*>(&kerr_render<S, A, false>);
 }
 
-template <bool A>
-const void* kernel_ptr_w3(bool inv) {
-    return inv ? reinterpret_cast<const void*>(&kerr_render_f32c_w3<A, true>)
-               : reinterpret_cast<const void*>(&kerr_render_f32c_w3<A, false>);
+template <bool A, int W>
+const void* kernel_ptr_capped(bool inv) {
+    return inv ? reinterpret_cast<const void*>(&kerr_render_f32c_capped<A, true, W>)
+               : reinterpret_cast<const void*>(&kerr_render_f32c_capped<A, false, W>);
 }
 
+template <bool A>
+const void* kernel_ptr_f32c(bool inv, int variant) {
+    if (variant == 4) return kernel_ptr_capped<A, 4>(inv);
+    if (variant == 3) return kernel_ptr_capped<A, 3>(inv);
+    return kernel_ptr<StepF32C, A>(inv);
+}
+
+template <bool A>
+const void* batch_ptr_f32c(int variant) {
+    if (variant == 4) return reinterpret_cast<const void*>(&kerr_batch_f32c_capped<A, 4>);
+    if (variant == 3) return reinterpret_cast<const void*>(&kerr_batch_f32c_capped<A, 3>);
+    return reinterpret_cast<const void*>(&kerr_batch<StepF32C, A>);
+}
+
+// variant (FrameConst::variant): FP32C waves-per-SIMD cap, 0 = none.
 const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
-    if (precision == NP_PREC_FP32C) {
-        if (variant == 1) return adaptive ? kernel_ptr_w3<true>(inv) : kernel_ptr_w3<false>(inv);
-        return adaptive ? kernel_ptr<StepF32C, true>(inv) : kernel_ptr<StepF32C, false>(inv);
-    }
+    if (precision == NP_PREC_FP32C)
+        return adaptive ? kernel_ptr_f32c<true>(inv, variant) : kernel_ptr_f32c<false>(inv, variant);
     return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
 }
 
 const void* select_batch_kernel(int precision, bool adaptive, int variant) {
-    if (precision == NP_PREC_FP32C) {
-        if (variant == 1)
-            return adaptive ? reinterpret_cast<const void*>(&kerr_batch_f32c_w3<true>)
-                            : reinterpret_cast<const void*>(&kerr_batch_f32c_w3<false>);
-        return adaptive ? reinterpret_cast<const void*>(&kerr_batch<StepF32C, true>)
-                        : reinterpret_cast<const void*>(&kerr_batch<StepF32C, false>);
-    }
+    if (precision == NP_PREC_FP32C)
+        return adaptive ? batch_ptr_f32c<true>(variant) : batch_ptr_f32c<false>(variant);
     return adaptive ? reinterpret_cast<const void*>(&kerr_batch<StepF64, true>)
                     : reinterpret_cast<const void*>(&kerr_batch<StepF64, false>);
 }

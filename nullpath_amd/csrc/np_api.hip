@@ -236,7 +236,7 @@ int autotune_batch(const np_render_params* frames, int nframes, int spp, int sub
     std::vector<np_tuning> cand;
     const int blocks[] = {128, 256};
     const int chunks[] = {1, 4};
-    const int variants32[] = {3, 2};
+    const int variants32[] = {4, 3, 2};
     // FP64 radius: 1.5 r_out is the single-frame default; with the Hermite
     // crossing radius the smaller ones measured the same radiance error
     const double radii32[] = {0.25, 0.5, 1.5};

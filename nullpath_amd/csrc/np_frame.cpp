@@ -60,6 +60,7 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     if (!(std::fabs(p->spin) < 1.0f)) return NP_EINVAL;
     if (!(p->fov_y_deg > 0.0f && p->fov_y_deg < 179.0f)) return NP_EINVAL;
     if (p->max_steps < 0 || p->disk_max_orders < 1) return NP_EINVAL;
+    if (p->waves_per_simd != 0 && (p->waves_per_simd < 2 || p->waves_per_simd > 4)) return NP_EINVAL;
     if (p->integrator == NP_INTEGRATOR_ADAPTIVE && !(p->rtol > 0.0 && p->atol >= 0.0)) return NP_EINVAL;
 
     // render.cu:131-139 (float arithmetic)
@@ -102,7 +103,10 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->rtol = p->rtol;
     f->atol = p->atol;
     f->crossing = p->crossing == NP_CROSSING_HERMITE ? 1 : 0;
-    f->variant = (p->precision == NP_PREC_FP32C && p->waves_per_simd != 2) ? 1 : 0;
+    // FP32C kernel variant = waves-per-SIMD cap: 3 by default, 4, or 0 (the
+    // compiler's register budget, asked for as 2)
+    f->variant = (p->precision != NP_PREC_FP32C || p->waves_per_simd == 2) ? 0
+                 : (p->waves_per_simd == 4) ? 4 : 3;
     f->r_isl = p->fp64_radius > 0.0 ? p->fp64_radius : std::fmax(30.0 * M, 1.5 * p->disk_rout);
     f->s_eq = std::sin(3.141592653589793238462643383279502884 * 0.5);
     f->c_eq = std::cos(3.141592653589793238462643383279502884 * 0.5);

@@ -38,7 +38,7 @@ struct FrameConst {
     int32_t width, height;
     int32_t zamo;         // 1 when the ZAMO tetrad is used
     int32_t bg_mode, disk_on, disk_max_orders, check_inv, max_steps, emiss_m_units;
-    int32_t variant;      // kernel variant (FP32C: 1 = 3 waves per SIMD)
+    int32_t variant;      // kernel variant (FP32C: waves-per-SIMD cap 3 or 4; 0 = none)
     int32_t crossing;     // 0 linear crossing radius (reference), 1 cubic Hermite
 };
 

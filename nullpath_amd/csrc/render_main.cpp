@@ -11,8 +11,8 @@
 //   --emissivity-index q     (emissivity (r/M)^-q; default r^-2 absolute)
 //   --raw-out file.f32       (linear float32 RGB, accum / spp, row-major)
 //   --fp64-radius X          (fp32c: FP64 steps inside X M; default max(30, 1.5 disk-rout))
-//   --waves-per-simd 2|3     (fp32c kernel variant: 3 = capped at 168 VGPRs, the default;
-//                             2 = the compiler's register budget)
+//   --waves-per-simd 2|3|4   (fp32c kernel variant: 3 = capped at 168 VGPRs, the default;
+//                             4 = capped at 128 VGPRs; 2 = the compiler's register budget)
 //   --crossing linear|hermite (disk-crossing radius; default linear as the reference)
 #include <hip/hip_runtime.h>
 

@@ -120,16 +120,17 @@ def test_fp32c_matches_fp64_on_disk_workload(device):
 
 @pytest.mark.gpu
 def test_fp32c_kernel_variants_agree(device):
-    """The 168-VGPR kernel variant and the compiler-budget variant compute
-    the same thing: outputs and records are bitwise identical."""
+    """The 168-VGPR (default), 128-VGPR and compiler-budget kernel variants
+    compute the same thing: outputs and records are bitwise identical."""
     p3 = npa.kerr_disk_params(width=64, height=36, fov=6.0, seed=1, precision=1, crossing=1)
-    p2 = npa.kerr_disk_params(width=64, height=36, fov=6.0, seed=1, precision=1, crossing=1,
-                              waves_per_simd=2)
     i3, c3, r3 = _run_gpu(p3, 1)
-    i2, c2, r2 = _run_gpu(p2, 1)
-    assert np.array_equal(i3, i2)
-    assert np.array_equal(r3, r2)
-    assert c3 == c2
+    for w in (4, 2):
+        p = npa.kerr_disk_params(width=64, height=36, fov=6.0, seed=1, precision=1, crossing=1,
+                                 waves_per_simd=w)
+        i, c, r = _run_gpu(p, 1)
+        assert np.array_equal(i3, i), w
+        assert np.array_equal(r3, r), w
+        assert c3 == c, w
 
 
 def _batch_frames(precision):
@@ -169,7 +170,9 @@ def test_batch_equals_per_frame_renders(device, precision, tuning):
 def test_autotune_batch_reports_candidates(device):
     frames = _batch_frames(1)
     best, trials = npa.autotune_batch(frames, spp=1, subsample=2)
-    assert len(trials) == 24
+    # block 2 x chunk 2 x kernel variant 3 (4/3/2 waves) x FP64 radius 3
+    assert len(trials) == 36
+    assert {t["waves_per_simd"] for t, _ in trials} == {4, 3, 2}
     assert all(ms > 0 for _, ms in trials)
     fastest = min(trials, key=lambda t: t[1])[0]
     assert best.as_dict() == fastest
