@@ -753,3 +753,233 @@ void orc_geokerr_eval_lambda_batch(const double* u0, const double* mu0, const do
                                 t + o, phi + o, affine + o, tpmi + i, tpri + i, info + i);
     }
 }
+
+/* ---------------- the whole GEOKERR driver ---------------- */
+
+/* FINDMROOTS (geokerr_original.f:607-642): the mu turning points.  With
+ * a = 0 the Fortran divides by zero here; the IEEE results are kept. */
+void orc_findmroots(double q2, double l, double a, double mu0, double* muminus, double* muplus) {
+    double ql2 = q2 + l * l;
+    double amq = a * a - ql2;
+    double yy = -0.5 * (amq + sgn1(amq) * sqrt(amq * amq + 4.0 * q2 * a * a));
+    double mneg, mpos;
+    if (amq < 0.0) { mneg = -yy / a / a; mpos = q2 / yy; }
+    else { mneg = q2 / yy; mpos = -yy / a / a; }
+    if (mpos > 1.0) mpos = 1.0;
+    if (mneg > 0.0) {
+        *muplus = sgn1(mu0) * sqrt(mpos);
+        *muminus = sgn1(mu0) * sqrt(mneg);
+        if (fabs(*muminus) > fabs(mu0)) *muminus = mu0;
+        if (fabs(mu0) > fabs(*muplus)) *muplus = mu0;
+    } else {
+        *muplus = sqrt(mpos);
+        if (*muplus < mu0) *muplus = mu0;
+        *muminus = -*muplus;
+    }
+}
+
+/* (+-1) ** n for the Fortran's integer and default-real powers of -1. */
+static int m1pow(int n) { return (n & 1) ? -1 : 1; }
+
+/* INDEP_MUF (geokerr_original.f:645-706): the k-th of npts mu values spaced
+ * evenly in path length sum |dmu| from mu0 (after tpm0 turning points) to
+ * muf (after tpmf), and the turning points before it.  The Fortran mixes
+ * default-real (single precision) constants into integer expressions; for
+ * the small integers involved they are exact, and the float casts below
+ * keep the same operations. */
+void orc_indep_muf(double muminus, double muplus, double mu0, double muf, int32_t tpm0, int32_t tpmf,
+                   double sm, int32_t k, int32_t npts, double offset, double* mun, int32_t* tpmk_out) {
+    int tpmk;
+    if (muf <= muplus && muf >= muminus) {
+        int dtpm = tpmf - tpm0;
+        int a1 = f_int(sm * (double)m1pow(tpm0));
+        int a2 = f_int(sm * (double)m1pow(tpmf));
+        int a3 = 2 * (int)floorf((2.0f * (float)dtpm + 3.0f - (float)a1) / 4.0f) - 1;
+        double dmu = a1 * (muplus - mu0) + a2 * (muf - muminus) + a3 * (muplus - muminus);
+        double mu1 = (double)(((float)a1 + 1.0f) / 2.0f) * muplus + (double)((1.0f - (float)a1) / 2.0f) * muminus;
+        double mu2 = (double)((1.0f - (float)a1) / 2.0f) * muplus + (double)(((float)a1 + 1.0f) / 2.0f) * muminus;
+        double dmu1 = fabs(mu1 - mu0);
+        double dmu2 = fabs(mu2 - mu1);
+        double dmuk = ((double)k - offset) * dmu / (double)(float)npts;
+        int dmukdmu1 = f_int(dmuk / dmu1);
+        if (dmukdmu1 < 0) dmukdmu1 = 1;
+        int tp2 = f_int((dmuk - dmu1) / dmu2);
+        tpmk = (dmukdmu1 < 1 ? dmukdmu1 : 1) + (tp2 > 0 ? tp2 : 0);
+        int a1k = f_int(sm * (double)m1pow(tpm0));
+        int a2k = f_int(sm * (double)m1pow(tpmk + tpm0));
+        int a3k = (int)(2.0f * (float)(int)((2.0f * (float)tpmk + 3.0f - (float)a1k) / 4.0f) - 1.0f);
+        *mun = muminus + (double)(1.0f / (float)a2k) * (dmuk - a1k * (muplus - mu0) - a3k * (muplus - muminus));
+    } else {
+        *mun = 0.0;
+        tpmk = 0;
+    }
+    *tpmk_out = tpmk + tpm0;
+}
+
+void orc_gk_geokerr_full(const orc_gk_opts* op, orc_gk_args* g, double* ufi, double* mufi, double* dti,
+                         double* dphi, int32_t* tpmi, int32_t* tpri, double* lami, int32_t* ncase_out,
+                         int32_t* nup_out, int32_t* npts_out) {
+    const double one = 1.0, two = 2.0;
+    const int total = op->nup + op->npts - 2 * op->kext;
+    for (int i = 0; i < total; ++i) {
+        ufi[i] = mufi[i] = dti[i] = dphi[i] = lami[i] = NAN;
+        tpmi[i] = tpri[i] = -1;
+    }
+    orc_gk_vars v;
+    memset(&v, 0, sizeof v);
+    int nup = op->nup, npts = op->npts;
+    const int kext = op->kext, phit = op->phit;
+    const double offset = op->offset, uout = g->uout, mu0 = g->mu0, a = g->a, l = g->l, q2 = g->q2;
+    const double su = g->su, sm = g->sm;
+    double U0 = g->u0, UF = g->uf, MUF = g->muf;
+    int32_t TPM = g->tpm, TPR = g->tpr;
+    const double uplus = one / (one + sqrt(one - a * a));
+    /* GEOKERR's locals IU, TMU, TU, PHIMU, PHIU, LAMBDA start unset in the
+     * Fortran (read before any write when PHIT is false); 0 here. */
+    double IU = 0.0, TMU = 0.0, TU = 0.0, PHIMU = 0.0, PHIU = 0.0, LAMBDA = 0.0;
+    double MUN = 0.0, UN = 0.0;
+#define GK_STORE(idx1, u_, mu_, tpm_, tpr_)                    \
+    do {                                                        \
+        const int64_t j_ = (int64_t)(idx1)-1;                   \
+        if (j_ >= 0 && j_ < (int64_t)total) {                   \
+            ufi[j_] = (u_);                                     \
+            mufi[j_] = (mu_);                                   \
+            dti[j_] = TMU + TU;                                 \
+            dphi[j_] = PHIMU + PHIU;                            \
+            tpmi[j_] = (tpm_);                                  \
+            tpri[j_] = (tpr_);                                  \
+            lami[j_] = LAMBDA;                                  \
+        }                                                       \
+    } while (0)
+
+    if (op->usegeor) {
+        /* samples evenly in mu between mu0 and MUF (:421-467) */
+        for (int k = 1; k <= nup; ++k) {
+            MUN = mu0 + ((double)k - offset) * (MUF - mu0) / (double)nup;
+            if (k >= 2) TPM = f_int((sgn1(mu0) * sm + one) / two);
+            if (mu0 != 0.0 || g->beta != 0.0) {
+                orc_gk_geor(&v, &U0, &UF, mu0, MUN, a, l, q2, &IU, TPM, &TPR, su, sm, phit, k == 1, NULL);
+                if (phit)
+                    orc_gk_geophitime(&v, U0, UF, mu0, MUN, a, l, q2, TPM, TPR, su, sm, IU, 1, &PHIMU, &TMU, &PHIU,
+                                      &TU, &LAMBDA);
+            } else {
+                UF = U0;
+            }
+            GK_STORE(k, UF, MUN, TPM, TPR);
+        }
+    } else {
+        /* first solve on (U0, UF): case, roots, cached u0 / mu0 pieces */
+        orc_gk_geomu(&v, &U0, &UF, mu0, a, l, q2, &TPM, &TPR, su, sm, phit, 1, &MUF, &IU, NULL);
+        const int nc = v.ncase;
+        if ((nc > 2 && nc < 7) || (nc == 1 && su < 0.0) || (nc == 7 && su < 0.0) || (nc == 2 && su > 0.0) ||
+            (nc == 8 && su > 0.0))
+            TPR = 0;
+        double ub;
+        if (nc > 2 && nc < 7) {
+            ub = (su > 0.0) ? uplus : 0.0;
+        } else if (nc == 1 || nc == 7) {
+            ub = v.u2;
+            if (TPR == 1 && UF == ub) UF = uout;
+        } else {
+            ub = v.u3;
+            if (TPR == 1 && UF == ub) UF = uout;
+        }
+        int32_t TPR1 = 0;
+        double du = sgn1(ub - uout) * su * ((ub - uout) + (two * TPR - one) * (ub - UF));
+        du = du / (double)nup;
+        int kmax = f_int((ub - uout) / du + offset);
+        if (kmax > nup) kmax = nup;
+        if (du == 0.0) kmax = 0;
+        if (sgn1(uout - ub) != sgn1(uout - U0) || uout == U0) {
+            /* samples 1..KMAX from UOUT toward the turning point or UF; the
+             * first fills GEOPHITIME's u0 pieces (:495-537) */
+            if (kmax != 0) {
+                UN = uout + (one - offset) * du;
+                orc_gk_geomu(&v, &U0, &UN, mu0, a, l, q2, &TPM, &TPR1, su, sm, phit, 0, &MUN, &IU, NULL);
+                if (phit)
+                    orc_gk_geophitime(&v, U0, UN, mu0, MUN, a, l, q2, TPM, TPR1, su, sm, IU, 1, &PHIMU, &TMU, &PHIU,
+                                      &TU, &LAMBDA);
+                GK_STORE(1, UN, MUN, TPM, 0);
+            }
+            for (int k = 2; k <= kmax; ++k) {
+                UN = uout + ((double)k - offset) * du;
+                orc_gk_geomu(&v, &U0, &UN, mu0, a, l, q2, &TPM, &TPR1, su, sm, phit, 0, &MUN, &IU, NULL);
+                if (phit)
+                    orc_gk_geophitime(&v, U0, UN, mu0, MUN, a, l, q2, TPM, TPR1, su, sm, IU, 0, &PHIMU, &TMU, &PHIU,
+                                      &TU, &LAMBDA);
+                GK_STORE(k, UN, MUN, TPM, 0);
+            }
+            if (TPR == 1) {
+                const int firstpt = (nup == 1);
+                if (op->mufill) {
+                    /* mu as the independent variable across the u turning
+                     * point (:539-575): MUF and TPM1 on the far side, then
+                     * npts GEOR solves at INDEP_MUF's mu values.  The
+                     * Fortran reads MUFI(KMAX-KEXT) and TPMI(KMAX-KEXT); an
+                     * index before the array (KMAX <= KEXT) is taken as the
+                     * start of the geodesic (mu0, 0 turning points).  TPM1
+                     * is unset in the Fortran when GEOMU leaves TPM alone
+                     * (q2 = 0); it starts from TPM here. */
+                    int32_t TPM1 = TPM;
+                    UN = two * ub - (uout + ((double)((int64_t)kmax + kext + 1) - offset) * du);
+                    orc_gk_geomu(&v, &UF, &UN, mu0, a, l, q2, &TPM1, &TPR, su, sm, phit, 0, &MUN, &IU, NULL);
+                    MUF = MUN;
+                    double muminus, muplus;
+                    orc_findmroots(q2, l, a, mu0, &muminus, &muplus);
+                    const int64_t jb = (int64_t)kmax - kext;   /* 1-based */
+                    const double mustart = (jb >= 1 && jb <= total) ? mufi[jb - 1] : mu0;
+                    const int32_t tpmstart = (jb >= 1 && jb <= total) ? tpmi[jb - 1] : 0;
+                    for (int k = 1; k <= npts; ++k) {
+                        orc_indep_muf(muminus, muplus, mustart, MUF, tpmstart, TPM1, sm, k, npts, offset, &MUN, &TPM);
+                        orc_gk_geor(&v, &U0, &UN, mu0, MUN, a, l, q2, &IU, TPM, &TPR, su, sm, phit, 0, NULL);
+                        if (phit)
+                            orc_gk_geophitime(&v, U0, UN, mu0, MUN, a, l, q2, TPM, TPR, su, sm, IU, 0, &PHIMU, &TMU,
+                                              &PHIU, &TU, &LAMBDA);
+                        GK_STORE((int64_t)k + kmax - kext, UN, MUN, TPM, TPR);
+                    }
+                    npts = npts - 2 * kext;
+                }
+                /* back from the turning point to UF (:576-597); samples the
+                 * Fortran would store before its arrays are skipped */
+                int64_t kfirst = (int64_t)kmax + 1 + kext;
+                if (kfirst < 1 - (int64_t)npts) kfirst = 1 - (int64_t)npts;
+                for (int64_t k = kfirst; k <= nup; ++k) {
+                    UN = two * ub - (uout + ((double)k - offset) * du);
+                    orc_gk_geomu(&v, &UF, &UN, mu0, a, l, q2, &TPM, &TPR, su, sm, phit, 0, &MUN, &IU, NULL);
+                    if (phit)
+                        orc_gk_geophitime(&v, UF, UN, mu0, MUN, a, l, q2, TPM, TPR, su, sm, IU, firstpt, &PHIMU, &TMU,
+                                          &PHIU, &TU, &LAMBDA);
+                    GK_STORE(k + npts, UN, MUN, TPM, 1);
+                }
+                nup = nup + npts;
+            }
+        } else {
+            nup = 0;
+        }
+    }
+#undef GK_STORE
+    g->u0 = U0;
+    g->uf = UF;
+    g->muf = MUF;
+    g->tpm = TPM;
+    g->tpr = TPR;
+    *ncase_out = v.ncase;
+    *nup_out = nup;
+    *npts_out = npts;
+}
+
+void orc_gk_geokerr_full_batch(const orc_gk_opts* o, orc_gk_args* g, int n, double* ufi, double* mufi,
+                               double* dti, double* dphi, int32_t* tpmi, int32_t* tpri, double* lami,
+                               int32_t* ncase, int32_t* nup_out, int32_t* npts_out, int nthreads) {
+    const size_t total = (size_t)(o->nup + o->npts - 2 * o->kext);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int i = 0; i < n; ++i) {
+        const size_t s = (size_t)i * total;
+        orc_gk_geokerr_full(o, g + i, ufi + s, mufi + s, dti + s, dphi + s, tpmi + s, tpri + s, lami + s, ncase + i,
+                            nup_out + i, npts_out + i);
+    }
+    (void)nthreads;
+}

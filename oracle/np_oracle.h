@@ -176,6 +176,48 @@ void orc_gk_geophitime(orc_gk_vars* v, double u0, double uf, double mu0, double 
                        double iu, int firstpt, double* phimu, double* tmu, double* phiu,
                        double* tu, double* lambda);
 
+/* GEOR with the Fortran's argument contract: *u0, *uf, *tpr by reference,
+ * IMU written into *imu (left as it was on the asymmetric-roots exit), the
+ * shared variables in v written where FIRSTPT (or the Fortran
+ * unconditionally) writes them and read otherwise; PHT forms RFFU0/RFFU1/
+ * RFFMU2 for GEOPHITIME. */
+void orc_gk_geor(orc_gk_vars* v, double* u0, double* uf, double mu0, double muf, double a, double l,
+                 double q2, double* imu, int32_t tpm, int32_t* tpr, double su, double sm, int pht,
+                 int firstpt, int32_t* warned);
+
+/* FINDMROOTS (geokerr_original.f:607-642) and INDEP_MUF (:645-706). */
+void orc_findmroots(double q2, double l, double a, double mu0, double* muminus, double* muplus);
+void orc_indep_muf(double muminus, double muplus, double mu0, double muf, int32_t tpm0, int32_t tpmf,
+                   double sm, int32_t k, int32_t npts, double offset, double* mun, int32_t* tpmk);
+
+/* The GEOKERR driver's switches and per-geodesic arguments
+ * (geokerr_original.f:357-360).  ALPHA is not read by the driver. */
+typedef struct orc_gk_opts {
+    int32_t nup, npts, kext, phit, usegeor, mufill;
+    double offset;
+} orc_gk_opts;
+typedef struct orc_gk_args {
+    double u0, uf, uout, mu0, muf, a, l, q2, beta, su, sm;
+    int32_t tpm, tpr;
+} orc_gk_args;
+
+/* The whole GEOKERR driver (geokerr_original.f:357-604): the USEGEOR
+ * branch (samples evenly in mu, GEOR + GEOPHITIME) and the u-sampled branch
+ * with the MUFILL switch to mu near the u turning point (INDEP_MUF + GEOR).
+ * Arrays hold nup + npts - 2 kext entries; entries the driver does not
+ * write are NaN (doubles) and -1 (ints).  g comes back as the Fortran
+ * leaves its by-reference arguments (U0, UF, MUF, TPM, TPR); *ncase,
+ * *nup_out and *npts_out are NCASE, NUP and NPTS on return.  Samples the
+ * Fortran would store outside its arrays are not computed. */
+void orc_gk_geokerr_full(const orc_gk_opts* o, orc_gk_args* g, double* ufi, double* mufi, double* dti,
+                         double* dphi, int32_t* tpmi, int32_t* tpri, double* lami, int32_t* ncase,
+                         int32_t* nup_out, int32_t* npts_out);
+/* n independent drivers; g is n entries (in/out), the arrays n x (nup +
+ * npts - 2 kext), OpenMP. */
+void orc_gk_geokerr_full_batch(const orc_gk_opts* o, orc_gk_args* g, int n, double* ufi, double* mufi,
+                               double* dti, double* dphi, int32_t* tpmi, int32_t* tpri, double* lami,
+                               int32_t* ncase, int32_t* nup_out, int32_t* npts_out, int nthreads);
+
 /* GEOKERR driver (geokerr_original.f:357-604) in the configuration
  * geokerr_eval_lambda and geokerr_estimate_periods use: USEGEOR = .false.,
  * PHIT = .true., MUFILL with NPTS = KEXT = 0, ALPHA = BETA = 0.  uf_is_u0

@@ -547,6 +547,62 @@ def geokerr_driver(u0, uf, mu0, a, l, q2, nup, su=None, sm=None, tpr=1, offset=0
     return out
 
 
+class GkOpts(C.Structure):
+    """orc_gk_opts: the GEOKERR driver's switches (geokerr_original.f:357-360)."""
+    _fields_ = [("nup", C.c_int32), ("npts", C.c_int32), ("kext", C.c_int32), ("phit", C.c_int32),
+                ("usegeor", C.c_int32), ("mufill", C.c_int32), ("offset", C.c_double)]
+
+
+GK_ARGS_DTYPE = np.dtype([(k, np.float64) for k in ("u0", "uf", "uout", "mu0", "muf", "a", "l", "q2",
+                                                     "beta", "su", "sm")] +
+                         [("tpm", np.int32), ("tpr", np.int32)], align=True)
+
+
+def findmroots(q2, l, a, mu0):
+    L = _gklib()
+    L.orc_findmroots.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)] * 2
+    L.orc_findmroots.restype = None
+    mm, mp = C.c_double(), C.c_double()
+    L.orc_findmroots(q2, l, a, mu0, C.byref(mm), C.byref(mp))
+    return mm.value, mp.value
+
+
+def indep_muf(muminus, muplus, mu0, muf, tpm0, tpmf, sm, k, npts, offset):
+    L = _gklib()
+    L.orc_indep_muf.argtypes = [C.c_double] * 4 + [C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32,
+                                                   C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.orc_indep_muf.restype = None
+    mun, tpmk = C.c_double(), C.c_int32()
+    L.orc_indep_muf(muminus, muplus, mu0, muf, tpm0, tpmf, sm, k, npts, offset, C.byref(mun), C.byref(tpmk))
+    return mun.value, tpmk.value
+
+
+def geokerr_full_batch(opts: dict, args: np.ndarray, nthreads=0) -> dict:
+    """The whole GEOKERR driver (orc_gk_geokerr_full) over n geodesics.
+    opts: nup, npts, kext, phit, usegeor, mufill, offset.  args: a
+    GK_ARGS_DTYPE array (copied; the returned 'args' holds the by-reference
+    arguments as the driver leaves them).  Sample arrays are n x (nup +
+    npts - 2 kext)."""
+    L = _gklib()
+    L.orc_gk_geokerr_full_batch.argtypes = [C.POINTER(GkOpts), C.c_void_p, C.c_int] + [C.c_void_p] * 10 + [C.c_int]
+    L.orc_gk_geokerr_full_batch.restype = None
+    o = GkOpts(**{k: opts[k] for k in ("nup", "npts", "kext", "phit", "usegeor", "mufill", "offset")})
+    g = np.ascontiguousarray(args, dtype=GK_ARGS_DTYPE).copy()
+    n = len(g)
+    tot = o.nup + o.npts - 2 * o.kext
+    out = {k: np.empty((n, tot)) for k in ("ufi", "mufi", "dti", "dphi", "lami")}
+    out["tpmi"] = np.empty((n, tot), np.int32)
+    out["tpri"] = np.empty((n, tot), np.int32)
+    for k in ("ncase", "nup_out", "npts_out"):
+        out[k] = np.empty(n, np.int32)
+    L.orc_gk_geokerr_full_batch(C.byref(o), g.ctypes.data, n, out["ufi"].ctypes.data, out["mufi"].ctypes.data,
+                                out["dti"].ctypes.data, out["dphi"].ctypes.data, out["tpmi"].ctypes.data,
+                                out["tpri"].ctypes.data, out["lami"].ctypes.data, out["ncase"].ctypes.data,
+                                out["nup_out"].ctypes.data, out["npts_out"].ctypes.data, nthreads)
+    out["args"] = g
+    return out
+
+
 def geokerr_eval_lambda(u0, mu0, uf, a, lz, q2, t0, lam0, dlam, s):
     arrs = [np.zeros(max(s, 0)) for _ in range(5)]
     tpmi, tpri, info = C.c_int(), C.c_int(), C.c_int()
