@@ -235,6 +235,46 @@ int np_geokerr_phi_omega_batch(const double* d_u0, const double* d_mu0, const do
                                const double* d_lz, const double* d_q2, int64_t n, double* d_out,
                                void* stream);
 
+/*
+ * The whole GEOKERR driver (SUBROUTINE GEOKERR, geokerr_original.f:357-604)
+ * for n geodesics: the USEGEOR branch (NUP samples evenly in mu between MU0
+ * and MUF, each a GEOR + GEOPHITIME solve) and the u-sampled branch with
+ * the MUFILL switch to mu as the independent variable across the u turning
+ * point (FINDMROOTS :607-642, INDEP_MUF :645-706, GEOR).  The switches and
+ * sizes are the Fortran's NUP, NPTS, KEXT, PHIT, USEGEOR, MUFILL, OFFSET;
+ * per geodesic the Fortran's U0, UF, UOUT, MU0, MUF, A, L, Q2, BETA, TPM,
+ * TPR, SU, SM (ALPHA is not read by the driver).  Each geodesic's sample
+ * arrays have NUP + NPTS - 2 KEXT entries (the Fortran's DIMENSION), row
+ * after row; entries the driver does not write are NaN (doubles) and -1
+ * (ints), and samples the Fortran would store outside its arrays are not
+ * computed.  With PHIT false DTI, DPHI and LAMBDAI are 0 (the Fortran
+ * stores unset locals).  The *_out arrays (each may be NULL) receive the
+ * by-reference arguments as the driver leaves them and NCASE, NUP, NPTS.
+ * NP_EINVAL for NUP < 1, NPTS < 0, KEXT < 0, NUP + NPTS - 2 KEXT < 1, a
+ * NULL input or sample array, n < 0.  Numerics follow the oracle's
+ * restatement, including the reference's quirks stated there.
+ */
+typedef struct np_geokerr_opts {
+    int32_t nup, npts, kext;
+    int32_t phit, usegeor, mufill;   /* 0 / 1 */
+    double offset;
+} np_geokerr_opts;
+
+typedef struct np_geokerr_rays {     /* device arrays, n entries each */
+    const double *u0, *uf, *uout, *mu0, *muf, *a, *l, *q2, *beta, *su, *sm;
+    const int32_t *tpm, *tpr;
+} np_geokerr_rays;
+
+typedef struct np_geokerr_samples {  /* device arrays */
+    double *ufi, *mufi, *dti, *dphi, *lambdai;   /* n x (nup + npts - 2 kext) */
+    int32_t *tpmi, *tpri;                        /* n x (nup + npts - 2 kext) */
+    double *u0_out, *uf_out, *muf_out;           /* n each, or NULL */
+    int32_t *tpm_out, *tpr_out, *ncase_out, *nup_out, *npts_out;   /* n each, or NULL */
+} np_geokerr_samples;
+
+int np_geokerr_batch(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
+                     const np_geokerr_samples* out, void* stream);
+
 /* ---------------------------------------------------------------------
  * Schwarzschild equatorial sweeps (schwarzschild_blackhole.cu)
  * ------------------------------------------------------------------- */

@@ -6,6 +6,7 @@ torch tensors on a HIP device; torch is used only for memory and streams.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 from ._lib import (COUNTER_FIELDS, NullPathError, RenderParams, TraceCounters, check, lib)
@@ -165,6 +166,38 @@ def geor_batch(u0, mu0, muf, spin, L, Q2, su, sm, tpm, uf_in=None, stream=None):
                                _stream_ptr(stream))
     check(code, "np_geor_batch")
     return uf, imu, tpr, ncase
+
+
+def geokerr_batch(opts, rays, stream=None):
+    """The whole GEOKERR driver (np_geokerr_batch) for n geodesics.
+
+    opts: dict with nup, npts, kext, phit, usegeor, mufill, offset (the
+    Fortran's switches, geokerr_original.f:357-360).  rays: dict of device
+    tensors, n entries each -- float64 u0, uf, uout, mu0, muf, a, l, q2,
+    beta, su, sm and int32 tpm, tpr.  Returns a dict of device tensors:
+    ufi, mufi, dti, dphi, lambdai (float64) and tpmi, tpri (int32), each
+    n x (nup + npts - 2 kext), and per geodesic u0_out, uf_out, muf_out,
+    tpm_out, tpr_out, ncase_out, nup_out, npts_out."""
+    import torch
+    from ._lib import GEOKERR_RAY_FIELDS, GeokerrOpts, GeokerrRays, GeokerrSamples
+    o = GeokerrOpts(**{k: opts[k] for k in ("nup", "npts", "kext", "phit", "usegeor", "mufill", "offset")})
+    n = rays["u0"].numel()
+    dev = rays["u0"].device
+    tot = max(int(o.nup) + int(o.npts) - 2 * int(o.kext), 0)
+    for k in GEOKERR_RAY_FIELDS:
+        want = torch.int32 if k in ("tpm", "tpr") else torch.float64
+        t = rays[k]
+        if t.dtype != want or t.numel() != n or not t.is_contiguous() or t.device != dev:
+            raise ValueError(f"geokerr_batch: rays[{k!r}] must be a contiguous {want} tensor of {n} on {dev}")
+    out = {k: torch.empty((n, tot), dtype=torch.float64, device=dev) for k in ("ufi", "mufi", "dti", "dphi", "lambdai")}
+    out.update({k: torch.empty((n, tot), dtype=torch.int32, device=dev) for k in ("tpmi", "tpri")})
+    out.update({k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("u0_out", "uf_out", "muf_out")})
+    out.update({k: torch.empty(n, dtype=torch.int32, device=dev)
+                for k in ("tpm_out", "tpr_out", "ncase_out", "nup_out", "npts_out")})
+    r = GeokerrRays(**{k: rays[k].data_ptr() for k in GEOKERR_RAY_FIELDS})
+    s = GeokerrSamples(**{k: v.data_ptr() for k, v in out.items()})
+    check(lib().np_geokerr_batch(C.byref(o), C.byref(r), n, C.byref(s), _stream_ptr(stream)), "np_geokerr_batch")
+    return out
 
 
 GEOKERR_PERIOD_SAMPLES = 4096

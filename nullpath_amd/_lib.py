@@ -102,6 +102,27 @@ def lib() -> C.CDLL:
     return _lib
 
 
+class GeokerrOpts(C.Structure):
+    """np_geokerr_opts (nullpath.h): the GEOKERR driver's switches."""
+    _fields_ = [("nup", C.c_int32), ("npts", C.c_int32), ("kext", C.c_int32), ("phit", C.c_int32),
+                ("usegeor", C.c_int32), ("mufill", C.c_int32), ("offset", C.c_double)]
+
+
+GEOKERR_RAY_FIELDS = ("u0", "uf", "uout", "mu0", "muf", "a", "l", "q2", "beta", "su", "sm", "tpm", "tpr")
+GEOKERR_SAMPLE_FIELDS = ("ufi", "mufi", "dti", "dphi", "lambdai", "tpmi", "tpri", "u0_out", "uf_out", "muf_out",
+                         "tpm_out", "tpr_out", "ncase_out", "nup_out", "npts_out")
+
+
+class GeokerrRays(C.Structure):
+    """np_geokerr_rays: device pointers, n entries each."""
+    _fields_ = [(k, C.c_void_p) for k in GEOKERR_RAY_FIELDS]
+
+
+class GeokerrSamples(C.Structure):
+    """np_geokerr_samples: device pointers."""
+    _fields_ = [(k, C.c_void_p) for k in GEOKERR_SAMPLE_FIELDS]
+
+
 def _declare(L: C.CDLL) -> None:
     i32, vp = C.c_int, C.c_void_p
     L.np_status_string.argtypes = [i32]
@@ -117,6 +138,8 @@ def _declare(L: C.CDLL) -> None:
     L.np_geomu_batch.restype = i32
     L.np_geor_batch.argtypes = [vp] * 10 + [i64] + [vp] * 5
     L.np_geor_batch.restype = i32
+    L.np_geokerr_batch.argtypes = [C.POINTER(GeokerrOpts), C.POINTER(GeokerrRays), i64, C.POINTER(GeokerrSamples), vp]
+    L.np_geokerr_batch.restype = i32
     L.np_geomu_camera_batch.argtypes = [vp] * 5 + [d, i64] + [vp] * 5
     L.np_geomu_camera_batch.restype = i32
     f = C.c_float

@@ -1,7 +1,9 @@
 // geokerr.hip -- gfx950 GEOKERR driver and the geokerr_ref_api entry points
 // (geokerr_port/reference/geokerr_eval_lambda.f90, GEOKERR driver
-// geokerr_original.f:357-604 with USEGEOR = .false., PHIT = .true., MUFILL
-// and NPTS = KEXT = 0).
+// geokerr_original.f:357-604).  geokerr_eval_lambda's configuration
+// (USEGEOR = .false., PHIT = .true., MUFILL with NPTS = KEXT = 0) runs
+// through the split kernels below; np_geokerr_batch runs every
+// configuration of the driver, one lane per geodesic (gk_full_*_kernel).
 //
 // The Fortran samples one geodesic sequentially: a first GEOMU on (u0, uf)
 // fixes the case, roots and the u0/mu0 pieces; every later GEOMU/GEOPHITIME
@@ -384,6 +386,192 @@ gk_phi_omega_kernel(const double* __restrict__ u0, const double* __restrict__ mu
         out[i] = phi_omega(u0[i], mu0[i], a[i], lz[i]);
 }
 
+// ------------------------ the whole GEOKERR driver ------------------------
+// One lane per geodesic running the Fortran's solves in its order
+// (geokerr_original.f:357-604), every by-reference argument and shared
+// variable carried as the Fortran carries it.  The u-sampled branch runs as
+// one loop over its steps (first leg, MUFILL's far-side solve, MUFILL
+// samples, the leg back from the turning point) so that GEOMU, GEOR and
+// each GEOPHITIME form are inlined once.
+
+__device__ __forceinline__ void full_prefill(const np_geokerr_samples& s, size_t base, int total) {
+    const double qnan = __builtin_nan("");
+    for (int j = 0; j < total; ++j) {
+        s.ufi[base + j] = qnan;
+        s.mufi[base + j] = qnan;
+        s.dti[base + j] = qnan;
+        s.dphi[base + j] = qnan;
+        s.lambdai[base + j] = qnan;
+        s.tpmi[base + j] = -1;
+        s.tpri[base + j] = -1;
+    }
+}
+
+__device__ __forceinline__ void full_store(const np_geokerr_samples& s, size_t base, int total, int64_t idx1,
+                                           double u, double mu, double dt, double dphi, int tpm, int tpr,
+                                           double lam) {
+    const int64_t j = idx1 - 1;
+    if (j < 0 || j >= (int64_t)total) return;
+    s.ufi[base + j] = u;
+    s.mufi[base + j] = mu;
+    s.dti[base + j] = dt;
+    s.dphi[base + j] = dphi;
+    s.tpmi[base + j] = tpm;
+    s.tpri[base + j] = tpr;
+    s.lambdai[base + j] = lam;
+}
+
+__device__ __forceinline__ void full_finish(const np_geokerr_samples& s, int64_t g, double U0, double UF, double MUF,
+                                            int TPM, int TPR, int ncase, int nup, int npts) {
+    if (s.u0_out) s.u0_out[g] = U0;
+    if (s.uf_out) s.uf_out[g] = UF;
+    if (s.muf_out) s.muf_out[g] = MUF;
+    if (s.tpm_out) s.tpm_out[g] = TPM;
+    if (s.tpr_out) s.tpr_out[g] = TPR;
+    if (s.ncase_out) s.ncase_out[g] = ncase;
+    if (s.nup_out) s.nup_out[g] = nup;
+    if (s.npts_out) s.npts_out[g] = npts;
+}
+
+// USEGEOR: samples evenly in mu from MU0 to MUF (:421-467).  GEOR and
+// GEOMU always form their PHT pieces: they write only RF values that
+// GEOPHITIME alone reads, so PHIT = false changes no position.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_full_geor_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int total = o.nup + o.npts - 2 * o.kext;
+    const size_t base = (size_t)g * (size_t)total;
+    full_prefill(s, base, total);
+    GkVars v{};
+    const double mu0 = in.mu0[g], a = in.a[g], l = in.l[g], q2 = in.q2[g], su = in.su[g], sm = in.sm[g];
+    const double beta = in.beta ? in.beta[g] : 0.0;
+    double U0 = in.u0[g], UF = in.uf[g];
+    const double MUF = in.muf[g];
+    int TPM = in.tpm[g], TPR = in.tpr[g];
+    double IU = 0.0, dt = 0.0, dph = 0.0, lam = 0.0;
+    for (int k = 1; k <= o.nup; ++k) {
+        const double MUN = mu0 + ((double)k - o.offset) * (MUF - mu0) / (double)o.nup;
+        if (k >= 2) TPM = f_int((sgn1(mu0) * sm + 1.0) / 2.0);
+        if (mu0 != 0.0 || beta != 0.0) {
+            if (k == 1) geor_v<true, true>(v, U0, UF, mu0, MUN, a, l, q2, IU, TPM, TPR, su, sm);
+            else geor_v<false, true>(v, U0, UF, mu0, MUN, a, l, q2, IU, TPM, TPR, su, sm);
+            if (o.phit) {
+                const PhiTime p = geophitime<true>(v, U0, UF, mu0, MUN, a, l, q2, TPM, TPR, su, sm, IU);
+                dt = p.tmu + p.tu;
+                dph = p.phimu + p.phiu;
+                lam = p.lambda;
+            }
+        } else {
+            UF = U0;
+        }
+        full_store(s, base, total, k, UF, MUN, dt, dph, TPM, TPR, lam);
+    }
+    full_finish(s, g, U0, UF, MUF, TPM, TPR, v.ncase, o.nup, o.npts);
+}
+
+// The u-sampled branch (:468-597) with the MUFILL switch (:539-575).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_full_u_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int total = o.nup + o.npts - 2 * o.kext;
+    const size_t base = (size_t)g * (size_t)total;
+    full_prefill(s, base, total);
+    GkVars v{};
+    const double uout = in.uout[g], mu0 = in.mu0[g], a = in.a[g], l = in.l[g], q2 = in.q2[g];
+    const double su = in.su[g], sm = in.sm[g], offset = o.offset;
+    const int nup = o.nup, kext = o.kext;
+    double U0 = in.u0[g], UF = in.uf[g], MUF = in.muf[g];
+    int TPM = in.tpm[g], TPR = in.tpr[g];
+    const double uplus = 1.0 / (1.0 + sqrt(1.0 - a * a));
+    double IU = 0.0, dt = 0.0, dph = 0.0, lam = 0.0;
+
+    // first solve on (U0, UF): case, roots, cached u0 / mu0 pieces
+    geomu<true, true>(v, U0, UF, false, mu0, a, l, q2, TPM, TPR, su, sm, MUF, IU);
+    const int nc = v.ncase;
+    if ((nc > 2 && nc < 7) || (nc == 1 && su < 0.0) || (nc == 7 && su < 0.0) || (nc == 2 && su > 0.0) ||
+        (nc == 8 && su > 0.0))
+        TPR = 0;
+    double ub;
+    if (nc > 2 && nc < 7) {
+        ub = (su > 0.0) ? uplus : 0.0;
+    } else if (nc == 1 || nc == 7) {
+        ub = v.u2;
+        if (TPR == 1 && UF == ub) UF = uout;
+    } else {
+        ub = v.u3;
+        if (TPR == 1 && UF == ub) UF = uout;
+    }
+    double du = sgn1(ub - uout) * su * ((ub - uout) + (2.0 * TPR - 1.0) * (ub - UF));
+    du = du / (double)nup;
+    int kmax = f_int((ub - uout) / du + offset);
+    if (kmax > nup) kmax = nup;
+    if (du == 0.0) kmax = 0;
+    if (!(sgn1(uout - ub) != sgn1(uout - U0) || uout == U0)) {
+        full_finish(s, g, U0, UF, MUF, TPM, TPR, v.ncase, 0, o.npts);
+        return;
+    }
+    const bool back = (TPR == 1);
+    const bool fill = back && o.mufill;
+    const int npts2 = fill ? o.npts - 2 * kext : o.npts;
+    const int64_t nfirst = (kmax == 0) ? 0 : (kmax > 1 ? kmax : 1);
+    int64_t kfirst = (int64_t)kmax + 1 + kext;
+    if (kfirst < 1 - (int64_t)npts2) kfirst = 1 - (int64_t)npts2;
+    const int64_t nret = back ? ((int64_t)nup - kfirst + 1 > 0 ? (int64_t)nup - kfirst + 1 : 0) : 0;
+    const int64_t nfill = fill ? 1 + (int64_t)o.npts : 0;
+    const int64_t nsteps = nfirst + nfill + nret;
+
+    int TPR1 = 0, TPM1 = TPM, tpmstart = 0;
+    double UN = 0.0, MUN = 0.0, muminus = 0.0, muplus = 0.0, mustart = mu0;
+    for (int64_t it = 0; it < nsteps; ++it) {
+        // phase 0: first leg; 1: MUFILL far side; 2: MUFILL samples; 3: back
+        int phase;
+        int64_t k;
+        if (it < nfirst) { phase = 0; k = it + 1; }
+        else if (it < nfirst + nfill) { k = it - nfirst; phase = (k == 0) ? 1 : 2; }
+        else { phase = 3; k = kfirst + (it - nfirst - nfill); }
+        if (phase == 2) {
+            indep_muf(muminus, muplus, mustart, MUF, tpmstart, TPM1, sm, (int)k, o.npts, offset, MUN, TPM);
+            geor_v<false, true>(v, U0, UN, mu0, MUN, a, l, q2, IU, TPM, TPR, su, sm);
+        } else {
+            const double kk = (phase == 1) ? (double)((int64_t)kmax + kext + 1) : (double)k;
+            UN = (phase == 0) ? uout + (kk - offset) * du : 2.0 * ub - (uout + (kk - offset) * du);
+            double us = (phase == 0) ? U0 : UF;
+            int tpmv = (phase == 1) ? TPM1 : TPM;
+            int tprv = (phase == 0) ? TPR1 : TPR;
+            geomu<false, true>(v, us, UN, false, mu0, a, l, q2, tpmv, tprv, su, sm, MUN, IU);
+            if (phase == 0) U0 = us; else UF = us;
+            if (phase == 1) TPM1 = tpmv; else TPM = tpmv;
+            if (phase == 0) TPR1 = tprv; else TPR = tprv;
+        }
+        if (phase == 1) {
+            MUF = MUN;
+            findmroots(q2, l, a, mu0, muminus, muplus);
+            const int64_t jb = (int64_t)kmax - kext;   // the Fortran reads MUFI(KMAX-KEXT)
+            if (jb >= 1 && jb <= (int64_t)total) {
+                mustart = s.mufi[base + (size_t)(jb - 1)];
+                tpmstart = s.tpmi[base + (size_t)(jb - 1)];
+            }
+            continue;
+        }
+        if (o.phit) {
+            const bool fp = (phase == 0 && k == 1) || (phase == 3 && nup == 1);
+            const double us = (phase == 3) ? UF : U0;
+            const int tprv = (phase == 0) ? TPR1 : TPR;
+            const PhiTime p = fp ? geophitime<true>(v, us, UN, mu0, MUN, a, l, q2, TPM, tprv, su, sm, IU)
+                                 : geophitime<false>(v, us, UN, mu0, MUN, a, l, q2, TPM, tprv, su, sm, IU);
+            dt = p.tmu + p.tu;
+            dph = p.phimu + p.phiu;
+            lam = p.lambda;
+        }
+        const int64_t idx = (phase == 0) ? k : (phase == 2) ? k + kmax - kext : k + npts2;
+        const int tpri = (phase == 0) ? 0 : (phase == 2) ? TPR : 1;
+        full_store(s, base, total, idx, UN, MUN, dt, dph, TPM, tpri, lam);
+    }
+    full_finish(s, g, U0, UF, MUF, TPM, TPR, v.ncase, back ? nup + npts2 : nup, npts2);
+}
+
 int cus() {
     int dev = 0, c = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
@@ -476,6 +664,30 @@ int np_geokerr_phi_omega_batch(const double* d_u0, const double* d_mu0, const do
     if (!d_u0 || !d_mu0 || !d_a || !d_lz || !d_out) return NP_EINVAL;
     hipLaunchKernelGGL(gk_phi_omega_kernel, dim3(blocks_for(n, 256, 16)), dim3(256), 0, (hipStream_t)stream, d_u0,
                        d_mu0, d_a, d_lz, n, d_out);
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int np_geokerr_batch(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
+                     const np_geokerr_samples* out, void* stream) {
+    using namespace np::gk;
+    if (!opts || !rays || !out || n < 0) return NP_EINVAL;
+    const np_geokerr_opts o = *opts;
+    if (o.nup < 1 || o.npts < 0 || o.kext < 0) return NP_EINVAL;
+    const int64_t total = (int64_t)o.nup + o.npts - 2 * (int64_t)o.kext;
+    if (total < 1 || total > INT32_MAX) return NP_EINVAL;
+    if (n == 0) return NP_OK;
+    const np_geokerr_rays r = *rays;
+    const np_geokerr_samples s = *out;
+    if (!r.u0 || !r.uf || !r.uout || !r.mu0 || !r.muf || !r.a || !r.l || !r.q2 || !r.su || !r.sm || !r.tpm ||
+        !r.tpr)
+        return NP_EINVAL;
+    if (!s.ufi || !s.mufi || !s.dti || !s.dphi || !s.lambdai || !s.tpmi || !s.tpri) return NP_EINVAL;
+    if (n > INT64_MAX / total) return NP_EINVAL;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (o.usegeor)
+        hipLaunchKernelGGL(gk_full_geor_kernel, grid, dim3(256), 0, (hipStream_t)stream, o, r, n, s);
+    else
+        hipLaunchKernelGGL(gk_full_u_kernel, grid, dim3(256), 0, (hipStream_t)stream, o, r, n, s);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

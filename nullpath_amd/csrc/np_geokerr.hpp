@@ -870,13 +870,12 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
 // -------------------------------- GEOR --------------------------------------
 
 // calcimusymf / calcimuasymf (geokerr_original.f:3244-3274 / 3175-3204):
-// the mu integral from muf to the turning point muplus; i3mu is
-// calcimusym's half-period integral (the symmetric tan(phif) < 0 test is
-// phif > pi/2, as in calcimu).
-__device__ __forceinline__ double calcimuf(bool sym, double a, double mneg, double mpos, double muf, double muplus,
-                                           double i3mu) {
+// the mu integral from muf to the turning point muplus from its RF piece
+// rff (calcimuf_rf); i3mu is calcimusym's half-period integral (the
+// symmetric tan(phif) < 0 test is phif > pi/2, as in calcimu).
+__device__ __forceinline__ double calcimuf_from_rf(bool sym, double a, double mneg, double mpos, double muf,
+                                                   double muplus, double i3mu, double rff) {
     const double fac = sym ? fabs(a) * sqrt(mpos - mneg) : fabs(a) * muplus;
-    const double rff = calcimuf_rf(sym, mneg, mpos, muf, muplus);
     if (!sym) return sqrt((mpos - muf * muf) / (mpos - mneg)) * rff / fac;
     const double phif = acos(muf / muplus);
     double imuf = sin(phif) * rff;
@@ -889,17 +888,20 @@ struct GeorResult {
     int tpr, ncase;
 };
 
-// GEOR (geokerr_original.f:2672-3134) with FIRSTPT = .true. and PHT =
-// .false.: the inverse radius uf where mu reaches muf after tpm mu turning
-// points, from (u0, mu0); IMU first, then the u integral inverted with the
-// Jacobi functions of Table 2.  uf_in is the caller's UF, which only the
-// equal-root cubic branch reads; tpr stays tpr_in where the Fortran does
-// not write TPR.  Rows 3, 5, 6 and the q2 = 0 / forbidden-mu cases return
-// uf = -1, ncase = 0 as the Fortran does.  The forbidden-region retry runs
-// at most 3 times (the Fortran's is unbounded).
-__device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const double mu0, const double muf,
-                                           const double a, const double l, const double q2, const int tpm,
-                                           const int tpr_in, const double su, const double sm) {
+// GEOR (geokerr_original.f:2672-3134) with the Fortran's argument contract:
+// u0, uf and tpr by reference, IMU written into imu (the caller's IU; left
+// as it was on the asymmetric-roots exit), the variables shared along a
+// geodesic in v written where FIRSTPT (or the Fortran unconditionally)
+// writes them and read otherwise; PHT forms the RF pieces GEOPHITIME takes
+// (RFFU0 in rows 3/5/6, RFFU1 in rows 1/2/7/8, RFFMU2 always).  The
+// equal-root cubic branch compares against the uf passed in; row 2 with
+// FIRSTPT off sets IU0 = 0 whenever u0 != u3 (:2795-2799); rows 3, 5, 6
+// and the q2 = 0 / forbidden-mu cases return uf = -1, ncase = 0.  The
+// forbidden-region retry runs at most 3 times (the Fortran's is unbounded).
+template <bool FIRSTPT, bool PHT>
+__device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const double mu0, const double muf,
+                                       const double a, const double l, const double q2, double& imu,
+                                       const int tpm, int& tpr, const double su, const double sm) {
     const double one = 1.0, two = 2.0, half = 0.5, third = 0.3333333333333333, three = 3.0;
     const double uplus = one / (one + sqrt(one - a * a));
     const double l2 = l * l;
@@ -908,29 +910,31 @@ __device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const 
     const double dd = two * ((a - l) * (a - l) + q2);
     const double ee = -a * a * q2;
     const unsigned utpm = (unsigned)tpm;
-    GeorResult R{uf_in, 0.0, tpr_in, 0};
-    double imu = 0.0, iu0 = 0.0, dummy = 0.0, sn, cn, dn;
+    double dummy = 0.0, sn, cn, dn;
 
     // the mu integral in the special cases (:2749-2770)
     if (q2 == 0.0) {
         if (l2 >= a * a || mu0 == 0.0) {
-            R.uf = -1.0;
-            return R;
+            imu = 0.0;
+            uf = -1.0;
+            v.ncase = 0;
+            return;
         }
         const double s1 = sgn1(mu0);
         const double muplus = s1 * sqrt(one - l2 / a / a);
         const double x0 = mu0 / muplus, xf = muf / muplus;
         const double as0 = (x0 > 0.0) ? log((one + sqrt(one - x0 * x0)) / x0) : 0.0;
         const double asf = (xf > 0.0) ? log((one + sqrt(one - xf * xf)) / xf) : 0.0;
-        imu = s1 * sm * (one / fabs(a * muplus) * as0) + s1 * sm * neg1pow(utpm + 1u) * (one / fabs(a * muplus) * asf);
+        v.i1mu = one / fabs(a * muplus) * as0;
+        imu = s1 * sm * v.i1mu + s1 * sm * neg1pow(utpm + 1u) * (one / fabs(a * muplus) * asf);
     } else if (a == 0.0) {
         double muplus = sqrt(q2 / ql2);
         if (mu0 > muplus) muplus = mu0;
-        const double i1mu = (half * kPi - asin(mu0 / muplus)) / sqrt(ql2);
-        const double i3mu = kPi / sqrt(ql2);
+        v.i1mu = (half * kPi - asin(mu0 / muplus)) / sqrt(ql2);
+        v.i3mu = kPi / sqrt(ql2);
         const double a3 = two * f_int((two * (double)tpm + 3.0 - sm) / 4.0) - one;
         const double i2mu = (half * kPi + asin(muf / muplus)) / sqrt(ql2);
-        imu = sm * i1mu + sm * neg1pow(utpm) * i2mu + a3 * i3mu;
+        imu = sm * v.i1mu + sm * neg1pow(utpm) * i2mu + a3 * v.i3mu;
     }
 
     if (ee == 0.0 && dd != 0.0) {
@@ -942,83 +946,95 @@ __device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const 
             const double u1 = -two * sqrt(qq) * cos(theta / 3.0) - cc / dd / 3.0;
             const double u2 = -two * sqrt(qq) * cos((theta - two * kPi) / 3.0) - cc / dd / 3.0;
             const double u3 = -two * sqrt(qq) * cos((theta + two * kPi) / 3.0) - cc / dd / 3.0;
+            v.u1 = u1; v.u2 = u2; v.u3 = u3;
             for (int guard = 0; guard < 3; ++guard) {
                 if (u0 <= u2) {
                     // Table 2 row 1
-                    R.ncase = 1;
-                    iu0 = (u0 != u2) ? ellcubicreal<0>(-u1, one, u2, -one, u3, -one, 0.0, 0.0, dummy, u0, u2) : 0.0;
+                    v.ncase = 1;
+                    if (FIRSTPT && u0 != u2)
+                        v.iu0 = ellcubicreal<0>(-u1, one, u2, -one, u3, -one, 0.0, 0.0, v.rffu0, u0, u2);
+                    else if (u0 == u2)
+                        v.iu0 = 0.0;
                     const double m1 = (u3 - u2) / (u3 - u1);
-                    const double jarg = sqrt((u3 - u1) * dd) * half * (imu - su * iu0 / sqrt(dd));
+                    const double jarg = sqrt((u3 - u1) * dd) * half * (imu - su * v.iu0 / sqrt(dd));
                     sncndn(jarg, m1, sn, cn, dn);
-                    R.uf = u1 + (u2 - u1) * (cn * cn / (dn * dn));
-                    R.tpr = f_int((sgn1(su * (imu - su * iu0 / sqrt(dd))) + one) / two);
+                    uf = u1 + (u2 - u1) * (cn * cn / (dn * dn));
+                    tpr = f_int((sgn1(su * (imu - su * v.iu0 / sqrt(dd))) + one) / two);
+                    if (PHT) (void)ellcubicreal<0>(-u1, one, u2, -one, u3, -one, 0.0, 0.0, v.rffu1, uf, u2);
                     break;
                 } else if (u0 >= u3) {
                     // Table 2 row 2
-                    R.ncase = 2;
-                    if (u0 != u3) iu0 = ellcubicreal<0>(-u1, one, -u2, one, -u3, one, 0.0, 0.0, dummy, u3, u0);
+                    v.ncase = 2;
+                    if (FIRSTPT && u0 != u3)
+                        v.iu0 = ellcubicreal<0>(-u1, one, -u2, one, -u3, one, 0.0, 0.0, v.rffu0, u3, u0);
+                    else if (u0 != u3)
+                        v.iu0 = 0.0;
                     const double m1 = (u3 - u2) / (u3 - u1);
-                    const double jarg = sqrt((u3 - u1) * dd) * half * (imu + su * iu0 / sqrt(dd));
+                    const double jarg = sqrt((u3 - u1) * dd) * half * (imu + su * v.iu0 / sqrt(dd));
                     sncndn(jarg, m1, sn, cn, dn);
-                    R.uf = u1 + (u3 - u1) * (dn * dn / (cn * cn));
-                    R.tpr = f_int((-sgn1(su * (imu + iu0 / sqrt(dd))) + one) / two);
+                    uf = u1 + (u3 - u1) * (dn * dn / (cn * cn));
+                    tpr = f_int((-sgn1(su * (imu + v.iu0 / sqrt(dd))) + one) / two);
+                    if (PHT) (void)ellcubicreal<0>(-u1, one, -u2, one, -u3, one, 0.0, 0.0, v.rffu1, u3, uf);
                     break;
                 }
                 u0 = (su == 1.0) ? u3 : u2;   // forbidden region: the reference's input edit
             }
         } else if (fabs(dis) < 1e-16) {
             // equal roots u2 = u3 (:2832-2860)
-            R.tpr = 0;
+            tpr = 0;
             const double u1 = -two * sqrt(qq) - cc / dd / 3.0;
             const double u2 = -two * sqrt(qq) * cos((two * kPi) / 3.0) - cc / dd / 3.0;
+            v.u1 = u1; v.u2 = u2; v.u3 = u2;
             if (u0 <= u2) {
-                R.ncase = 1;
-                if (uf_in > u2) {
-                    R.uf = u2;
+                v.ncase = 1;
+                if (uf > u2) {
+                    uf = u2;
                 } else {
                     const double sarg = sqrt((u0 - u1) / (u2 - u1));
                     const double th = tanh(sqrt((u2 - u1) * dd) * half * su * imu + half * log((one + sarg) / (one - sarg)));
-                    R.uf = u1 + (u2 - u1) * th * th;
+                    uf = u1 + (u2 - u1) * th * th;
                 }
             } else {
-                R.ncase = 2;
-                if (uf_in < u2) {
-                    R.uf = u2;
+                v.ncase = 2;
+                if (uf < u2) {
+                    uf = u2;
                 } else {
                     const double sarg = sqrt((u2 - u1) / (u0 - u1));
                     const double th = tanh(-sqrt((u2 - u1) * dd) * half * su * imu + half * log((one + sarg) / (one - sarg)));
-                    R.uf = u1 + (u2 - u1) / (th * th);
+                    uf = u1 + (u2 - u1) / (th * th);
                 }
             }
         } else {
             // one real root: Table 2 row 3 (:2861-2895)
-            R.ncase = 3;
-            R.tpr = 0;
+            v.ncase = 3;
+            tpr = 0;
             const double aa = -sgn1(rr) * pow(fabs(rr) + sqrt(dis), third);
             const double bb = (aa != 0.0) ? qq / aa : 0.0;
             const double u1 = (aa + bb) - cc / dd / 3.0;
+            v.u1 = u1;
             const double f = -one / dd / u1;
             const double g = f / u1;
             const double iut = (su > 0.0)
                 ? ellcubiccomplex<0>(-u1, one, 0.0, 0.0, f, g, one, dummy, u0, uplus) / sqrt(dd)
                 : ellcubiccomplex<0>(-u1, one, 0.0, 0.0, f, g, one, dummy, 0.0, u0) / sqrt(dd);
             if (imu > iut) {
-                R.uf = -1.0;
-                R.ncase = 0;
-            } else {
-                iu0 = su * ellcubiccomplex<0>(-u1, one, 0.0, 0.0, f, g, one, dummy, u1, u0) / sqrt(dd);
-                const double c3 = (a != 0.0 || l != 0.0) ? (a + l) / (a - l) : -one;
-                const double c2 = sqrt(u1 * (three * u1 + c3));
-                const double c1 = sqrt(c2 * dd);
-                const double m1 = half + (6.0 * u1 + c3) / (8.0 * c2);
-                sncndn(c1 * (imu + iu0), m1, sn, cn, dn);
-                R.uf = (c2 + u1 - (c2 - u1) * cn) / (one + cn);
+                uf = -1.0;
+                v.ncase = 0;
+                return;
             }
+            if (FIRSTPT) v.iu0 = su * ellcubiccomplex<0>(-u1, one, 0.0, 0.0, f, g, one, dummy, u1, u0) / sqrt(dd);
+            const double c3 = (a != 0.0 || l != 0.0) ? (a + l) / (a - l) : -one;
+            const double c2 = sqrt(u1 * (three * u1 + c3));
+            const double c1 = sqrt(c2 * dd);
+            const double m1 = half + (6.0 * u1 + c3) / (8.0 * c2);
+            sncndn(c1 * (imu + v.iu0), m1, sn, cn, dn);
+            uf = (c2 + u1 - (c2 - u1) * cn) / (one + cn);
+            if (PHT) (void)ellcubiccomplex<0>(-u1, one, 0.0, 0.0, f, g, one, v.rffu0, u0, uf);
         }
     } else if (ee == 0.0 && dd == 0.0) {
-        R.ncase = 4;
-        R.uf = -1.0;
-        R.tpr = 0;
+        v.ncase = 4;
+        uf = -1.0;
+        tpr = 0;
     } else {
         // roots of M(mu) and the mu integral (:2901-2945)
         const double amq = a * a - ql2;
@@ -1034,45 +1050,48 @@ __device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const 
             if (mu0 > muplus) { muplus = mu0; mpos = muplus * muplus; }
         } else {
             if (fabs(muf) < sqrt(mneg)) {
-                R.uf = -1.0;
-                R.tpr = 0;
-                return R;
+                uf = -1.0;
+                v.ncase = 0;
+                tpr = 0;
+                return;
             }
             if (sgn1(mu0) == -1.0) muplus = -muplus;
             if (fabs(muplus) < fabs(mu0)) { muplus = mu0; mpos = muplus * muplus; }
             mneg = fmin(mu0 * mu0, mneg);
         }
-        double i1mu, i3mu, rf0, rfc;
-        calcimu(sym, a, mneg, mpos, mu0, muplus, i1mu, i3mu, rf0, rfc);
-        const double i2mu = calcimuf(sym, a, mneg, mpos, muf, muplus, i3mu);
-        imu = sm * i1mu + a2 * i2mu + a3 * i3mu;
+        if constexpr (FIRSTPT) calcimu(sym, a, mneg, mpos, mu0, muplus, v.i1mu, v.i3mu, v.rffmu1, v.rffmu3);
+        v.rffmu2 = calcimuf_rf(sym, mneg, mpos, muf, muplus);
+        const double i2mu = calcimuf_from_rf(sym, a, mneg, mpos, muf, muplus, v.i3mu, v.rffmu2);
+        imu = sm * v.i1mu + a2 * i2mu + a3 * v.i3mu;
 
         // roots of U(u) and the case (:2947-2993)
-        cplx c[5], root[4];
-        c[0] = mk(one); c[1] = mk(0.0); c[2] = mk(cc); c[3] = mk(dd); c[4] = mk(ee);
-        zroots(c, 4, root);
-        int nreal = 0;
-        for (int i = 0; i < 4; ++i) nreal += (root[i].im == 0.0);
-        double u1 = 0.0, u2 = 0.0, u3 = 0.0, u4 = 0.0;
-        if (nreal == 2) {
-            R.ncase = 5;
-            R.tpr = 0;
-            u1 = root[0].re;
-            u4 = (root[1].im == 0.0) ? root[1].re : root[3].re;
-        } else if (nreal == 0) {
-            R.ncase = 6;
-            R.tpr = 0;
-        } else {
-            u1 = root[0].re; u2 = root[1].re; u3 = root[2].re; u4 = root[3].re;
-            for (int guard = 0; guard < 3; ++guard) {
-                if (u2 > uplus && u3 > uplus) { R.ncase = 5; break; }
-                else if (u0 <= u2) { R.ncase = 7; break; }
-                else if (u0 >= u3) { R.ncase = 8; break; }
-                u0 = (su == 1.0) ? u3 : u2;
+        if constexpr (FIRSTPT) {
+            cplx c[5], root[4];
+            c[0] = mk(one); c[1] = mk(0.0); c[2] = mk(cc); c[3] = mk(dd); c[4] = mk(ee);
+            zroots(c, 4, root);
+            int nreal = 0;
+            for (int i = 0; i < 4; ++i) nreal += (root[i].im == 0.0);
+            if (nreal == 2) {
+                v.ncase = 5;
+                tpr = 0;
+                v.u1 = root[0].re;
+                v.u4 = (root[1].im == 0.0) ? root[1].re : root[3].re;
+            } else if (nreal == 0) {
+                v.ncase = 6;
+                tpr = 0;
+            } else {
+                v.u1 = root[0].re; v.u2 = root[1].re; v.u3 = root[2].re; v.u4 = root[3].re;
+                for (int guard = 0; guard < 3; ++guard) {
+                    if (v.u2 > uplus && v.u3 > uplus) { v.ncase = 5; break; }
+                    else if (u0 <= v.u2) { v.ncase = 7; break; }
+                    else if (u0 >= v.u3) { v.ncase = 8; break; }
+                    u0 = (su == 1.0) ? v.u3 : v.u2;
+                }
             }
         }
+        const double u1 = v.u1, u2 = v.u2, u3 = v.u3, u4 = v.u4;
         const double sqe = sqrt(fabs(ee));
-        if (R.ncase == 5) {
+        if (v.ncase == 5) {
             // Table 2 row 5 (:2995-3027)
             const double qs = sgn1(q2);
             const double f = -qs * one / fabs(ee) / u1 / u4;
@@ -1081,22 +1100,24 @@ __device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const 
                 ? ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, dummy, u0, uplus) / sqe
                 : ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, dummy, 0.0, u0) / sqe;
             if (imu > iut) {
-                R.uf = -1.0;
-                R.ncase = 0;
-                R.tpr = 0;
-            } else {
-                const double ub = (qs == one) ? u1 : u4;
-                const double m = -g * half;
-                const double n2 = f - g * g / 4.0;
-                const double c4 = sqrt((m - u4) * (m - u4) + n2);
-                const double c5 = sqrt((m - u1) * (m - u1) + n2);
-                const double c1 = sqrt(fabs(ee * c4 * c5));
-                iu0 = su * ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, dummy, ub, u0) / sqe;
-                const double m1 = qs * ((c4 + qs * c5) * (c4 + qs * c5) - (u4 - u1) * (u4 - u1)) / (4.0 * c4 * c5);
-                sncndn(c1 * (imu + iu0), m1, sn, cn, dn);
-                R.uf = (u4 * c5 + qs * u1 * c4 - (qs * u4 * c5 - u1 * c4) * cn) / ((c4 - qs * c5) * cn + (qs * c4 + c5));
+                uf = -1.0;
+                v.ncase = 0;
+                tpr = 0;
+                return;
             }
-        } else if (R.ncase == 6) {
+            const double ub = (qs == one) ? u1 : u4;
+            const double m = -g * half;
+            const double n2 = f - g * g / 4.0;
+            const double c4 = sqrt((m - u4) * (m - u4) + n2);
+            const double c5 = sqrt((m - u1) * (m - u1) + n2);
+            const double c1 = sqrt(fabs(ee * c4 * c5));
+            if (FIRSTPT)
+                v.iu0 = su * ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, dummy, ub, u0) / sqe;
+            const double m1 = qs * ((c4 + qs * c5) * (c4 + qs * c5) - (u4 - u1) * (u4 - u1)) / (4.0 * c4 * c5);
+            sncndn(c1 * (imu + v.iu0), m1, sn, cn, dn);
+            uf = (u4 * c5 + qs * u1 * c4 - (qs * u4 * c5 - u1 * c4) * cn) / ((c4 - qs * c5) * cn + (qs * c4 + c5));
+            if (PHT) (void)ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, v.rffu0, u0, uf);
+        } else if (v.ncase == 6) {
             // Table 2 row 6 (:3028-3095): h1 and mn2 from two sextics
             const double se = sqrt(ee);
             cplx co[7], hr[6];
@@ -1108,6 +1129,7 @@ __device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const 
             for (int i = 0; i < 6 && h1 == 0.0; ++i)
                 if (hr[i].im == 0.0) h1 = hr[i].re;
             if (h1 == 0.0) h1 = __builtin_nan("");
+            v.h1 = h1;
             const double h2 = one / h1;
             const double g1 = dd / ee / (h2 - h1);
             const double g2 = -g1;
@@ -1117,57 +1139,134 @@ __device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const 
                 ? elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, dummy, u0, uplus) / sqe
                 : elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, dummy, 0.0, u0) / sqe;
             if (imu > iut) {
-                R.uf = -1.0;
-                R.ncase = 0;
-                R.tpr = 0;
-            } else {
-                co[0] = mk(pow(ee, -three));
-                co[1] = mk(-cc / pow(ee, three));
-                co[2] = mk(-pow(ee, -two));
-                co[3] = mk(-pow(ee, -two) * (pow(dd, two) / ee - two * cc));
-                co[4] = mk(-one / ee);
-                co[5] = mk(-cc / ee);
-                co[6] = mk(one);
-                zroots(co, 6, hr);
-                double mn2 = 0.0;
-                for (int i = 0; i < 6 && mn2 == 0.0; ++i)
-                    if (hr[i].im == 0.0) mn2 = hr[i].re;
-                if (mn2 == 0.0) mn2 = __builtin_nan("");
-                double pp = dd / (two * (ee * ee) * (mn2 * mn2 - one / ee));
-                double m = -half * dd / ee - pp;
-                if (m < pp) { const double t = pp; pp = m; m = t; }
-                const double pr2 = one / (mn2 * ee);
-                const double n = sqrt(mn2 - m * m);
-                const double r = sqrt(pr2 - pp * pp);
-                const double c4 = sqrt((m - pp) * (m - pp) + (n + r) * (n + r));
-                const double c5 = sqrt((m - pp) * (m - pp) + (n - r) * (n - r));
-                const double c1 = (c4 + c5) * half * sqe;
-                const double c2 = sqrt((4.0 * n * n - (c4 - c5) * (c4 - c5)) / ((c4 + c5) * (c4 + c5) - 4.0 * n * n));
-                const double c3 = m + c2 * n;
-                iu0 = sgn1(u0 - c3) * elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, dummy, c3, u0) / sqe;
-                const double m1 = ((c4 - c5) / (c4 + c5)) * ((c4 - c5) / (c4 + c5));
-                sncndn(c1 * (su * imu + iu0), m1, sn, cn, dn);
-                const double sc = sn / cn;
-                R.uf = c3 + (n * (one + c2 * c2) * sc) / (one - c2 * sc);
+                uf = -1.0;
+                v.ncase = 0;
+                tpr = 0;
+                return;
             }
-        } else if (R.ncase == 7) {
+            co[0] = mk(pow(ee, -three));
+            co[1] = mk(-cc / pow(ee, three));
+            co[2] = mk(-pow(ee, -two));
+            co[3] = mk(-pow(ee, -two) * (pow(dd, two) / ee - two * cc));
+            co[4] = mk(-one / ee);
+            co[5] = mk(-cc / ee);
+            co[6] = mk(one);
+            zroots(co, 6, hr);
+            double mn2 = 0.0;
+            for (int i = 0; i < 6 && mn2 == 0.0; ++i)
+                if (hr[i].im == 0.0) mn2 = hr[i].re;
+            if (mn2 == 0.0) mn2 = __builtin_nan("");
+            double pp = dd / (two * (ee * ee) * (mn2 * mn2 - one / ee));
+            double m = -half * dd / ee - pp;
+            if (m < pp) { const double t = pp; pp = m; m = t; }
+            const double pr2 = one / (mn2 * ee);
+            const double n = sqrt(mn2 - m * m);
+            const double r = sqrt(pr2 - pp * pp);
+            const double c4 = sqrt((m - pp) * (m - pp) + (n + r) * (n + r));
+            const double c5 = sqrt((m - pp) * (m - pp) + (n - r) * (n - r));
+            const double c1 = (c4 + c5) * half * sqe;
+            const double c2 = sqrt((4.0 * n * n - (c4 - c5) * (c4 - c5)) / ((c4 + c5) * (c4 + c5) - 4.0 * n * n));
+            const double c3 = m + c2 * n;
+            if (FIRSTPT)
+                v.iu0 = sgn1(u0 - c3) * elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, dummy, c3, u0) / sqe;
+            const double m1 = ((c4 - c5) / (c4 + c5)) * ((c4 - c5) / (c4 + c5));
+            sncndn(c1 * (su * imu + v.iu0), m1, sn, cn, dn);
+            const double sc = sn / cn;
+            uf = c3 + (n * (one + c2 * c2) * sc) / (one - c2 * sc);
+            if (PHT) (void)elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, v.rffu0, u0, uf);
+        } else if (v.ncase == 7) {
             // Table 2 row 7 (:3098-3108)
-            if (u0 != u2) iu0 = ellquarticreal<0>(-u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, dummy, u0, u2);
-            const double jarg = sqrt(fabs(ee) * (u3 - u1) * (u4 - u2)) * half * (imu - su * iu0 / sqrt(-ee));
+            if (FIRSTPT && u0 != u2)
+                v.iu0 = ellquarticreal<0>(-u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, v.rffu0, u0, u2);
+            const double jarg = sqrt(fabs(ee) * (u3 - u1) * (u4 - u2)) * half * (imu - su * v.iu0 / sqrt(-ee));
             sncndn(jarg, (u4 - u1) * (u3 - u2) / ((u4 - u2) * (u3 - u1)), sn, cn, dn);
-            R.uf = ((u2 - u1) * u3 * sn * sn - u2 * (u3 - u1)) / ((u2 - u1) * sn * sn - (u3 - u1));
-            R.tpr = f_int((sgn1(su * (imu - su * iu0 / sqe)) + one) / two);
-        } else if (R.ncase == 8) {
+            uf = ((u2 - u1) * u3 * sn * sn - u2 * (u3 - u1)) / ((u2 - u1) * sn * sn - (u3 - u1));
+            tpr = f_int((sgn1(su * (imu - su * v.iu0 / sqe)) + one) / two);
+            if (PHT) (void)ellquarticreal<0>(-u1, one, u2, -one, u3, -one, u4, -one, 0.0, 0.0, v.rffu1, uf, u2);
+        } else if (v.ncase == 8) {
             // Table 2 row 8 (:3109-3118)
-            if (u0 != u3) iu0 = ellquarticreal<0>(-u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, dummy, u3, u0);
-            const double jarg = sqrt(fabs(ee) * (u3 - u1) * (u4 - u2)) * half * (imu + su * iu0 / sqrt(-ee));
+            if (FIRSTPT && u0 != u3)
+                v.iu0 = ellquarticreal<0>(-u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, v.rffu0, u3, u0);
+            const double jarg = sqrt(fabs(ee) * (u3 - u1) * (u4 - u2)) * half * (imu + su * v.iu0 / sqrt(-ee));
             sncndn(jarg, (u4 - u1) * (u3 - u2) / ((u4 - u2) * (u3 - u1)), sn, cn, dn);
-            R.uf = ((u4 - u3) * u2 * sn * sn - u3 * (u4 - u2)) / ((u4 - u3) * sn * sn - (u4 - u2));
-            R.tpr = f_int((-sgn1(su * (imu + su * iu0 / sqrt(-ee))) + one) / two);
+            uf = ((u4 - u3) * u2 * sn * sn - u3 * (u4 - u2)) / ((u4 - u3) * sn * sn - (u4 - u2));
+            tpr = f_int((-sgn1(su * (imu + su * v.iu0 / sqrt(-ee))) + one) / two);
+            if (PHT) (void)ellquarticreal<0>(-u1, one, -u2, one, -u3, one, u4, -one, 0.0, 0.0, v.rffu1, u3, uf);
         }
     }
-    R.imu = imu;
+}
+
+// GEOR on a fresh geodesic (FIRSTPT on, PHT off): uf_in is the caller's UF,
+// which only the equal-root cubic branch reads; tpr stays tpr_in where the
+// Fortran does not write TPR.
+__device__ __forceinline__ GeorResult geor(double u0, const double uf_in, const double mu0, const double muf,
+                                           const double a, const double l, const double q2, const int tpm,
+                                           const int tpr_in, const double su, const double sm) {
+    GkVars v{};
+    GeorResult R{uf_in, 0.0, tpr_in, 0};
+    geor_v<true, false>(v, u0, R.uf, mu0, muf, a, l, q2, R.imu, tpm, R.tpr, su, sm);
+    R.ncase = v.ncase;
     return R;
+}
+
+// FINDMROOTS (geokerr_original.f:607-642): the mu turning points (with
+// a = 0 the Fortran divides by zero; the IEEE results are kept).
+__device__ __forceinline__ void findmroots(double q2, double l, double a, double mu0, double& muminus,
+                                           double& muplus) {
+    const double ql2 = q2 + l * l;
+    const double amq = a * a - ql2;
+    const double yy = -0.5 * (amq + sgn1(amq) * sqrt(amq * amq + 4.0 * q2 * a * a));
+    double mneg, mpos;
+    if (amq < 0.0) { mneg = -yy / a / a; mpos = q2 / yy; }
+    else { mneg = q2 / yy; mpos = -yy / a / a; }
+    if (mpos > 1.0) mpos = 1.0;
+    if (mneg > 0.0) {
+        muplus = sgn1(mu0) * sqrt(mpos);
+        muminus = sgn1(mu0) * sqrt(mneg);
+        if (fabs(muminus) > fabs(mu0)) muminus = mu0;
+        if (fabs(mu0) > fabs(muplus)) muplus = mu0;
+    } else {
+        muplus = sqrt(mpos);
+        if (muplus < mu0) muplus = mu0;
+        muminus = -muplus;
+    }
+}
+
+__device__ __forceinline__ int m1pow_i(int n) { return (n & 1) ? -1 : 1; }
+
+// INDEP_MUF (geokerr_original.f:645-706): the k-th of npts mu values spaced
+// evenly in path length sum |dmu| from mu0 (after tpm0 turning points) to
+// muf (after tpmf), and the turning points before it.  The Fortran's
+// default-real constants in its integer expressions are kept as float.
+__device__ __forceinline__ void indep_muf(double muminus, double muplus, double mu0, double muf, int tpm0,
+                                          int tpmf, double sm, int k, int npts, double offset, double& mun,
+                                          int& tpmk_out) {
+    int tpmk;
+    if (muf <= muplus && muf >= muminus) {
+        const int dtpm = tpmf - tpm0;
+        const int a1 = f_int(sm * (double)m1pow_i(tpm0));
+        const int a2 = f_int(sm * (double)m1pow_i(tpmf));
+        const int a3 = 2 * (int)floorf((2.0f * (float)dtpm + 3.0f - (float)a1) / 4.0f) - 1;
+        const double dmu = a1 * (muplus - mu0) + a2 * (muf - muminus) + a3 * (muplus - muminus);
+        const double w1 = (double)(((float)a1 + 1.0f) / 2.0f), w2 = (double)((1.0f - (float)a1) / 2.0f);
+        const double mu1 = w1 * muplus + w2 * muminus;
+        const double mu2 = w2 * muplus + w1 * muminus;
+        const double dmu1 = fabs(mu1 - mu0);
+        const double dmu2 = fabs(mu2 - mu1);
+        const double dmuk = ((double)k - offset) * dmu / (double)(float)npts;
+        int dmukdmu1 = f_int(dmuk / dmu1);
+        if (dmukdmu1 < 0) dmukdmu1 = 1;
+        const int tp2 = f_int((dmuk - dmu1) / dmu2);
+        tpmk = (dmukdmu1 < 1 ? dmukdmu1 : 1) + (tp2 > 0 ? tp2 : 0);
+        const int a1k = f_int(sm * (double)m1pow_i(tpm0));
+        const int a2k = f_int(sm * (double)m1pow_i(tpmk + tpm0));
+        const int a3k = (int)(2.0f * (float)(int)((2.0f * (float)tpmk + 3.0f - (float)a1k) / 4.0f) - 1.0f);
+        mun = muminus + (double)(1.0f / (float)a2k) * (dmuk - a1k * (muplus - mu0) - a3k * (muplus - muminus));
+    } else {
+        mun = 0.0;
+        tpmk = 0;
+    }
+    tpmk_out = tpmk + tpm0;
 }
 
 // ------------------------------ GEOPHITIME ----------------------------------

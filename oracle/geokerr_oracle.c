@@ -5,7 +5,10 @@
  *   calcphitmuasym / calcphitmusym 2361-2434 / 2435-2507
  *   ellphitmu                     2508-2592
  *   phifnkerr / tfnkerr           2593-2628 / 2629-2671
- *   GEOKERR driver                357-604 (USEGEOR = .false. branch)
+ *   GEOKERR driver                357-604 (USEGEOR = .false. branch as
+ *                                 geokerr_eval_lambda runs it; the whole
+ *                                 driver with FINDMROOTS 607-642 and
+ *                                 INDEP_MUF 645-706 in orc_gk_geokerr_full)
  *   geokerr_eval_lambda, geokerr_estimate_periods, geokerr_phi_omega
  *                                 geokerr_eval_lambda.f90:62-306
  *

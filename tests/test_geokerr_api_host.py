@@ -36,6 +36,25 @@ def test_geor_batch_argument_checks():
     assert L.np_geor_batch(*([None] * 10), 0, *([None] * 5)) == 0
 
 
+def test_geokerr_batch_argument_checks():
+    # the whole-driver entry point checks its switches and pointers on the
+    # host; n = 0 with valid switches is a no-op
+    from nullpath_amd._lib import GeokerrOpts, GeokerrRays, GeokerrSamples
+    L = npa.lib()
+    r, s = GeokerrRays(), GeokerrSamples()
+    ok = GeokerrOpts(nup=8, npts=4, kext=1, phit=1, usegeor=0, mufill=1, offset=0.5)
+    assert L.np_geokerr_batch(None, C.byref(r), 0, C.byref(s), None) == -1
+    assert L.np_geokerr_batch(C.byref(ok), None, 0, C.byref(s), None) == -1
+    assert L.np_geokerr_batch(C.byref(ok), C.byref(r), -1, C.byref(s), None) == -1
+    assert L.np_geokerr_batch(C.byref(ok), C.byref(r), 0, C.byref(s), None) == 0
+    assert L.np_geokerr_batch(C.byref(ok), C.byref(r), 5, C.byref(s), None) == -1   # NULL arrays
+    for bad in (dict(nup=0), dict(npts=-1), dict(kext=-1), dict(nup=1, npts=0, kext=1)):
+        o = GeokerrOpts(nup=8, npts=4, kext=1, phit=1, usegeor=0, mufill=1, offset=0.5)
+        for k, v in bad.items():
+            setattr(o, k, v)
+        assert L.np_geokerr_batch(C.byref(o), C.byref(r), 0, C.byref(s), None) == -1, bad
+
+
 def test_phi_omega_host_matches_oracle(oracle):
     rng = np.random.default_rng(17)
     for _ in range(500):
