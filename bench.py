@@ -84,6 +84,9 @@ def parse():
     ap.add_argument("--geokerr-only", type=int, default=0,
                     help="only time geokerr_eval_lambda batches of this many geodesics x 4096 "
                          "samples (for profiling that path) and exit")
+    ap.add_argument("--geokerr-driver-only", type=int, default=0,
+                    help="only time the whole GEOKERR driver (MUFILL and USEGEOR runs) on this many "
+                         "geodesics and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (a reference "
                          "kernel of known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) "
@@ -472,6 +475,79 @@ def bench_geokerr(n: int, s: int, cpu_geodesics: int) -> dict:
     return res
 
 
+def bench_geokerr_driver(n: int, cpu_geodesics: int) -> dict:
+    """The whole GEOKERR driver (np_geokerr_batch) in its two other
+    configurations: u-sampled with MUFILL (NUP 512, NPTS 64, KEXT 2) and
+    mu-sampled (USEGEOR, NUP 512), both with PHIT, on n camera geodesics
+    from u0 = 1e-4 (alpha, beta ~ U[-12, 12] M, a ~ U[0.05, 0.998],
+    mu0 ~ U[0.05, 0.95], seed 2027); HIP events on the launch stream; the
+    oracle on cpu_geodesics of the same geodesics on the host cores."""
+    import numpy as np
+    import torch
+
+    import nullpath_amd as npa
+    rng = np.random.default_rng(2027)
+    a = rng.uniform(0.05, 0.998, n)
+    mu0 = rng.uniform(0.05, 0.95, n)
+    al, be = rng.uniform(-12, 12, (2, n))
+    lz = -al * np.sqrt(1 - mu0 ** 2)
+    q2 = be ** 2 - (a * a - al * al) * mu0 ** 2
+    sm = np.where(be >= 0, 1.0, -1.0)
+    uplus = 1.0 / (1.0 + np.sqrt(1.0 - a * a))
+    # mu_- from M(mu) = Q2 + (a^2 - Q2 - L^2) mu^2 - a^2 mu^4 (FINDMROOTS)
+    amq = a * a - q2 - lz * lz
+    yy = -0.5 * (amq + np.sign(amq) * np.sqrt(amq * amq + 4 * q2 * a * a))
+    mneg = np.where(amq < 0, -yy / (a * a), q2 / yy)
+    mpos = np.minimum(np.where(amq < 0, q2 / yy, -yy / (a * a)), 1.0)
+    muminus = np.where(mneg > 0, np.sqrt(np.abs(mneg)), -np.sqrt(mpos))
+    cases = {
+        "mufill": (dict(nup=512, npts=64, kext=2, phit=1, usegeor=0, mufill=1, offset=0.5),
+                   rng.uniform(0, 1, n) * uplus, np.zeros(n), 1),
+        "usegeor": (dict(nup=512, npts=0, kext=0, phit=1, usegeor=1, mufill=0, offset=0.5),
+                    np.zeros(n), mu0 + 0.6 * (muminus - mu0), 0),
+    }
+    t = lambda x, dt=torch.float64: torch.tensor(np.ascontiguousarray(x), dtype=dt, device="cuda")
+    out = {}
+    for name, (opts, uf, muf, tpr) in cases.items():
+        tpm = ((np.sign(mu0) * sm + 1) / 2).astype(np.int32) if opts["usegeor"] else np.zeros(n, np.int32)
+        rays = {"u0": t(np.full(n, 1e-4)), "uf": t(uf), "uout": t(np.full(n, 1e-4)), "mu0": t(mu0), "muf": t(muf),
+                "a": t(a), "l": t(lz), "q2": t(q2), "beta": t(be), "su": t(np.ones(n)), "sm": t(sm),
+                "tpm": t(tpm, torch.int32), "tpr": t(np.full(n, tpr), torch.int32)}
+        npa.geokerr_batch(opts, {k: v[:64] for k, v in rays.items()})    # warm-up
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = npa.geokerr_batch(opts, rays, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        written = int((res["tpri"] >= 0).sum().item())
+        entry = {"opts": opts, "batch_ms": ms, "samples_written": written,
+                 "samples_per_s": written / (ms / 1e3),
+                 "geodesics_with_mufill": int((res["nup_out"] > opts["nup"]).sum().item())}
+        if cpu_geodesics > 0:
+            import os
+            import time
+
+            from oracle import np_oracle as O
+            m = min(cpu_geodesics, n)
+            g = np.zeros(m, O.GK_ARGS_DTYPE)
+            for k in ("u0", "uf", "uout", "mu0", "muf", "a", "l", "q2", "beta", "su", "sm", "tpm", "tpr"):
+                g[k] = rays[k][:m].cpu().numpy()
+            threads = min(16, len(os.sched_getaffinity(0)))
+            t0 = time.perf_counter()
+            ref = O.geokerr_full_batch(opts, g, nthreads=threads)
+            dt = time.perf_counter() - t0
+            entry["cpu_baseline"] = {"value": int((ref["tpri"] >= 0).sum()) / dt, "unit": "samples/s",
+                                     "cores": threads, "kind": "port",
+                                     "sample": f"first {m} of the {n} geodesics"}
+        out[name] = entry
+    out["workload"] = (f"GEOKERR driver (np_geokerr_batch), {n} camera geodesics from u0 = 1e-4: MUFILL "
+                       "(NUP 512, NPTS 64, KEXT 2) and USEGEOR (NUP 512), PHIT on, FP64")
+    return out
+
+
 def cpu_baseline(args, fp64: dict) -> dict:
     """The FP64 oracle (a plain-C restatement of the reference integrator)
     on a bounded random sample of the same frame, on the host cores."""
@@ -517,6 +593,9 @@ def main() -> int:
         return 2
     if args.copy_calibration_mib > 0:
         return copy_calibration(args.copy_calibration_mib)
+    if args.geokerr_driver_only > 0:
+        print(json.dumps(bench_geokerr_driver(args.geokerr_driver_only, 0)))
+        return 0
     if args.geokerr_only > 0:
         print(json.dumps(bench_geokerr(args.geokerr_only, 4096, 0)))
         return 0
@@ -595,6 +674,7 @@ def main() -> int:
         line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000)
         line["config"]["geomu_batch"] = bench_geomu(1 << 20)
         line["config"]["geokerr_eval_lambda"] = bench_geokerr(256, 4096, 64 if args.cpu_pixels > 0 else 0)
+        line["config"]["geokerr_driver"] = bench_geokerr_driver(131072, 32 if args.cpu_pixels > 0 else 0)
 
     if args.cpu_pixels > 0 and "fp64" in res:
         line["cpu_baseline"] = cpu_baseline(args, res["fp64"])
