@@ -402,11 +402,29 @@ def bench_geomu(n: int) -> dict:
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     nc = np.bincount(ncase.cpu().numpy(), minlength=9)
+    # the compared mode: the quartic's roots from a float deflation polished
+    # in double, FP64 root finding for near-degenerate arguments
+    npa.geomu_camera_batch_mixed(*[x[:256] for x in args], 1e-4)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    mm, tm, cm, im, fb = npa.geomu_camera_batch_mixed(*args, 1e-4, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_mixed = e0.elapsed_time(e1)
+    same = (cm == ncase) & (tm == tpm)
+    dmu = (mm - muf).abs()
     return {
         "workload": f"GEOMU {n} rays, alpha,beta~U[-12,12]M, a~U[0,0.998], mu0~U[0.05,0.99], u~U(0,u+), FP64",
         "rays_per_s": n / (ms / 1e3), "kernel_ms": ms,
         "ncase_counts": [int(v) for v in nc],
         "nan_muf": int(torch.isnan(muf).sum().item()),
+        "mixed_roots": {
+            "rays_per_s": n / (ms_mixed / 1e3), "kernel_ms": ms_mixed, "speedup": ms / ms_mixed,
+            "fp64_root_fallback_frac": float(fb.double().mean().item()),
+            "case_or_tpm_differ_frac": float((~same).double().mean().item()),
+            "muf_abs_diff_max": float(torch.nan_to_num(dmu, nan=0.0).max().item()),
+            "muf_abs_diff_gt_1e-10_frac": float((dmu > 1e-10).double().mean().item()),
+        },
     }
 
 

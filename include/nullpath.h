@@ -172,6 +172,22 @@ int np_geomu_camera_batch(const double* d_alpha, const double* d_beta, const dou
                           void* stream);
 
 /*
+ * The same with the quartic's roots in mixed precision (the GEOKERR batch's
+ * compared mode: FP32 bulk, FP64 for near-degenerate arguments): ZROOTS'
+ * deflation runs in float, each root is polished on the full polynomial in
+ * double (LAGUER, geokerr_original.f:3527-3606), and the FP64 ZROOTS runs
+ * instead where the float phase cannot tell a real root from a complex pair
+ * (imaginary part between 1e-5 and 1e-2 of the modulus) or finds two real
+ * roots within 1e-3 of each other.  d_fp64_roots (n flags, or NULL) marks
+ * those rays.  Everything after the roots is FP64 as in
+ * np_geomu_camera_batch.
+ */
+int np_geomu_camera_batch_mixed(const double* d_alpha, const double* d_beta, const double* d_spin,
+                                const double* d_mu0, const double* d_uf, double u0, int64_t n,
+                                double* d_muf, int32_t* d_tpm, int32_t* d_ncase, double* d_iu,
+                                int32_t* d_fp64_roots, void* stream);
+
+/*
  * GEOR for n rays (geokerr_original.f:2672-3134, FIRSTPT = .true., PHT =
  * .false.): the inverse radius uf at which a photon from (u0[i], mu0[i])
  * with constants of motion L[i], Q2[i], spin[i] and initial signs su[i],

@@ -135,6 +135,26 @@ def geomu_camera_batch(alpha, beta, spin, mu0, uf, u0, stream=None):
     return muf, tpm, ncase, iu
 
 
+def geomu_camera_batch_mixed(alpha, beta, spin, mu0, uf, u0, stream=None):
+    """GEOMU from screen coordinates with the quartic's roots in mixed
+    precision (np_geomu_camera_batch_mixed); returns (muf, tpm, ncase, iu,
+    fp64_roots) device tensors, fp64_roots = 1 where the FP64 root finder
+    ran."""
+    import torch
+    n = alpha.numel()
+    muf = torch.empty(n, dtype=torch.float64, device=alpha.device)
+    iu = torch.empty_like(muf)
+    tpm = torch.empty(n, dtype=torch.int32, device=alpha.device)
+    ncase = torch.empty_like(tpm)
+    fb = torch.empty_like(tpm)
+    code = lib().np_geomu_camera_batch_mixed(alpha.data_ptr(), beta.data_ptr(), spin.data_ptr(),
+                                             mu0.data_ptr(), uf.data_ptr(), float(u0), n, muf.data_ptr(),
+                                             tpm.data_ptr(), ncase.data_ptr(), iu.data_ptr(), fb.data_ptr(),
+                                             _stream_ptr(stream))
+    check(code, "np_geomu_camera_batch_mixed")
+    return muf, tpm, ncase, iu, fb
+
+
 def geomu_batch(u0, uf, mu0, spin, L, Q2, su, sm, tpr=None, stream=None):
     """GEOMU on constants of motion (np_geomu_batch)."""
     import torch

@@ -142,6 +142,8 @@ def _declare(L: C.CDLL) -> None:
     L.np_geokerr_batch.restype = i32
     L.np_geomu_camera_batch.argtypes = [vp] * 5 + [d, i64] + [vp] * 5
     L.np_geomu_camera_batch.restype = i32
+    L.np_geomu_camera_batch_mixed.argtypes = [vp] * 5 + [d, i64] + [vp] * 6
+    L.np_geomu_camera_batch_mixed.restype = i32
     f = C.c_float
     L.np_schw_precompute.argtypes = [f, f, vp, i64, vp, vp, vp, vp]
     L.np_schw_precompute.restype = i32

@@ -20,20 +20,22 @@ using gk::GkVars;
 
 struct GeomuResult {
     double muf, iu;
-    int tpm, ncase;
+    int tpm, ncase, fp64_roots;
 };
 
 // GEOMU with FIRSTPT = .true., PHT = .false. on a fresh geodesic; inlined
 // so that no kernel here calls a function (a long leaf function's
 // long-branch sequences may reuse its return-address registers).
+template <bool MIXED = false>
 __device__ __forceinline__ GeomuResult geomu_once(double u0, double uf, double mu0, double a, double l, double q2, int tpr,
                                          double su, double sm) {
     GkVars v{};
     int tpm = 0;
     GeomuResult r;
-    gk::geomu<true, false>(v, u0, uf, false, mu0, a, l, q2, tpm, tpr, su, sm, r.muf, r.iu);
+    gk::geomu<true, false, MIXED>(v, u0, uf, false, mu0, a, l, q2, tpm, tpr, su, sm, r.muf, r.iu);
     r.tpm = tpm;
     r.ncase = v.ncase;
+    r.fp64_roots = v.fp64_roots;
     return r;
 }
 
@@ -54,21 +56,26 @@ __device__ __forceinline__ void camera_constants(double alpha, double beta, doub
 // Both GEOMU kernels are capped at 256 registers (2 waves per SIMD): left
 // alone the compiler takes 256 VGPRs plus AGPRs, 1 wave per SIMD, and the
 // 2^20-ray batch runs 1.4x slower.
+// MIXED: the quartic's roots from a float deflation polished in double
+// (gk::zroots4_mixed), FP64 ZROOTS where that hands back; fp64_roots flags
+// those rays.
+template <bool MIXED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 geomu_camera_kernel(const double* __restrict__ alpha, const double* __restrict__ beta,
                     const double* __restrict__ spin, const double* __restrict__ mu0,
                     const double* __restrict__ uf, const double u0, const int64_t n,
                     double* __restrict__ muf, int32_t* __restrict__ tpm, int32_t* __restrict__ ncase,
-                    double* __restrict__ iu) {
+                    double* __restrict__ iu, int32_t* __restrict__ fp64_roots) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         double L, Q2, a, su, sm;
         camera_constants(alpha[i], beta[i], mu0[i], spin[i], L, Q2, a, su, sm);
-        const GeomuResult r = geomu_once(u0, uf[i], mu0[i], a, L, Q2, 0, su, sm);
+        const GeomuResult r = geomu_once<MIXED>(u0, uf[i], mu0[i], a, L, Q2, 0, su, sm);
         muf[i] = r.muf;
         if (tpm) tpm[i] = r.tpm;
         if (ncase) ncase[i] = r.ncase;
         if (iu) iu[i] = r.iu;
+        if (MIXED && fp64_roots) fp64_roots[i] = r.fp64_roots;
     }
 }
 
@@ -148,9 +155,22 @@ int np_geomu_camera_batch(const double* d_alpha, const double* d_beta, const dou
     if (n < 0 || (n > 0 && (!d_alpha || !d_beta || !d_spin || !d_mu0 || !d_uf || !d_muf)))
         return NP_EINVAL;
     if (n == 0) return NP_OK;
-    hipLaunchKernelGGL(np::geomu_camera_kernel, dim3(np::grid_for(n)), dim3(256), 0,
+    hipLaunchKernelGGL(np::geomu_camera_kernel<false>, dim3(np::grid_for(n)), dim3(256), 0,
                        (hipStream_t)stream, d_alpha, d_beta, d_spin, d_mu0, d_uf, u0, n, d_muf,
-                       d_tpm, d_ncase, d_iu);
+                       d_tpm, d_ncase, d_iu, (int32_t*)nullptr);
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int np_geomu_camera_batch_mixed(const double* d_alpha, const double* d_beta, const double* d_spin,
+                                const double* d_mu0, const double* d_uf, double u0, int64_t n,
+                                double* d_muf, int32_t* d_tpm, int32_t* d_ncase, double* d_iu,
+                                int32_t* d_fp64_roots, void* stream) {
+    if (n < 0 || (n > 0 && (!d_alpha || !d_beta || !d_spin || !d_mu0 || !d_uf || !d_muf)))
+        return NP_EINVAL;
+    if (n == 0) return NP_OK;
+    hipLaunchKernelGGL(np::geomu_camera_kernel<true>, dim3(np::grid_for(n)), dim3(256), 0,
+                       (hipStream_t)stream, d_alpha, d_beta, d_spin, d_mu0, d_uf, u0, n, d_muf,
+                       d_tpm, d_ncase, d_iu, d_fp64_roots);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

@@ -512,6 +512,131 @@ __device__ inline void zroots(const cplx* a, int m, cplx* roots) {
     }
 }
 
+// ---------------- mixed-precision roots of the quartic U(u) ----------------
+// The deflation half of ZROOTS in float, the polish on the full polynomial
+// in double (LAGUER as above), for GEOMU's quartic U(u) = 1 + cc u^2 +
+// dd u^3 + ee u^4.  The float phase decides which roots are real; it hands
+// the polynomial back to the FP64 ZROOTS (returns false) when any root's
+// imaginary part lies between 1e-5 and 1e-2 of its modulus or two real
+// roots lie within 1e-3 of each other (relative) -- the near-degenerate
+// arguments where a float deflation could pick another case.
+struct cplxf {
+    float re, im;
+};
+__device__ __forceinline__ cplxf mkf(float r, float i = 0.0f) { return cplxf{r, i}; }
+__device__ __forceinline__ cplxf operator+(cplxf a, cplxf b) { return mkf(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cplxf operator-(cplxf a, cplxf b) { return mkf(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ cplxf operator*(cplxf a, cplxf b) {
+    return mkf(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+__device__ __forceinline__ cplxf operator*(float s, cplxf a) { return mkf(s * a.re, s * a.im); }
+__device__ __forceinline__ float cabs_f(cplxf a) { return hypotf(a.re, a.im); }
+__device__ __forceinline__ cplxf operator/(cplxf a, cplxf b) {
+    if (fabsf(b.re) >= fabsf(b.im)) {
+        const float r = b.im / b.re, d = b.re + b.im * r;
+        return mkf((a.re + a.im * r) / d, (a.im - a.re * r) / d);
+    }
+    const float r = b.re / b.im, d = b.re * r + b.im;
+    return mkf((a.re * r + a.im) / d, (a.im * r - a.re) / d);
+}
+__device__ __forceinline__ cplxf csqrt_f(cplxf z) {
+    if (z.re == 0.0f && z.im == 0.0f) return mkf(0.0f, z.im);
+    const float m = cabs_f(z);
+    const float t = sqrtf(0.5f * (m + fabsf(z.re)));
+    if (z.re >= 0.0f) return mkf(t, z.im / (2.0f * t));
+    return mkf(fabsf(z.im) / (2.0f * t), copysignf(t, z.im));
+}
+
+// LAGUER's iteration in float (same step control; float round-off bound).
+__device__ inline void laguer_f(const cplxf* a, int m, cplxf& x) {
+    const float frac[8] = {0.5f, 0.25f, 0.75f, 0.13f, 0.38f, 0.62f, 0.88f, 1.0f};
+    const int MT = 10, MAXIT = 80;
+    const float EPSS = 1e-7f;
+    for (int iter = 1; iter <= MAXIT; ++iter) {
+        cplxf b = a[m];
+        float err = cabs_f(b);
+        cplxf d = mkf(0.0f), f = mkf(0.0f);
+        const float abx = cabs_f(x);
+        for (int j = m - 1; j >= 0; --j) {
+            f = x * f + d;
+            d = x * d + b;
+            b = x * b + a[j];
+            err = cabs_f(b) + abx * err;
+        }
+        err = EPSS * err;
+        if (cabs_f(b) <= err) return;
+        const cplxf g = d / b;
+        const cplxf g2 = g * g;
+        const cplxf h = g2 - 2.0f * (f / b);
+        const cplxf sq = csqrt_f((float)(m - 1) * ((float)m * h - g2));
+        cplxf gp = g + sq;
+        const cplxf gm = g - sq;
+        const float abp = cabs_f(gp), abm = cabs_f(gm);
+        if (abp < abm) gp = gm;
+        cplxf dx;
+        if (fmaxf(abp, abm) > 0.0f) {
+            dx = mkf((float)m) / gp;
+        } else {
+            const float e = expf(logf(1.0f + abx));
+            dx = mkf(e * cosf((float)iter), e * sinf((float)iter));
+        }
+        const cplxf x1 = x - dx;
+        if (x.re == x1.re && x.im == x1.im) return;
+        if (iter % MT != 0) x = x1;
+        else x = x - frac[iter / MT - 1] * dx;
+    }
+}
+
+__device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
+    cplxf ad[5];
+    for (int j = 0; j <= 4; ++j) ad[j] = mkf((float)a[j].re, (float)a[j].im);
+    cplxf xstart = mkf(0.0f);
+    int ncc = 0;
+    cplxf rf[4];
+    for (int j = 4; j >= 1; --j) {
+        cplxf x = xstart;
+        if (j < 4 && rf[j].im != 0.0f && ncc == 0) {
+            xstart = mkf(rf[j].re, -rf[j].re);
+            ncc = 1;
+        } else {
+            xstart = mkf(0.0f);
+            ncc = 0;
+        }
+        laguer_f(ad, j, x);
+        const float ax = cabs_f(x);
+        if (!(ax > 0.0f) || !isfinite(ax)) return false;
+        const float rim = fabsf(x.im) / ax;
+        if (rim <= 1e-5f) x = mkf(x.re);
+        else if (rim < 1e-2f) return false;
+        rf[j - 1] = x;
+        cplxf b = ad[j];
+        for (int jj = j - 1; jj >= 0; --jj) {
+            const cplxf c = ad[jj];
+            ad[jj] = b;
+            b = x * b + c;
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j)
+            if (rf[i].im == 0.0f && rf[j].im == 0.0f &&
+                fabsf(rf[i].re - rf[j].re) <= 1e-3f * fmaxf(fabsf(rf[i].re), fabsf(rf[j].re)))
+                return false;
+    for (int j = 0; j < 4; ++j) {
+        roots[j] = mk((double)rf[j].re, (double)rf[j].im);
+        laguer(a, 4, roots[j]);
+    }
+    for (int j = 1; j < 4; ++j) {
+        const cplx x = roots[j];
+        int i;
+        for (i = j - 1; i >= 0; --i) {
+            if (roots[i].re <= x.re) break;
+            roots[i + 1] = roots[i];
+        }
+        roots[i + 1] = x;
+    }
+    return true;
+}
+
 // SNCNDN (geokerr_original.f:3277-3340).
 __device__ inline void sncndn(double uu, double emmc, double& sn, double& cn, double& dn) {
     const double CA = 3e-8;
@@ -571,7 +696,7 @@ __device__ inline void sncndn(double uu, double emmc, double& sn, double& cn, do
 // GEOKERR's variables shared by its GEOMU and GEOPHITIME calls along one
 // geodesic (geokerr_original.f:398-412).
 struct GkVars {
-    int32_t ncase, pad_;
+    int32_t ncase, fp64_roots;   // fp64_roots: set when the mixed GEOMU fell back to FP64 ZROOTS
     double u1, u2, u3, u4, h1;
     double iu0, rffu0, rffu1;
     double i1mu, i3mu, rffmu1, rffmu2, rffmu3;
@@ -621,7 +746,7 @@ __device__ inline double calcimuf_rf(bool sym, double mneg, double mpos, double 
 // variable as both u0 and uf (geokerr_estimate_periods).  With FIRSTPT =
 // false the quartic roots, ncase, the u0 piece of rows 1/2/7/8 and the mu0
 // integrals come from v.
-template <bool FIRSTPT, bool PHT>
+template <bool FIRSTPT, bool PHT, bool MIXED = false>
 __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool alias, const double mu0, const double a,
                              const double l, const double q2, int& tpm, int& tpr, const double su,
                              const double sm, double& muf, double& iu) {
@@ -738,7 +863,14 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
         if constexpr (FIRSTPT) {
             cplx c[5], root[4];
             c[0] = mk(one); c[1] = mk(0.0); c[2] = mk(cc); c[3] = mk(dd); c[4] = mk(ee);
-            zroots(c, 4, root);
+            if constexpr (MIXED) {
+                if (!zroots4_mixed(c, root)) {
+                    zroots(c, 4, root);
+                    v.fp64_roots = 1;
+                }
+            } else {
+                zroots(c, 4, root);
+            }
             int nreal = 0;
             for (int i = 0; i < 4; ++i) nreal += (root[i].im == 0.0);
             if (nreal == 2) {

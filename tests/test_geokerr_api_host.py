@@ -36,6 +36,13 @@ def test_geor_batch_argument_checks():
     assert L.np_geor_batch(*([None] * 10), 0, *([None] * 5)) == 0
 
 
+def test_geomu_camera_batch_mixed_argument_checks():
+    L = npa.lib()
+    assert L.np_geomu_camera_batch_mixed(*([None] * 5), 1e-4, 1, *([None] * 6)) == -1
+    assert L.np_geomu_camera_batch_mixed(*([None] * 5), 1e-4, -1, *([None] * 6)) == -1
+    assert L.np_geomu_camera_batch_mixed(*([None] * 5), 1e-4, 0, *([None] * 6)) == 0
+
+
 def test_geokerr_batch_argument_checks():
     # the whole-driver entry point checks its switches and pointers on the
     # host; n = 0 with valid switches is a no-op
