@@ -174,8 +174,9 @@ int np_geomu_camera_batch(const double* d_alpha, const double* d_beta, const dou
 /*
  * The same with the quartic's roots in mixed precision (the GEOKERR batch's
  * compared mode: FP32 bulk, FP64 for near-degenerate arguments): ZROOTS'
- * deflation runs in float, each root is polished on the full polynomial in
- * double (LAGUER, geokerr_original.f:3527-3606), and the FP64 ZROOTS runs
+ * deflation (LAGUER, geokerr_original.f:3527-3606) runs in float, each root
+ * is polished on the full polynomial with three Newton steps in double, and
+ * the FP64 ZROOTS runs
  * instead where the float phase cannot tell a real root from a complex pair
  * (imaginary part between 1e-5 and 1e-2 of the modulus) or finds two real
  * roots within 1e-3 of each other.  d_fp64_roots (n flags, or NULL) marks

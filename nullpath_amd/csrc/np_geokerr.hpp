@@ -514,7 +514,7 @@ __device__ inline void zroots(const cplx* a, int m, cplx* roots) {
 
 // ---------------- mixed-precision roots of the quartic U(u) ----------------
 // The deflation half of ZROOTS in float, the polish on the full polynomial
-// in double (LAGUER as above), for GEOMU's quartic U(u) = 1 + cc u^2 +
+// in double (Newton steps), for GEOMU's quartic U(u) = 1 + cc u^2 +
 // dd u^3 + ee u^4.  The float phase decides which roots are real; it hands
 // the polynomial back to the FP64 ZROOTS (returns false) when any root's
 // imaginary part lies between 1e-5 and 1e-2 of its modulus or two real
@@ -621,9 +621,21 @@ __device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
             if (rf[i].im == 0.0f && rf[j].im == 0.0f &&
                 fabsf(rf[i].re - rf[j].re) <= 1e-3f * fmaxf(fabsf(rf[i].re), fabsf(rf[j].re)))
                 return false;
+    // three Newton steps in double on the full polynomial: from a float
+    // root of a simple (separated) root they reach double precision; a real
+    // root stays on the real axis
     for (int j = 0; j < 4; ++j) {
-        roots[j] = mk((double)rf[j].re, (double)rf[j].im);
-        laguer(a, 4, roots[j]);
+        cplx x = mk((double)rf[j].re, (double)rf[j].im);
+        for (int it = 0; it < 3; ++it) {
+            cplx p = a[4], dp = mk(0.0);
+            for (int k = 3; k >= 0; --k) {
+                dp = x * dp + p;
+                p = x * p + a[k];
+            }
+            if (dp.re == 0.0 && dp.im == 0.0) break;
+            x = x - p / dp;
+        }
+        roots[j] = x;
     }
     for (int j = 1; j < 4; ++j) {
         const cplx x = roots[j];
