@@ -682,7 +682,7 @@ int np_geokerr_batch(const np_geokerr_opts* opts, const np_geokerr_rays* rays, i
         !r.tpr)
         return NP_EINVAL;
     if (!s.ufi || !s.mufi || !s.dti || !s.dphi || !s.lambdai || !s.tpmi || !s.tpri) return NP_EINVAL;
-    if (n > INT64_MAX / total) return NP_EINVAL;
+    if (n > INT64_MAX / total || (n + 255) / 256 > INT32_MAX) return NP_EINVAL;
     const dim3 grid((unsigned)((n + 255) / 256));
     if (o.usegeor)
         hipLaunchKernelGGL(gk_full_geor_kernel, grid, dim3(256), 0, (hipStream_t)stream, o, r, n, s);
