@@ -28,7 +28,7 @@ mpmath = pytest.importorskip("mpmath")
 U0 = min(1e-4, 1.0 / (100.0 * (144.0 ** 2 + 144.0 ** 2)))
 
 
-def config3_rays(n, seed):
+def geokerr_batch_rays(n, seed):
     rng = np.random.default_rng(seed)
     alpha = rng.uniform(-12, 12, n)
     beta = rng.uniform(-12, 12, n)
@@ -103,9 +103,9 @@ def _imu_path(a, l, q2, mu0, muf, tpm, sm):
     return float(total)
 
 
-def test_mino_time_identity_config3_rays(oracle):
+def test_mino_time_identity_geokerr_batch_rays(oracle):
     mpmath.mp.dps = 30
-    alpha, beta, a, mu0, uf = config3_rays(4000, 11)
+    alpha, beta, a, mu0, uf = geokerr_batch_rays(4000, 11)
     muf, tpm, ncase, iu = oracle.geomu_batch(alpha, beta, a, mu0, uf, U0)
     picked = []
     for c in (5, 6, 7):

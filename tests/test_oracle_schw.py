@@ -76,4 +76,4 @@ def test_fp64_sweep_classes_and_deflection(oracle):
     # photon-sphere band
     cls2, _, _ = oracle.schw_sweep(M, r0, 2.01 * M, 1e-3 * M, 2_000_000, np.array([bc + 5e-4 * M]))
     assert cls2[0] == 2
-    assert steps.max() > 10000   # config-1 rays need more than the reference's 10000-step cap
+    assert steps.max() > 10000   # rays of the FP64 sweep from r0 = 1000 M need more than the reference's 10000-step cap
