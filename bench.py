@@ -322,9 +322,13 @@ def bench_batch(spp: int = 2) -> dict:
     e1.record(stream)
     torch.cuda.synchronize()
     seq_ms = e0.elapsed_time(e1)
-    same = all(torch.equal(o, out[3 * off:3 * (off + f.width * f.height)])
-               for o, f, off in zip(outs, frames,
-                                    [sum(g.width * g.height for g in frames[:i]) for i in range(len(frames))]))
+    offs = [sum(g.width * g.height for g in frames[:i]) for i in range(len(frames))]
+    same = all(torch.equal(o, out[3 * off:3 * (off + f.width * f.height)]) for o, f, off in zip(outs, frames, offs))
+    seq = torch.cat(outs)
+    ndiff = int((seq != out).view(-1, 3).any(dim=1).sum().item())
+    relpix = ((seq - out).abs() / seq.abs().clamp_min(1e-30)).view(-1, 3).max(dim=1).values
+    rel = relpix.max().item()
+    n_gt = int((relpix > 1e-5).sum().item())
     fl = FLOPS["fp32c"]
     f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + (c["attempts"] - c["attempts_f64"]) * fl["attempt"]
     f64 = c["rhs_evals_f64"] * fl["rhs64"] + c["attempts_f64"] * fl["attempt64"]
@@ -338,6 +342,9 @@ def bench_batch(spp: int = 2) -> dict:
         "per_frame_launches_ms": seq_ms,
         "speedup_over_per_frame_launches": seq_ms / batch_ms,
         "outputs_bitwise_equal_to_per_frame": bool(same),
+        "pixels_differing_from_per_frame": ndiff,
+        "max_rel_diff_vs_per_frame": rel,
+        "pixels_rel_diff_above_1e-5": n_gt,
         "tuning": best.as_dict(),
         "autotune_s": tune_s,
         "autotune_trials": [{"tuning": t, "ms": ms} for t, ms in trials],
