@@ -14,6 +14,8 @@ DRIVER_OPTS = {
     "mufill_k0": dict(nup=24, npts=6, kext=0, phit=1, usegeor=0, mufill=1, offset=0.0),
     "mufill_k2": dict(nup=20, npts=10, kext=2, phit=1, usegeor=0, mufill=1, offset=0.5),
     "nophit": dict(nup=12, npts=4, kext=1, phit=0, usegeor=0, mufill=1, offset=0.5),
+    # the reference's validation programs (geokerr_single.f:60-67): one point at UF
+    "single_point": dict(nup=1, npts=0, kext=0, phit=1, usegeor=0, mufill=0, offset=0.0),
 }
 
 
