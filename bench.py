@@ -84,6 +84,9 @@ def parse():
     ap.add_argument("--geokerr-only", type=int, default=0,
                     help="only time geokerr_eval_lambda batches of this many geodesics x 4096 "
                          "samples (for profiling that path) and exit")
+    ap.add_argument("--geomu-only", type=int, default=0,
+                    help="only time the GEOMU batch (FP64 and the mixed-precision roots) on this many "
+                         "rays and exit")
     ap.add_argument("--geokerr-driver-only", type=int, default=0,
                     help="only time the whole GEOKERR driver (MUFILL and USEGEOR runs) on this many "
                          "geodesics and exit")
@@ -620,6 +623,9 @@ def main() -> int:
         return copy_calibration(args.copy_calibration_mib)
     if args.geokerr_driver_only > 0:
         print(json.dumps(bench_geokerr_driver(args.geokerr_driver_only, 0)))
+        return 0
+    if args.geomu_only > 0:
+        print(json.dumps(bench_geomu(args.geomu_only)))
         return 0
     if args.geokerr_only > 0:
         print(json.dumps(bench_geokerr(args.geokerr_only, 4096, 0)))
