@@ -3,9 +3,9 @@
  *
  * Member names and order match the reference's include/bh_types.h:52-135
  * (SchwarzschildMetric), :163-251 (GeodesicState, 8 floats),
- * :281-357 (GeodesicLookupTable, caller-owned new[] arrays) and
- * include/bh_kerr.h:553-556 (GeodesicStateD, 8 doubles), so the
+ * :281-357 (GeodesicLookupTable, caller-owned new[] arrays), so the
  * structures are layout-compatible with code built against them.
+ * GeodesicStateD is in bh_kerr.h, as in the reference.
  */
 #ifndef BH_TYPES_H
 #define BH_TYPES_H
@@ -39,12 +39,6 @@ struct GeodesicState {
     float pphi;
 
     BH_HD GeodesicState() : t(0), r(0), theta(0), phi(0), pt(0), pr(0), ptheta(0), pphi(0) {}
-};
-
-/* FP64 photon state. */
-struct GeodesicStateD {
-    double t, r, theta, phi;
-    double pt, pr, ptheta, pphi;
 };
 
 /* Result of precomputeGeodesics(); every array has num_entries elements and

@@ -443,6 +443,37 @@ double orc_hermite_crossing(double h, double r0, double th0, double dr0, double 
     return (2 * s3 - 3 * s2 + 1) * r0 + (s3 - 2 * s2 + s) * n0 + (3 * s2 - 2 * s3) * r1 + (s3 - s2) * n1;
 }
 
+/* Initial state of one sample (render.cu:143-169): camera direction, then
+ * the static or ZAMO tetrad (ZAMO forced where the static observer is not
+ * timelike). */
+static void ray_start(const np_render_params* p, int x, int y, uint32_t sample_id, double state[8],
+                      float cam[2]) {
+    const float Rs = 2.0f * p->mass;
+    const float Mf = p->mass;
+    const float af = p->spin * Mf;
+    const float r_cam = p->r_cam_Rs * Rs;
+    const double M = (double)Mf, a = (double)af;
+    float n[3];
+    orc_camera_dir(p, x, y, sample_id, n, cam);
+    int static_ok = orc_static_timelike_d(M, a, (double)r_cam, (double)p->theta_cam);
+    if (p->camera == NP_CAMERA_ZAMO || !static_ok)
+        orc_init_zamo_d(M, a, r_cam, p->theta_cam, p->phi_cam, n[0], n[1], n[2], state);
+    else
+        orc_init_static_d(M, a, r_cam, p->theta_cam, p->phi_cam, n[0], n[1], n[2], state);
+}
+
+void orc_ray_p0_pixels(const np_render_params* p, int npix, const int32_t* xs, const int32_t* ys,
+                       int sample_offset, int spp, double* p0) {
+    for (int i = 0; i < npix; ++i)
+        for (int s = 0; s < spp; ++s) {
+            double st[8];
+            float cam[2];
+            ray_start(p, xs[i], ys[i], (uint32_t)(sample_offset + s), st, cam);
+            double* o = p0 + 4 * ((size_t)i * spp + s);
+            o[0] = st[PT_]; o[1] = st[PR_]; o[2] = st[PTH_]; o[3] = st[PPH_];
+        }
+}
+
 void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
                     orc_ray_info* info) {
     const float Rs = 2.0f * p->mass;
@@ -450,15 +481,9 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
     const float af = p->spin * Mf;
     const float r_cam = p->r_cam_Rs * Rs;
     const double M = (double)Mf, a = (double)af;
-    float n[3], cam[2];
-    orc_camera_dir(p, x, y, sample_id, n, cam);
-
+    float cam[2];
     double state[8];
-    int static_ok = orc_static_timelike_d(M, a, (double)r_cam, (double)p->theta_cam);
-    if (p->camera == NP_CAMERA_ZAMO || !static_ok)
-        orc_init_zamo_d(M, a, r_cam, p->theta_cam, p->phi_cam, n[0], n[1], n[2], state);
-    else
-        orc_init_static_d(M, a, r_cam, p->theta_cam, p->phi_cam, n[0], n[1], n[2], state);
+    ray_start(p, x, y, sample_id, state, cam);
 
     const double pt0 = state[PT_], pph0 = state[PPH_];
     info->p0[0] = state[PT_]; info->p0[1] = state[PR_]; info->p0[2] = state[PTH_]; info->p0[3] = state[PPH_];

@@ -85,6 +85,10 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
  * the reference's order (render.cu:141-363).  rgb is npix*3 floats.
  * info (optional) gets npix*spp entries, pixel-major.  nthreads <= 0 uses
  * every core. */
+/* Initial covariant momenta (p_t, p_r, p_theta, p_phi) of every sample,
+ * pixel-major then sample, without tracing. */
+void orc_ray_p0_pixels(const np_render_params* p, int npix, const int32_t* xs, const int32_t* ys,
+                       int sample_offset, int spp, double* p0);
 void orc_render_pixels(const np_render_params* p, int npix, const int32_t* xs,
                        const int32_t* ys, int sample_offset, int spp, float* rgb,
                        orc_ray_info* info, int nthreads);

@@ -115,6 +115,9 @@ def _declare(L: C.CDLL) -> None:
     L.orc_doppler_d.restype = d
     L.orc_rk4_sum_d.argtypes = [d, d, d, d]
     L.orc_rk4_sum_d.restype = d
+    L.orc_ray_p0_pixels.argtypes = [C.POINTER(RenderParams), i32, C.c_void_p, C.c_void_p, i32, i32,
+                                    C.c_void_p]
+    L.orc_ray_p0_pixels.restype = None
     L.orc_wanghash.argtypes = [C.c_uint32]
     L.orc_wanghash.restype = C.c_uint32
     L.orc_camera_dir.argtypes = [C.POINTER(RenderParams), i32, i32, C.c_uint32,
@@ -314,6 +317,16 @@ def render_pixels(p: RenderParams, xs, ys, sample_offset=0, spp=1, nthreads=0, w
     lib().orc_render_pixels(C.byref(p), n, xs.ctypes.data, ys.ctypes.data, sample_offset, spp,
                             rgb.ctypes.data, info.ctypes.data if with_info else None, nthreads)
     return rgb, info
+
+
+def ray_p0_pixels(p: RenderParams, xs, ys, sample_offset=0, spp=1):
+    """Initial (p_t, p_r, p_theta, p_phi) of every sample, shape (npix*spp, 4)."""
+    xs = _arr(xs, np.int32)
+    ys = _arr(ys, np.int32)
+    out = np.zeros((xs.size * spp, 4), dtype=np.float64)
+    lib().orc_ray_p0_pixels(C.byref(p), xs.size, xs.ctypes.data, ys.ctypes.data, sample_offset, spp,
+                            out.ctypes.data)
+    return out
 
 
 def render_image(p: RenderParams, spp=1, sample_offset=0, nthreads=0):

@@ -99,22 +99,39 @@ def _disk_pair(width, height, fov, spp=1, **kw):
 
 
 @pytest.mark.gpu
-def test_fp32c_matches_fp64_on_disk_workload(device):
+def test_fp32c_matches_fp64_on_disk_workload(oracle, device):
     """FP32-compensated vs FP64 (both Hermite crossings) on the Kerr disk
-    workload zoomed on the hole: identical class and crossing count for every
-    ray whose FP64 closest approach is not within 0.05 M of the photon-orbit
-    region's edge of capture, disk radiance within 1e-5 (p99) and 1e-4 (max)."""
+    workload zoomed on the hole (160x90, fov 3 deg): every ray outside the
+    band of kerr_band (critical margin |m| < 1e-4, sin(theta_min) < 1e-3, or
+    an oracle crossing radius within 1e-6 relative of a disk edge) has the
+    same class and crossing count in both; the radiance of every disk pixel
+    whose samples are all outside the band agrees to 1e-5 relative (max);
+    sky pixels are bitwise equal."""
+    import kerr_band as KB
     (ia, ca, ra), (ib, cb, rb) = _disk_pair(160, 90, 3.0, crossing=1)
+    p = npa.kerr_disk_params(width=160, height=90, fov=3.0, seed=1, precision=0, crossing=1)
+    q = oracle.default_params()
+    for name, _ in npa.RenderParams._fields_:
+        setattr(q, name, getattr(p, name))
+    ys, xs = np.mgrid[0:p.height, 0:p.width]
+    xs, ys = xs.ravel(), ys.ravel()
+    band, margin, _, pole = KB.ray_band(p, oracle.ray_p0_pixels(q, xs, ys, 0, 1))
     same = (ra["cls"] == rb["cls"]) & (ra["orders"] == rb["orders"])
-    assert np.mean(same) > 0.999, int((~same).sum())
-    disk = (ra["orders"] > 0) & same
+    diff = np.flatnonzero(~same & ~band)
+    if diff.size:   # the edge band needs the crossing radii: trace those rays
+        assert diff.size < 50, diff.size
+        _, info = oracle.render_pixels(q, xs[diff], ys[diff], 0, 1)
+        edge = KB.edge_distance(info["xr"], info["ncross"], p.disk_rin, p.disk_rout)
+        assert np.all(edge < KB.EDGE_BAND), (margin[diff], edge)
+        band[diff] = True
+    disk = (ra["orders"] > 0) & ~band
     assert disk.sum() > 1000
     rel = (np.abs(ib - ia) / np.maximum(np.abs(ia), 1e-30)).max(axis=1)[disk]
     print(f"[gpu] fp32c vs fp64 disk rel err: median {np.median(rel):.3g} p99 "
-          f"{np.quantile(rel, 0.99):.3g} max {rel.max():.3g}", flush=True)
-    assert np.quantile(rel, 0.99) <= 1e-5
-    assert rel.max() <= 1e-4
-    sky = same & (ra["orders"] == 0)
+          f"{np.quantile(rel, 0.99):.3g} max {rel.max():.3g}; {(~same).sum()} rays differ, "
+          f"{band.sum()} in the band", flush=True)
+    assert rel.max() <= 1e-5
+    sky = ~band & (ra["orders"] == 0)
     assert np.array_equal(ia[sky], ib[sky])
 
 
