@@ -23,7 +23,8 @@ typedef enum np_status {
     NP_EALLOC = -2,    /* device or pinned-host allocation failed */
     NP_ELAUNCH = -3,   /* kernel launch or execution failed */
     NP_ENODEV = -4,    /* no HIP device */
-    NP_EDISABLED = -5  /* entry point switched off in this build (see its comment) */
+    NP_EDISABLED = -5, /* entry point switched off in this build (see its comment) */
+    NP_ECAPACITY = -6  /* request beyond a fixed capacity (np_render_batch_noalloc) */
 } np_status;
 
 /* Human-readable text for a status code (static storage). */
@@ -114,6 +115,28 @@ typedef struct np_tuning {
 int np_render_batch(const np_render_params* frames, int nframes, int spp, int sample_offset,
                     float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
                     const np_tuning* tuning, void* stream);
+
+/* Frames np_render_batch_noalloc accepts per call. */
+#define NP_NOALLOC_MAX_FRAMES 128
+
+/*
+ * np_render_batch without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as np_render_batch; the
+ * frame table and the per-frame work counters live in a fixed device array
+ * of the library (about 47 KB per GPU, held until the process ends) instead
+ * of a per-call allocation.  At most NP_NOALLOC_MAX_FRAMES frames
+ * (NP_ECAPACITY beyond).  The caller still provides d_rgb and the optional
+ * d_records / d_counters.  The call copies the frame table into that array
+ * and waits for the copy on `stream` before launching (the host staging
+ * buffer is reused), so it returns after earlier work on `stream` has
+ * finished.  Every call on one GPU shares the array: calls that overlap on
+ * one GPU (other streams or threads) race.
+ */
+int np_render_batch_noalloc(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                            float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
+                            const np_tuning* tuning, void* stream);
 
 /*
  * Time every candidate tuning (block 128/256 x chunk 1/4, and for FP32C the

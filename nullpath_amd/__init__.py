@@ -386,10 +386,11 @@ def render_accumulate(p: RenderParams, accum, spp_done=0, samples_per_pass=1, pa
 
 
 def render_batch(frames, out, spp=1, sample_offset=0, records=None, counters=None, tuning=None,
-                 stream=None) -> None:
+                 stream=None, noalloc=False) -> None:
     """All frames (a list of RenderParams) in one persistent launch
     (np_render_batch); `out` is one float32 device tensor holding the frames'
-    colour sums back to back."""
+    colour sums back to back.  noalloc=True calls np_render_batch_noalloc
+    (no device allocation; at most NP_NOALLOC_MAX_FRAMES frames)."""
     import ctypes as C
 
     from ._lib import Tuning
@@ -399,11 +400,11 @@ def render_batch(frames, out, spp=1, sample_offset=0, records=None, counters=Non
     if out.numel() < 3 * total or out.dtype.itemsize != 4 or not out.is_contiguous():
         raise NullPathError("out must be a contiguous float32 device tensor of 3 * total pixels")
     t = None if tuning is None else C.byref(tuning if isinstance(tuning, Tuning) else Tuning(**tuning))
-    code = lib().np_render_batch(arr, n, spp, sample_offset, out.data_ptr(),
-                                 None if records is None else records.data_ptr(),
-                                 None if counters is None else counters.data_ptr(), t,
-                                 _stream_ptr(stream))
-    check(code, "np_render_batch")
+    fn = lib().np_render_batch_noalloc if noalloc else lib().np_render_batch
+    code = fn(arr, n, spp, sample_offset, out.data_ptr(),
+              None if records is None else records.data_ptr(),
+              None if counters is None else counters.data_ptr(), t, _stream_ptr(stream))
+    check(code, "np_render_batch_noalloc" if noalloc else "np_render_batch")
 
 
 def autotune_batch(frames, spp=1, subsample=8, max_trials=64):

@@ -156,6 +156,8 @@ def _declare(L: C.CDLL) -> None:
     prp, pt = C.POINTER(RenderParams), C.POINTER(Tuning)
     L.np_render_batch.argtypes = [prp, i32, i32, i32, vp, vp, vp, pt, vp]
     L.np_render_batch.restype = i32
+    L.np_render_batch_noalloc.argtypes = [prp, i32, i32, i32, vp, vp, vp, pt, vp]
+    L.np_render_batch_noalloc.restype = i32
     L.np_autotune_batch.argtypes = [prp, i32, i32, i32, pt, pt, C.POINTER(C.c_double), i32,
                                     C.POINTER(C.c_int)]
     L.np_autotune_batch.restype = i32

@@ -114,9 +114,8 @@ __global__ void zero_u64_n(unsigned long long* p, int n) {
 // The frames of a batch, checked and laid out: one precision and one
 // integrator for all frames, no invariant monitor.
 int batch_frames(const np_render_params* frames, int nframes, int spp, const np_tuning* t,
-                 std::vector<BatchFrame>& out, uint64_t* total_px) {
+                 BatchFrame* out, uint64_t* total_px) {
     if (!frames || nframes <= 0 || spp < 0) return NP_EINVAL;
-    out.resize((size_t)nframes);
     uint64_t off = 0;
     for (int i = 0; i < nframes; ++i) {
         np_render_params q = frames[i];
@@ -138,16 +137,46 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, const np_
     return NP_OK;
 }
 
+int check_tuning(const np_tuning* tuning, np_tuning* t) {
+    *t = tuning ? *tuning : np_tuning{256, 3, 1, 0.0};
+    if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256) return NP_EINVAL;
+    if (t->refill_chunk < 1 || t->refill_chunk > 1024) return NP_EINVAL;
+    return NP_OK;
+}
+
+// Persistent batch launch over a frame table and cleared per-frame counters
+// already in device memory.
+int batch_kernel(const np_render_params* frames, int nframes, const BatchFrame* host0,
+                 const BatchFrame* d_frames, unsigned long long* d_work, int cus, int spp,
+                 int sample_offset, uint64_t total_px, float* d_rgb, np_ray_record* d_records,
+                 np_trace_counters* d_counters, const np_tuning& t, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_u64_n, dim3(1), dim3(256), 0, stream, d_work, nframes);
+    if (hipGetLastError() != hipSuccess) return NP_ELAUNCH;
+    const bool adaptive = frames[0].integrator == NP_INTEGRATOR_ADAPTIVE;
+    const int variant = host0->F.variant;
+    int per_cu = 0;
+    if (occupancy_kerr_batch(frames[0].precision, adaptive, variant, t.block_size, &per_cu) != NP_OK)
+        per_cu = 1;
+    if (per_cu <= 0) per_cu = 1;
+    uint64_t grid = (uint64_t)per_cu * (uint64_t)cus;
+    const uint64_t need = (total_px + (uint64_t)t.block_size - 1) / (uint64_t)t.block_size;
+    if (need < grid) grid = need;
+    if (grid < 1) grid = 1;
+    return launch_kerr_batch(d_frames, nframes, frames[0].precision, adaptive, variant, spp,
+                             sample_offset, d_rgb, d_counters, d_records, d_work, stream, (int)grid,
+                             t.block_size, t.refill_chunk);
+}
+
 int batch_launch(const np_render_params* frames, int nframes, int spp, int sample_offset,
                  float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
                  const np_tuning* tuning, hipStream_t stream) {
     if (!d_rgb) return NP_EINVAL;
-    np_tuning t = tuning ? *tuning : np_tuning{256, 3, 1, 0.0};
-    if (t.block_size != 64 && t.block_size != 128 && t.block_size != 256) return NP_EINVAL;
-    if (t.refill_chunk < 1 || t.refill_chunk > 1024) return NP_EINVAL;
-    std::vector<BatchFrame> host;
+    np_tuning t;
+    if (check_tuning(tuning, &t) != NP_OK) return NP_EINVAL;
+    if (!frames || nframes <= 0) return NP_EINVAL;
+    std::vector<BatchFrame> host((size_t)nframes);
     uint64_t total_px = 0;
-    int err = batch_frames(frames, nframes, spp, &t, host, &total_px);
+    int err = batch_frames(frames, nframes, spp, &t, host.data(), &total_px);
     if (err != NP_OK) return err;
     DeviceState* d = device_state(&err);
     if (!d) return err;
@@ -167,21 +196,9 @@ int batch_launch(const np_render_params* frames, int nframes, int spp, int sampl
     if (hipMemcpyAsync(d_frames, host.data(), table, hipMemcpyHostToDevice, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)   // host table released on return
         err = NP_ELAUNCH;
-    if (err == NP_OK) {
-        hipLaunchKernelGGL(zero_u64_n, dim3(1), dim3(256), 0, stream, d_work, nframes);
-        if (hipGetLastError() != hipSuccess) err = NP_ELAUNCH;
-    }
-    if (err == NP_OK) {
-        const bool adaptive = frames[0].integrator == NP_INTEGRATOR_ADAPTIVE;
-        const int variant = host[0].F.variant;
-        int per_cu = 0;
-        if (occupancy_kerr_batch(frames[0].precision, adaptive, variant, t.block_size, &per_cu) != NP_OK)
-            per_cu = 1;
-        const int grid = persistent_grid(d, per_cu, t.block_size, total_px);
-        err = launch_kerr_batch(d_frames, nframes, frames[0].precision, adaptive, variant, spp,
-                                sample_offset, d_rgb, d_counters, d_records, d_work, stream, grid,
-                                t.block_size, t.refill_chunk);
-    }
+    if (err == NP_OK)
+        err = batch_kernel(frames, nframes, host.data(), d_frames, d_work, d->cus, spp,
+                           sample_offset, total_px, d_rgb, d_records, d_counters, t, stream);
     if (stream_ordered) {
         if (hipFreeAsync(buf, stream) != hipSuccess && err == NP_OK) err = NP_ELAUNCH;
     } else {
@@ -190,6 +207,82 @@ int batch_launch(const np_render_params* frames, int nframes, int spp, int sampl
         (void)hipFree(buf);
     }
     return err;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free batch render (np_render_batch_noalloc).  The frame table
+// and the per-frame work counters live in `scratch`, a fixed device array of
+// this file, instead of a stream-ordered allocation per call.  The host-side
+// name of a __device__ array is not its device address: the address is
+// taken once per device with hipGetSymbolAddress (that device current) and
+// only that pointer is handed to copies and kernels.  The host copy of the
+// table is staged in a fixed per-device buffer under a mutex.  Every call on
+// one device shares its scratch: calls that overlap on one GPU (different
+// streams, or threads) race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchTable =
+    ((size_t)NP_NOALLOC_MAX_FRAMES * sizeof(BatchFrame) + 255) & ~(size_t)255;
+constexpr size_t kScratchBytes = kScratchTable + (size_t)NP_NOALLOC_MAX_FRAMES * sizeof(unsigned long long);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct ScratchState {
+    std::once_flag once;
+    int status = NP_OK;
+    unsigned char* ptr = nullptr;   // device address of `scratch` on this device
+    int cus = 0;
+    std::mutex mu;                  // guards `stage`
+    BatchFrame stage[NP_NOALLOC_MAX_FRAMES];
+};
+
+ScratchState g_scratch[kMaxDevices];
+
+ScratchState* scratch_state(int* err) {
+    int dev = 0;
+    *err = current_device(&dev);
+    if (*err != NP_OK) return nullptr;
+    ScratchState& s = g_scratch[dev];
+    std::call_once(s.once, [&s, dev]() {
+        if (hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            s.cus <= 0) {
+            s.status = NP_ENODEV;
+            return;
+        }
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            s.status = NP_ELAUNCH;
+            return;
+        }
+        s.ptr = static_cast<unsigned char*>(p);
+    });
+    *err = s.status;
+    return s.status == NP_OK ? &s : nullptr;
+}
+
+int batch_launch_noalloc(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                         float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
+                         const np_tuning* tuning, hipStream_t stream) {
+    if (!d_rgb || !frames || nframes <= 0) return NP_EINVAL;
+    if (nframes > NP_NOALLOC_MAX_FRAMES) return NP_ECAPACITY;
+    np_tuning t;
+    if (check_tuning(tuning, &t) != NP_OK) return NP_EINVAL;
+    int err = NP_OK;
+    ScratchState* s = scratch_state(&err);
+    if (!s) return err;
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint64_t total_px = 0;
+    err = batch_frames(frames, nframes, spp, &t, s->stage, &total_px);
+    if (err != NP_OK) return err;
+    if (total_px == 0 || spp == 0) return NP_OK;
+    BatchFrame* d_frames = reinterpret_cast<BatchFrame*>(s->ptr);
+    unsigned long long* d_work = reinterpret_cast<unsigned long long*>(s->ptr + kScratchTable);
+    // the staging buffer is reused by the next call: wait for the copy
+    if (hipMemcpyAsync(d_frames, s->stage, (size_t)nframes * sizeof(BatchFrame), hipMemcpyHostToDevice,
+                       stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return NP_ELAUNCH;
+    return batch_kernel(frames, nframes, s->stage, d_frames, d_work, s->cus, spp, sample_offset,
+                        total_px, d_rgb, d_records, d_counters, t, stream);
 }
 
 // Autotuner: times every candidate tuning on a subsampled copy of the batch
@@ -314,6 +407,7 @@ const char* np_status_string(int code) {
         case NP_ELAUNCH: return "kernel launch or execution failed";
         case NP_ENODEV: return "no HIP device";
         case NP_EDISABLED: return "entry point disabled in this build";
+        case NP_ECAPACITY: return "request exceeds a fixed capacity of the entry point";
         default: return "unknown status";
     }
 }
@@ -369,6 +463,13 @@ int np_render_batch(const np_render_params* frames, int nframes, int spp, int sa
                     const np_tuning* tuning, void* stream) {
     return np::batch_launch(frames, nframes, spp, sample_offset, d_rgb, d_records, d_counters,
                             tuning, (hipStream_t)stream);
+}
+
+int np_render_batch_noalloc(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                            float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,
+                            const np_tuning* tuning, void* stream) {
+    return np::batch_launch_noalloc(frames, nframes, spp, sample_offset, d_rgb, d_records, d_counters,
+                                    tuning, (hipStream_t)stream);
 }
 
 int np_autotune_batch(const np_render_params* frames, int nframes, int spp, int subsample,
