@@ -49,14 +49,29 @@ FP32C_HBM_BYTES_1080P4 = (282358.5625 * 2.0 + 772341.625 * 1.0) * 1024.0
 #     sums (10 x 18), the full step's increments with float-float r and
 #     theta (26), the two half steps' float-float updates (2 x 56) and the
 #     noise-floored error norm and step factor (90).
+#   fp32c float steps without the error estimate (far field): 7 stage inputs
+#     and sums (7 x 18), the two half steps' float-float updates (2 x 56) and
+#     the pole bound that allows the skip (8).
 #   fp32c FP64 steps: the FP64 Carter right-hand side with its polynomial
 #     sincos is 105 flops; per attempt 12 stage inputs and sums (12 x 18),
 #     3 updates (3 x 10) and the error norm (40).
 FLOPS = {
     "fp64": {"rhs": 200, "attempt": 3 * 196 + 29},
-    "fp32c": {"rhs": 70, "attempt": 10 * 18 + 26 + 2 * 56 + 90,
+    "fp32c": {"rhs": 70, "attempt": 10 * 18 + 26 + 2 * 56 + 90, "attempt_lazy": 7 * 18 + 2 * 56 + 8,
               "rhs64": 105, "attempt64": 12 * 18 + 3 * 10 + 40},
 }
+
+
+def valu_flops(c: dict, prec: str):
+    """(FP32, FP64) algorithmic flops of a launch from its counters."""
+    fl = FLOPS[prec]
+    if prec == "fp64":
+        return 0, c["rhs_evals"] * fl["rhs"] + c["attempts"] * fl["attempt"]
+    lazy = c.get("attempts_lazy", 0)
+    f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + \
+        (c["attempts"] - c["attempts_f64"] - lazy) * fl["attempt"] + lazy * fl["attempt_lazy"]
+    f64 = c["rhs_evals_f64"] * fl["rhs64"] + c["attempts_f64"] * fl["attempt64"]
+    return f32, f64
 
 
 def parse():
@@ -128,7 +143,7 @@ def render_case(prec: str, crossing: str, width: int, height: int, spp: int, ste
                              rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid, crossing=cr,
                              waves_per_simd=waves_per_simd)
     out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
-    cnt = torch.zeros(10, dtype=torch.int64, device=dev)
+    cnt = npa.new_counters(dev)
     rec = torch.zeros(rays * 32, dtype=torch.uint8, device=dev)
     npa.render_tile(npa.kerr_disk_params(width=64, height=36, seed=1, precision=pid, crossing=cr),
                     out, spp=1)
@@ -153,14 +168,7 @@ def render_case(prec: str, crossing: str, width: int, height: int, spp: int, ste
     assert c["rays"] == rays, c
     kern_ms = [a.elapsed_time(b) for a, b in ev]
     mean_s = sum(kern_ms) / len(kern_ms) / 1e3
-    fl = FLOPS[prec]
-    if prec == "fp32c":
-        f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + \
-              (c["attempts"] - c["attempts_f64"]) * fl["attempt"]
-        f64 = c["rhs_evals_f64"] * fl["rhs64"] + c["attempts_f64"] * fl["attempt64"]
-    else:
-        f32 = 0
-        f64 = c["rhs_evals"] * fl["rhs"] + c["attempts"] * fl["attempt"]
+    f32, f64 = valu_flops(c, prec)
     # fraction of the VALU peak: FP32 work at the FP32 rate, FP64 at the FP64 rate
     valu_s = f32 / (PEAK_TFLOPS["fp32"] * 1e12) + f64 / (PEAK_TFLOPS["fp64"] * 1e12)
     return {
@@ -180,6 +188,7 @@ def case_summary(r: dict) -> dict:
         "accepted_steps_per_ray": cc["steps"] / cc["rays"],
         "attempts_per_ray": cc["attempts"] / cc["rays"],
         "fp64_attempt_frac": cc["attempts_f64"] / max(1, cc["attempts"]),
+        "lazy_attempt_frac": cc.get("attempts_lazy", 0) / max(1, cc["attempts"]),
         "rk4_steps_per_s": 3 * cc["attempts"] / r["mean_s"],
         "rhs_evals_per_s": cc["rhs_evals"] / r["mean_s"],
         "tflops_fp32": r["flops32"] / r["mean_s"] / 1e12,
@@ -305,7 +314,7 @@ def bench_batch(spp: int = 2) -> dict:
     best, trials = npa.autotune_batch(frames, spp=spp, subsample=8)
     tune_s = time.perf_counter() - t0
     out = torch.zeros(3 * total, dtype=torch.float32, device="cuda")
-    cnt = torch.zeros(10, dtype=torch.int64, device="cuda")
+    cnt = npa.new_counters()
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -335,9 +344,7 @@ def bench_batch(spp: int = 2) -> dict:
     relpix = ((seq - out).abs() / seq.abs().clamp_min(1e-30)).view(-1, 3).max(dim=1).values
     rel = relpix.max().item()
     n_gt = int((relpix > 1e-5).sum().item())
-    fl = FLOPS["fp32c"]
-    f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + (c["attempts"] - c["attempts_f64"]) * fl["attempt"]
-    f64 = c["rhs_evals_f64"] * fl["rhs64"] + c["attempts_f64"] * fl["attempt64"]
+    f32, f64 = valu_flops(c, "fp32c")
     achieved = (f32 + 2.0 * f64) / (batch_ms / 1e3) / 1e12
     return {
         "workload": f"{len(frames)} frames (256^2..1024^2, a in {{0,0.5,0.9,0.998}}, theta U[5,89] deg), "

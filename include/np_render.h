@@ -81,6 +81,12 @@ typedef struct np_render_params {
                                  (2 waves per SIMD); other values are rejected */
     int32_t crossing;         /* np_crossing: disk-crossing radius between accepted steps
                                  (reference: linear in theta) */
+    int32_t estimate_every_step; /* NP_PREC_FP32C, adaptive: 0 = a ray's steps at the maximum
+                                 step size beyond 100 M skip the step-doubling error estimate
+                                 (the full step) once one there came out below 1e-3, while the
+                                 pole stays clear of the step; there the truncation error of a
+                                 step is ~1e-5 of the tolerance, far from the 0.59 that would
+                                 shrink the next step; 1 = every step estimates its error */
 } np_render_params;
 
 /* Fill *p with the reference renderer's defaults (render.cu:15-46 with
@@ -98,6 +104,7 @@ static inline void np_render_params_default(np_render_params* p) {
     p->emiss_q = 2.0f; p->seed = 0u; p->fp64_radius = 0.0;
     p->waves_per_simd = 0;
     p->crossing = NP_CROSSING_LINEAR;
+    p->estimate_every_step = 0;
 }
 
 /* Kerr prograde ISCO radius in units of M (Bardeen, Press & Teukolsky 1972,

@@ -35,7 +35,9 @@ const char* np_status_string(int code);
  * adaptive attempts; rhs_evals counts Hamilton right-hand-side evaluations
  * actually performed.  attempts_f64 and rhs_evals_f64 count the part of
  * attempts and rhs_evals that an NP_PREC_FP32C launch ran in FP64 (inside
- * fp64_radius, or a float attempt's FP64 re-run); they stay 0 for FP64. */
+ * fp64_radius, or a float attempt's FP64 re-run); they stay 0 for FP64.
+ * attempts_lazy counts the float attempts that ran without the full step
+ * (estimate_every_step = 0: the two half steps only). */
 typedef struct np_trace_counters {
     uint64_t rays;
     uint64_t steps;
@@ -47,6 +49,7 @@ typedef struct np_trace_counters {
     uint64_t disk_hits;
     uint64_t attempts_f64;
     uint64_t rhs_evals_f64;
+    uint64_t attempts_lazy;
 } np_trace_counters;
 
 /* Optional per-sample diagnostic record (extension; the reference's kernel
@@ -125,7 +128,7 @@ int np_render_batch(const np_render_params* frames, int nframes, int spp, int sa
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, kernels and results as np_render_batch; the
  * frame table and the per-frame work counters live in a fixed device array
- * of the library (about 47 KB per GPU, held until the process ends) instead
+ * of the library (about 48 KB per GPU, held until the process ends) instead
  * of a per-call allocation.  At most NP_NOALLOC_MAX_FRAMES frames
  * (NP_ECAPACITY beyond).  The caller still provides d_rgb and the optional
  * d_records / d_counters.  The call copies the frame table into that array

@@ -95,6 +95,12 @@ def render_tile(p: RenderParams, out, y0=0, rows=None, spp=1, sample_offset=0, i
     check(code, "np_render_tile")
 
 
+def new_counters(device="cuda"):
+    """A zeroed device buffer for np_trace_counters (uint64 fields as int64)."""
+    import torch
+    return torch.zeros(len(COUNTER_FIELDS), dtype=torch.int64, device=device)
+
+
 def counters_dict(t) -> dict:
     """Decode a uint64 counter tensor (np_trace_counters, 10 elements) into a dict."""
     vals = t.cpu().tolist()

@@ -32,7 +32,7 @@ class RenderParams(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
         ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
-        ("crossing", C.c_int32),
+        ("crossing", C.c_int32), ("estimate_every_step", C.c_int32),
     ]
 
     def copy(self) -> "RenderParams":
@@ -74,7 +74,7 @@ class TraceCounters(C.Structure):
 
     _fields_ = [(n, C.c_uint64) for n in (
         "rays", "steps", "attempts", "rhs_evals", "captured", "escaped", "stepcap", "disk_hits",
-        "attempts_f64", "rhs_evals_f64")]
+        "attempts_f64", "rhs_evals_f64", "attempts_lazy")]
 
 
 COUNTER_FIELDS = [f[0] for f in TraceCounters._fields_]

@@ -518,7 +518,8 @@ struct StepF64 {
     // One integrator attempt.  Returns 1 accepted, 0 rejected (retry),
     // 2 failed (the ray is captured: horizon, pole or rejection budget).
     template <bool ADAPTIVE>
-    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& /*rhs64*/) {
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& /*rhs64*/,
+                                  int& /*lazy*/) {
         const double Rs = F.Rs_d;
         if (!ADAPTIVE) {
             // integrateGeodesicKerrD (bh_kerr.h:734-774), h = 0.01 * scale
@@ -854,10 +855,14 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
 // half1.  The full step is returned as its float increments inc = full - y
 // (all the error norm needs) and as float-float r and theta (for the domain
 // check); the half steps as float-float states.
+// full_step (wave-uniform) = false skips stages 0-2 (the full step), for a
+// wave whose lanes all skip the error estimate; inc, full_r and full_th are
+// then left unset.
 __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5& y, const DerivF& k1,
                                                   const float s0, const float c0, const float E,
-                                                  const float L, const float h, DerivF& inc, ff& full_r,
-                                                  ff& full_th, FF5& half1, FF5& half2) {
+                                                  const float L, const float h, const bool full_step,
+                                                  DerivF& inc, ff& full_r, ff& full_th, FF5& half1,
+                                                  FF5& half2) {
     const float h2 = 0.5f * h;
     DerivF S = k1, kp = k1;
     float br = ff_f(y.r), bthl = y.th.lo, bpr = ff_f(y.pr), bpth = ff_f(y.pth);
@@ -865,6 +870,7 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
     // unrolled: the stage coefficients below fold to constants
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
+        if (j < 3 && !full_step) continue;
         const float c = (j <= 1) ? 0.5f * h : (j == 2) ? h : (j == 5 || j == 9) ? h2 : 0.5f * h2;
         const float cc = (j == 6) ? 0.0f : c;
         const DerivF k = kerr_rhs_f32_sc(F, br + cc * kp.r, bs, bc, bthl + cc * kp.th, bpr + cc * kp.pr,
@@ -929,15 +935,27 @@ __device__ __forceinline__ bool okf(const FrameConst& F, float r_hi, float th_hi
     return r_hi > F.r_okf && th_hi > F.th_okf_lo && th_hi < F.th_okf_hi;
 }
 
+// Error level below which a ray's later steps at the maximum step size
+// beyond F.r_lazyf skip their estimate: the estimate would have to grow past
+// 0.59 (0.9 e^-0.2 < 1) to shrink the next step, and past 1 to reject, while
+// there the truncation error of a 0.2 M step is ~1e-5 of the tolerance.
+constexpr float kLazyError = 1e-3f;
+
 struct StepF32C {
     struct Ray : RayBase {
         FF5 y;
         DerivF k1;
         float E, L, h_adapt, h;
+        // sqrt(Q + a^2 E^2) >= |p_theta| anywhere on the ray (Q = Carter's
+        // constant): bounds how far one step can move theta
+        float qb;
         bool k1_valid;
         // 0: float attempt; 1: FP64 (step started inside r_isl); 2: FP64
         // re-run of a float attempt that failed or was rejected
         uint8_t mode;
+        // 1 once an error estimate beyond r_lazyf at the maximum step size
+        // came out below kLazyError; cleared when the ray leaves that regime
+        uint8_t skip;
     };
 
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
@@ -955,6 +973,14 @@ struct StepF32C {
         R.h = 0.0f;
         R.k1_valid = false;
         R.mode = 0;
+        R.skip = 0;
+        {
+            double st, ct;
+            sincos_d(F.th_cam, st, ct);
+            const double E = -R.pt, L = R.pph, a2E2 = F.a * F.a * E * E;
+            const double Q = pth * pth + ct * ct * (L * L / fmax(st * st, 1e-300) - a2E2);
+            R.qb = (float)(sqrt(fmax(Q + a2E2, 0.0)) * (1.0 + 1e-6));
+        }
         base_start(F, R, cam_x, cam_y);
     }
 
@@ -1036,7 +1062,8 @@ struct StepF32C {
     }
 
     template <bool ADAPTIVE>
-    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& rhs64) {
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& rhs64,
+                                  int& lazy_att) {
         const float hmin = (float)HMIN, hmax = (float)HMAX;
         if (!ADAPTIVE) {
             const float rf = ff_f(R.y.r);
@@ -1068,7 +1095,10 @@ struct StepF32C {
             R.h = fminf(fmaxf(h, hmin), hmax);
             R.mode = (ff_to_d(R.y.r) < F.r_isl) ? 1 : 0;
         }
-        if (R.mode != 0) return attempt64(F, R, rhs64);
+        if (R.mode != 0) {
+            R.skip = 0;
+            return attempt64(F, R, rhs64);
+        }
         const float h = R.h;
         float s0, c0;
         sincosf(R.y.th.hi, &s0, &c0);
@@ -1080,8 +1110,37 @@ struct StepF32C {
         DerivF inc;
         ff full_r, full_th;
         FF5 half1, half2;
-        step_doubling_f32(F, R.y, R.k1, s0, c0, R.E, R.L, h, inc, full_r, full_th, half1, half2);
-        rhs_count += 10;
+        const bool far = h == hmax && R.y.r.hi >= F.r_lazyf;
+        bool lazy = false;
+        if (far && R.skip) {
+            // The skipped full step stays inside the domain: every stage is
+            // beyond r - 1 >= 99 M from the hole, where |dtheta/dlambda| =
+            // |p_theta| / Sigma <= qb / (r - 1)^2; its theta moves by at most
+            // h times that, kept 1e-6 clear of the pole thresholds.
+            const float rm = R.y.r.hi - 1.0f * F.Mf;
+            const float dth = h * R.qb * rcpf(rm * rm) + 1e-6f;
+            lazy = R.y.th.hi - dth > F.th_okf_lo && R.y.th.hi + dth < F.th_okf_hi;
+        }
+        R.skip = lazy;
+        // the full step runs when any float lane of the wave estimates its
+        // error (a uniform branch: a lane's own result never depends on it)
+        const bool full_step = __ballot(!lazy) != 0;
+        step_doubling_f32(F, R.y, R.k1, s0, c0, R.E, R.L, h, full_step, inc, full_r, full_th, half1,
+                          half2);
+        rhs_count += full_step ? 10 : 7;
+        lazy_att = full_step ? 0 : 1;
+        if (lazy) {
+            if (okf(F, half1.r.hi, half1.th.hi) && okf(F, half2.r.hi, half2.th.hi)) {
+                R.y = half2;
+                R.k1_valid = false;
+                R.h_adapt = hmax;
+                R.attempt = 0;
+                return 1;
+            }
+            R.skip = 0;
+            R.mode = 2;
+            return 0;
+        }
         if (okf(F, full_r.hi, full_th.hi) && okf(F, half1.r.hi, half1.th.hi) &&
             okf(F, half2.r.hi, half2.th.hi)) {
             const float rt = F.rtolf, at = F.atolf;
@@ -1098,6 +1157,7 @@ struct StepF32C {
                 R.k1_valid = false;
                 R.h_adapt = fminf(fmaxf(h * fac, hmin), hmax);
                 R.attempt = 0;
+                R.skip = F.lazy && far && e < kLazyError;
                 return 1;
             }
         }
@@ -1113,25 +1173,35 @@ struct StepF32C {
 // Persistent renderer: work item = pixel (or a chunk of consecutive pixels),
 // lane-level refill, one integrator attempt per loop iteration.
 //
-// Single-frame mode renders rows [y0, y0+rows) of the frame F0.  Batch mode
-// renders whole frames from a device table; a wave works on one frame at a
-// time (so the frame constants stay wave-uniform and are read with scalar
-// loads), refilling its lanes from that frame's work counter, and moves to
-// the next frame with work left once all of its lanes are idle.  Waves start
-// staggered over the frames, so the drain at a frame switch is rare.
+// Every launch renders a table of frame records in device memory (one
+// record for a single tile: its row window; one per frame for a batch).  A
+// wave works on one frame at a time, so the frame constants stay
+// wave-uniform and are read with scalar loads from the record (they are not
+// held in SGPRs across the whole kernel), refilling its lanes from that
+// frame's work counter, and moves to the next frame with work left once all
+// of its lanes are idle.  Waves start staggered over the frames, so the
+// drain at a frame switch is rare.  A tile and a frame of a batch run the
+// same kernel code.
 // ---------------------------------------------------------------------------
-template <class S, bool ADAPTIVE, bool INV, bool BATCH>
+// The frame record's address behind an empty asm: loads through it cannot
+// be hoisted out of the attempt loop, so the constants the per-ray set-up and
+// shading read (tetrad, camera, background) are fetched when used instead of
+// occupying SGPRs for the whole frame.
+__device__ __forceinline__ const FrameConst& cold(const BatchFrame* B) {
+    const BatchFrame* p = B;
+    asm volatile("" : "+s"(p));
+    return p->F;
+}
+
+template <class S, bool ADAPTIVE, bool INV>
 __device__ __forceinline__ void
-kerr_body(const FrameConst& F0, const BatchFrame* __restrict__ frames, const int nframes,
-          const int y0, const int rows, const int spp, const int sample_offset,
-          float* __restrict__ out_base, const int accumulate, float* __restrict__ inv_max3,
-          np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records_base,
-          unsigned long long* __restrict__ work, const int chunk) {
-    int fid = 0;
-    if (BATCH) {
-        const int wave_global = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-        fid = __builtin_amdgcn_readfirstlane(wave_global % nframes);
-    }
+kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int spp,
+          const int sample_offset, float* __restrict__ out_base, const int accumulate,
+          float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
+          np_ray_record* __restrict__ records_base, unsigned long long* __restrict__ work,
+          const int chunk) {
+    const int wave_global = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int fid = __builtin_amdgcn_readfirstlane(wave_global % nframes);
     typename S::Ray R;
     bool has = false, done = false;
     uint64_t pix = 0, chunk_end = 0;
@@ -1140,138 +1210,140 @@ kerr_body(const FrameConst& F0, const BatchFrame* __restrict__ frames, const int
     float inv_lane[3] = {0.0f, 0.0f, 0.0f};
     // per-lane counters, flushed to the u64 totals before they can wrap
     uint32_t c_rhs = 0, c_att = 0, c_steps = 0, c_rays = 0, c_cap = 0, c_esc = 0, c_cap_steps = 0,
-             c_disk = 0, c_rhs64 = 0, c_att64 = 0;
+             c_disk = 0, c_rhs64 = 0, c_att64 = 0, c_lazy = 0;
     if (spp <= 0) done = true;
 
+    // Outer loop: one frame at a time; inner loop: one integrator attempt per
+    // iteration, in which the frame record's address is invariant, so the
+    // constants every attempt reads are loaded once per frame.
     while (true) {
-        const FrameConst& F = BATCH ? frames[fid].F : F0;
+        const BatchFrame* B = frames + fid;
+        const FrameConst& F = B->F;
         const int W = F.width;
-        const int ybase = BATCH ? 0 : y0;
-        uint64_t npix;
-        if (BATCH) {
-            npix = frames[fid].npix;
-        } else {
-            const int y_end = min(y0 + rows, F.height);
-            npix = (y_end > y0) ? (uint64_t)(y_end - y0) * (uint64_t)W : 0ull;
-        }
+        const uint64_t npix = B->npix;
         if (F.max_steps <= 0) done = true;
-        const bool need = !has && !done;
-        if (__ballot(need)) {
-            const bool in_chunk = need && (pix + 1 < chunk_end);
-            const bool fetch = need && !in_chunk;
-            const uint64_t unit = wave_fetch(fetch, work + (BATCH ? fid : 0));
-            if (need) {
-                uint64_t p;
-                if (in_chunk) {
-                    p = pix + 1;
-                } else {
-                    p = unit * (uint64_t)chunk;
-                    chunk_end = min(p + (uint64_t)chunk, npix);
-                }
-                if (p < npix) {
-                    pix = p;
-                    sample = 0;
-                    acc[0] = acc[1] = acc[2] = 0.0f;
-                    const int x = (int)(pix % (uint64_t)W);
-                    const int yy = ybase + (int)(pix / (uint64_t)W);
-                    float n_r, n_th, n_ph, cx, cy;
-                    camera_dir(F, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
-                    S::start(F, R, n_r, n_th, n_ph, cx, cy);
-                    has = true;
-                } else {
-                    done = true;
-                }
-            }
-        }
-        if (__ballot(has) == 0) {
-            if (!BATCH) break;
-            // every lane is idle: move to the next frame that still has work
-            int next = -1;
-            for (int k = 1; k <= nframes; ++k) {
-                const int f = (fid + k) % nframes;
-                const unsigned long long taken =
-                    __hip_atomic_load(work + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (taken * (unsigned long long)chunk < frames[f].npix) {
-                    next = f;
-                    break;
-                }
-            }
-            if (next < 0) break;
-            fid = __builtin_amdgcn_readfirstlane(next);
-            done = (spp <= 0);
-            pix = 0;
-            chunk_end = 0;
-            continue;
-        }
-        if (counters && __ballot(c_rhs > 0x40000000u || c_steps > 0x40000000u)) {
-            wave_add_u64(&counters->steps, c_steps);
-            wave_add_u64(&counters->attempts, c_att);
-            wave_add_u64(&counters->rhs_evals, c_rhs);
-            wave_add_u64(&counters->attempts_f64, c_att64);
-            wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
-            c_steps = c_att = c_rhs = c_att64 = c_rhs64 = 0;
-        }
-        if (has) {
-            int rhs_count = 0, rhs64 = 0;
-            R.attempts_total += 1;
-            c_att += 1;
-            const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count, rhs64);
-            c_rhs += (uint32_t)(rhs_count + rhs64);
-            c_rhs64 += (uint32_t)rhs64;
-            c_att64 += (rhs64 > 0);
-            int cls = -1;
-            if (st == 2) cls = NP_RAY_CAPTURED;
-            else if (st == 1) cls = after_accept<S, INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
-            if (cls >= 0) {
-                float col[3];
-                shade_final(F, R, cls, col);
-                if (INV && cls != NP_RAY_CAPTURED && !(R.orders > 0 && F.disk_on))
-                    invariants(F, S::r(R), S::th(R), R.pt, S::pr(R), S::pth(R), R.pph, R.pt0, R.pph0, R.inv);
-                acc[0] += col[0];
-                acc[1] += col[1];
-                acc[2] += col[2];
-                inv_lane[0] = fmaxf(inv_lane[0], R.inv[0]);
-                inv_lane[1] = fmaxf(inv_lane[1], R.inv[1]);
-                inv_lane[2] = fmaxf(inv_lane[2], R.inv[2]);
-                c_rays += 1;
-                c_steps += (uint32_t)R.step;
-                c_cap += (cls == NP_RAY_CAPTURED);
-                c_esc += (cls == NP_RAY_ESCAPED);
-                c_cap_steps += (cls == NP_RAY_STEPCAP);
-                c_disk += (R.orders > 0);
-                if (records_base) {
-                    np_ray_record rec;
-                    rec.cls = cls;
-                    rec.orders = R.orders;
-                    rec.steps = R.step;
-                    rec.attempts = R.attempts_total;
-                    rec.min_r = R.min_r;
-                    rec.r_end = S::r(R);
-                    const uint64_t r0 = BATCH ? frames[fid].rec_off : 0ull;
-                    records_base[r0 + pix * (uint64_t)spp + (uint64_t)sample] = rec;
-                }
-                sample += 1;
-                const int x = (int)(pix % (uint64_t)W);
-                const int yy = ybase + (int)(pix / (uint64_t)W);
-                if (sample < spp) {
-                    float n_r, n_th, n_ph, cx, cy;
-                    camera_dir(F, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
-                    S::start(F, R, n_r, n_th, n_ph, cx, cy);
-                } else {
-                    float* o = out_base + (BATCH ? frames[fid].out_off : 0ull) + 3 * pix;
-                    if (accumulate) {
-                        o[0] += acc[0];
-                        o[1] += acc[1];
-                        o[2] += acc[2];
+        while (true) {
+            const bool need = !has && !done;
+            if (__ballot(need)) {
+                const bool in_chunk = need && (pix + 1 < chunk_end);
+                const bool fetch = need && !in_chunk;
+                const uint64_t unit = wave_fetch(fetch, work + fid);
+                if (need) {
+                    uint64_t p;
+                    if (in_chunk) {
+                        p = pix + 1;
                     } else {
-                        o[0] = acc[0];
-                        o[1] = acc[1];
-                        o[2] = acc[2];
+                        p = unit * (uint64_t)chunk;
+                        chunk_end = min(p + (uint64_t)chunk, npix);
                     }
-                    has = false;
+                    if (p < npix) {
+                        pix = p;
+                        sample = 0;
+                        acc[0] = acc[1] = acc[2] = 0.0f;
+                        const FrameConst& Fc = cold(B);
+                        const uint64_t ap = B->pix0 + pix;
+                        const int x = (int)(ap % (uint64_t)W);
+                        const int yy = (int)(ap / (uint64_t)W);
+                        float n_r, n_th, n_ph, cx, cy;
+                        camera_dir(Fc, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
+                        S::start(Fc, R, n_r, n_th, n_ph, cx, cy);
+                        has = true;
+                    } else {
+                        done = true;
+                    }
+                }
+            }
+            if (__ballot(has) == 0) break;
+            if (counters && __ballot(c_rhs > 0x40000000u || c_steps > 0x40000000u)) {
+                wave_add_u64(&counters->steps, c_steps);
+                wave_add_u64(&counters->attempts, c_att);
+                wave_add_u64(&counters->rhs_evals, c_rhs);
+                wave_add_u64(&counters->attempts_f64, c_att64);
+                wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
+                wave_add_u64(&counters->attempts_lazy, c_lazy);
+                c_steps = c_att = c_rhs = c_att64 = c_rhs64 = c_lazy = 0;
+            }
+            if (has) {
+                int rhs_count = 0, rhs64 = 0, lazy_att = 0;
+                R.attempts_total += 1;
+                c_att += 1;
+                const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count, rhs64, lazy_att);
+                c_lazy += (uint32_t)lazy_att;
+                c_rhs += (uint32_t)(rhs_count + rhs64);
+                c_rhs64 += (uint32_t)rhs64;
+                c_att64 += (rhs64 > 0);
+                int cls = -1;
+                if (st == 2) cls = NP_RAY_CAPTURED;
+                else if (st == 1) cls = after_accept<S, INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
+                if (cls >= 0) {
+                    const FrameConst& Fc = cold(B);
+                    float col[3];
+                    shade_final(Fc, R, cls, col);
+                    if (INV && cls != NP_RAY_CAPTURED && !(R.orders > 0 && Fc.disk_on))
+                        invariants(Fc, S::r(R), S::th(R), R.pt, S::pr(R), S::pth(R), R.pph, R.pt0, R.pph0,
+                                   R.inv);
+                    acc[0] += col[0];
+                    acc[1] += col[1];
+                    acc[2] += col[2];
+                    inv_lane[0] = fmaxf(inv_lane[0], R.inv[0]);
+                    inv_lane[1] = fmaxf(inv_lane[1], R.inv[1]);
+                    inv_lane[2] = fmaxf(inv_lane[2], R.inv[2]);
+                    c_rays += 1;
+                    c_steps += (uint32_t)R.step;
+                    c_cap += (cls == NP_RAY_CAPTURED);
+                    c_esc += (cls == NP_RAY_ESCAPED);
+                    c_cap_steps += (cls == NP_RAY_STEPCAP);
+                    c_disk += (R.orders > 0);
+                    if (records_base) {
+                        np_ray_record rec;
+                        rec.cls = cls;
+                        rec.orders = R.orders;
+                        rec.steps = R.step;
+                        rec.attempts = R.attempts_total;
+                        rec.min_r = R.min_r;
+                        rec.r_end = S::r(R);
+                        records_base[B->rec_off + pix * (uint64_t)spp + (uint64_t)sample] = rec;
+                    }
+                    sample += 1;
+                    if (sample < spp) {
+                        const uint64_t ap = B->pix0 + pix;
+                        const int x = (int)(ap % (uint64_t)W);
+                        const int yy = (int)(ap / (uint64_t)W);
+                        float n_r, n_th, n_ph, cx, cy;
+                        camera_dir(Fc, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
+                        S::start(Fc, R, n_r, n_th, n_ph, cx, cy);
+                    } else {
+                        float* o = out_base + B->out_off + 3 * pix;
+                        if (accumulate) {
+                            o[0] += acc[0];
+                            o[1] += acc[1];
+                            o[2] += acc[2];
+                        } else {
+                            o[0] = acc[0];
+                            o[1] = acc[1];
+                            o[2] = acc[2];
+                        }
+                        has = false;
+                    }
                 }
             }
         }
+        // every lane is idle: move to the next frame that still has work
+        int next = -1;
+        for (int k = 1; k <= nframes; ++k) {
+            const int f = (fid + k) % nframes;
+            const unsigned long long taken =
+                __hip_atomic_load(work + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (taken * (unsigned long long)chunk < frames[f].npix) {
+                next = f;
+                break;
+            }
+        }
+        if (next < 0) break;
+        fid = __builtin_amdgcn_readfirstlane(next);
+        done = (spp <= 0);
+        pix = 0;
+        chunk_end = 0;
     }
     if (counters) {
         wave_add_u64(&counters->rays, c_rays);
@@ -1284,6 +1356,7 @@ kerr_body(const FrameConst& F0, const BatchFrame* __restrict__ frames, const int
         wave_add_u64(&counters->disk_hits, c_disk);
         wave_add_u64(&counters->attempts_f64, c_att64);
         wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
+        wave_add_u64(&counters->attempts_lazy, c_lazy);
     }
     if (INV && inv_max3) {
         atomic_max_nonneg(&inv_max3[0], inv_lane[0]);
@@ -1293,52 +1366,29 @@ kerr_body(const FrameConst& F0, const BatchFrame* __restrict__ frames, const int
 }
 
 #define NP_KERNEL_ARGS                                                                          \
-    const FrameConst F, const int y0, const int rows, const int spp, const int sample_offset,    \
-        float* __restrict__ out, const int accumulate, float* __restrict__ inv_max3,            \
-        np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records,          \
-        unsigned long long* __restrict__ work, const int chunk
-#define NP_KERNEL_PASS F, nullptr, 0, y0, rows, spp, sample_offset, out, accumulate, inv_max3, \
-        counters, records, work, chunk
-#define NP_BATCH_ARGS                                                                           \
     const BatchFrame* __restrict__ frames, const int nframes, const int spp,                    \
-        const int sample_offset, float* __restrict__ out, np_trace_counters* __restrict__ counters, \
+        const int sample_offset, float* __restrict__ out, const int accumulate,                 \
+        float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,                 \
         np_ray_record* __restrict__ records, unsigned long long* __restrict__ work, const int chunk
-#define NP_BATCH_PASS frames[0].F, frames, nframes, 0, 0, spp, sample_offset, out, 0, nullptr, \
-        counters, records, work, chunk
+#define NP_KERNEL_PASS frames, nframes, spp, sample_offset, out, accumulate, inv_max3, counters, \
+        records, work, chunk
 
 // Kernel entry points.  Variant 0 leaves the register budget to the
 // compiler; the FP32C variants W = 3 and W = 4 ask for W waves per SIMD,
-// which caps the kernel at 168 or 128 VGPRs and moves the excess to
-// scratch.  At 168 that is the rarely taken FP64 branch; at 128 the step
-// loop spills too, which on the 1080p frame is 1.7 % faster but writes
-// about 1 TB of scratch back to HBM per frame (against 0.8 GB), so 3 is the
-// default.
+// which caps the kernel at 168 or 128 VGPRs.
 template <class S, bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
-    kerr_body<S, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
+    kerr_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
 }
 
 template <bool ADAPTIVE, bool INV, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 kerr_render_f32c_capped(NP_KERNEL_ARGS) {
-    kerr_body<StepF32C, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
-}
-
-template <class S, bool ADAPTIVE>
-__global__ void __launch_bounds__(256) kerr_batch(NP_BATCH_ARGS) {
-    kerr_body<S, ADAPTIVE, false, true>(NP_BATCH_PASS);
-}
-
-template <bool ADAPTIVE, int W>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
-kerr_batch_f32c_capped(NP_BATCH_ARGS) {
-    kerr_body<StepF32C, ADAPTIVE, false, true>(NP_BATCH_PASS);
+    kerr_body<StepF32C, ADAPTIVE, INV>(NP_KERNEL_PASS);
 }
 
 #undef NP_KERNEL_ARGS
 #undef NP_KERNEL_PASS
-#undef NP_BATCH_ARGS
-#undef NP_BATCH_PASS
 
 template <class S, bool A>
 const void* kernel_ptr(bool inv) {
@@ -1359,13 +1409,6 @@ const void* kernel_ptr_f32c(bool inv, int variant) {
     return kernel_ptr<StepF32C, A>(inv);
 }
 
-template <bool A>
-const void* batch_ptr_f32c(int variant) {
-    if (variant == 4) return reinterpret_cast<const void*>(&kerr_batch_f32c_capped<A, 4>);
-    if (variant == 3) return reinterpret_cast<const void*>(&kerr_batch_f32c_capped<A, 3>);
-    return reinterpret_cast<const void*>(&kerr_batch<StepF32C, A>);
-}
-
 // variant (FrameConst::variant): FP32C waves-per-SIMD cap, 0 = none.
 const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
     if (precision == NP_PREC_FP32C)
@@ -1373,22 +1416,22 @@ const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
     return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
 }
 
-const void* select_batch_kernel(int precision, bool adaptive, int variant) {
-    if (precision == NP_PREC_FP32C)
-        return adaptive ? batch_ptr_f32c<true>(variant) : batch_ptr_f32c<false>(variant);
-    return adaptive ? reinterpret_cast<const void*>(&kerr_batch<StepF64, true>)
-                    : reinterpret_cast<const void*>(&kerr_batch<StepF64, false>);
+__global__ void init_frame_record(const BatchFrame rec, BatchFrame* __restrict__ dst,
+                                  unsigned long long* __restrict__ counter) {
+    if (threadIdx.x == 0) {
+        *dst = rec;
+        *counter = 0ull;
+    }
 }
 
 }  // namespace
 
-int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int rows, int spp,
-                int sample_offset, float* d_out, int accumulate, float* d_inv,
+int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
+                int variant, int spp, int sample_offset, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
-    const void* fn = select_kernel(precision, adaptive, F.check_inv != 0, F.variant);
-    FrameConst Fc = F;
-    void* args[] = {&Fc, &y0, &rows, &spp, &sample_offset, &d_out, &accumulate, &d_inv,
+    const void* fn = select_kernel(precision, adaptive, inv, variant);
+    void* args[] = {&d_frames, &nframes, &spp, &sample_offset, &d_out, &accumulate, &d_inv,
                     &d_counters, &d_records, &d_work, &chunk};
     const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
     return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
@@ -1400,21 +1443,10 @@ int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int bloc
     return e == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
-int launch_kerr_batch(const BatchFrame* d_frames, int nframes, int precision, bool adaptive,
-                      int variant, int spp, int sample_offset, float* d_out,
-                      np_trace_counters* d_counters, np_ray_record* d_records,
-                      unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
-    const void* fn = select_batch_kernel(precision, adaptive, variant);
-    void* args[] = {&d_frames, &nframes, &spp, &sample_offset, &d_out, &d_counters, &d_records,
-                    &d_work, &chunk};
-    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
-    return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
-}
-
-int occupancy_kerr_batch(int precision, bool adaptive, int variant, int block, int* blocks_per_cu) {
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, select_batch_kernel(precision, adaptive, variant), block, 0);
-    return e == hipSuccess ? NP_OK : NP_ELAUNCH;
+int write_frame_record(const BatchFrame& rec, BatchFrame* d_dst, unsigned long long* d_counter,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(init_frame_record, dim3(1), dim3(64), 0, stream, rec, d_dst, d_counter);
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
 }  // namespace np

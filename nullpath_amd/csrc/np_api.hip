@@ -24,6 +24,7 @@ struct DeviceState {
     std::once_flag once;
     int status = NP_OK;
     unsigned long long* slots = nullptr;  // kWorkSlots counters
+    BatchFrame* frames = nullptr;         // kWorkSlots single-frame records, beside the counters
     std::atomic<unsigned> next{0};
     int cus = 0;
 };
@@ -54,10 +55,14 @@ DeviceState* device_state(int* err) {
             d.status = NP_ENODEV;
             return;
         }
-        if (hipMalloc(&d.slots, kWorkSlots * sizeof(unsigned long long)) != hipSuccess) {
+        void* buf = nullptr;
+        const size_t counters = kWorkSlots * sizeof(unsigned long long);
+        if (hipMalloc(&buf, counters + kWorkSlots * sizeof(BatchFrame)) != hipSuccess) {
             d.status = NP_EALLOC;
-            d.slots = nullptr;
+            return;
         }
+        d.slots = static_cast<unsigned long long*>(buf);
+        d.frames = reinterpret_cast<BatchFrame*>(static_cast<char*>(buf) + counters);
     });
     *err = d.status;
     return d.status == NP_OK ? &d : nullptr;
@@ -70,6 +75,16 @@ int take_slot(DeviceState* d, hipStream_t stream, unsigned long long** slot) {
     *slot = d->slots + k;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, stream, *slot, 1);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+// Take a queue counter and its frame record, and write `rec` there (with
+// the counter cleared) on `stream`.
+int take_frame_slot(DeviceState* d, hipStream_t stream, const BatchFrame& rec, BatchFrame** d_rec,
+                    unsigned long long** slot) {
+    unsigned k = d->next.fetch_add(1u) % kWorkSlots;
+    *slot = d->slots + k;
+    *d_rec = d->frames + k;
+    return write_frame_record(rec, *d_rec, *slot, stream);
 }
 
 int persistent_grid(DeviceState* d, int per_cu, int block, uint64_t items) {
@@ -94,17 +109,24 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     const int y_end = std::min(y0 + rows, p->height);
     const uint64_t npix = y_end > y0 ? (uint64_t)(y_end - y0) * (uint64_t)p->width : 0ull;
     if (npix == 0 || spp == 0) return NP_OK;
+    BatchFrame rec;
+    rec.F = F;
+    rec.npix = npix;
+    rec.pix0 = (uint64_t)y0 * (uint64_t)p->width;
+    rec.out_off = 0;
+    rec.rec_off = 0;
     unsigned long long* slot = nullptr;
-    err = take_slot(d, stream, &slot);
+    BatchFrame* d_rec = nullptr;
+    err = take_frame_slot(d, stream, rec, &d_rec, &slot);
     if (err != NP_OK) return err;
     const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
+    const bool inv = p->check_inv != 0;
     const int block = 256;
     int per_cu = 0;
-    if (occupancy_kerr(p->precision, adaptive, p->check_inv != 0, F.variant, block, &per_cu) != NP_OK)
-        per_cu = 1;
+    if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu) != NP_OK) per_cu = 1;
     const int grid = persistent_grid(d, per_cu, block, npix);
-    return launch_kerr(F, p->precision, adaptive, y0, rows, spp, sample_offset, d_rgb, accumulate,
-                       d_inv, d_counters, d_records, slot, stream, grid, block, 1);
+    return launch_kerr(d_rec, 1, p->precision, adaptive, inv, F.variant, spp, sample_offset, d_rgb,
+                       accumulate, d_inv, d_counters, d_records, slot, stream, grid, block, 1);
 }
 
 __global__ void zero_u64_n(unsigned long long* p, int n) {
@@ -129,6 +151,7 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, const np_
         int err = make_frame(&q, &out[i].F);
         if (err != NP_OK) return err;
         out[i].npix = (uint64_t)q.width * (uint64_t)q.height;
+        out[i].pix0 = 0;
         out[i].out_off = 3 * off;
         out[i].rec_off = off * (uint64_t)spp;
         off += out[i].npix;
@@ -155,16 +178,16 @@ int batch_kernel(const np_render_params* frames, int nframes, const BatchFrame* 
     const bool adaptive = frames[0].integrator == NP_INTEGRATOR_ADAPTIVE;
     const int variant = host0->F.variant;
     int per_cu = 0;
-    if (occupancy_kerr_batch(frames[0].precision, adaptive, variant, t.block_size, &per_cu) != NP_OK)
+    if (occupancy_kerr(frames[0].precision, adaptive, false, variant, t.block_size, &per_cu) != NP_OK)
         per_cu = 1;
     if (per_cu <= 0) per_cu = 1;
     uint64_t grid = (uint64_t)per_cu * (uint64_t)cus;
     const uint64_t need = (total_px + (uint64_t)t.block_size - 1) / (uint64_t)t.block_size;
     if (need < grid) grid = need;
     if (grid < 1) grid = 1;
-    return launch_kerr_batch(d_frames, nframes, frames[0].precision, adaptive, variant, spp,
-                             sample_offset, d_rgb, d_counters, d_records, d_work, stream, (int)grid,
-                             t.block_size, t.refill_chunk);
+    return launch_kerr(d_frames, nframes, frames[0].precision, adaptive, false, variant, spp,
+                       sample_offset, d_rgb, 0, nullptr, d_counters, d_records, d_work, stream, (int)grid,
+                       t.block_size, t.refill_chunk);
 }
 
 int batch_launch(const np_render_params* frames, int nframes, int spp, int sample_offset,

@@ -40,6 +40,10 @@ struct FrameConst {
     int32_t bg_mode, disk_on, disk_max_orders, check_inv, max_steps, emiss_m_units;
     int32_t variant;      // kernel variant (FP32C: waves-per-SIMD cap 3 or 4; 0 = none)
     int32_t crossing;     // 0 linear crossing radius (reference), 1 cubic Hermite
+    // FP32C: 1 = steps at the maximum step size beyond r_lazyf may skip
+    // their error estimate
+    float r_lazyf;
+    int32_t lazy;
 };
 
 // Returns 0 on success, NP_EINVAL when the parameters are outside what the

@@ -8,34 +8,35 @@
 
 namespace np {
 
-// Kerr renderer; precision is an np_precision value.  `chunk` is the number
-// of consecutive pixels a lane takes per refill.
-int launch_kerr(const FrameConst& F, int precision, bool adaptive, int y0, int rows, int spp,
-                int sample_offset, float* d_out, int accumulate, float* d_inv,
-                np_trace_counters* d_counters, np_ray_record* d_records,
-                unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk);
-int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu);
-
-// One frame of a multi-frame batch in device memory: its constants, pixel
-// count, and offsets of its colours (floats) and ray records in the packed
-// outputs.
+// One frame of a launch in device memory: its constants, the window of
+// pixels to trace (npix pixels from absolute pixel index pix0 = y0 * width),
+// and the offsets of the window's colours (floats) and ray records in the
+// packed outputs.
 struct BatchFrame {
     FrameConst F;
     uint64_t npix;
+    uint64_t pix0;
     uint64_t out_off;
     uint64_t rec_off;
 };
 
-// All frames of d_frames in one persistent launch; d_work holds one
-// cleared counter per frame.
-int launch_kerr_batch(const BatchFrame* d_frames, int nframes, int precision, bool adaptive,
-                      int variant, int spp, int sample_offset, float* d_out,
-                      np_trace_counters* d_counters, np_ray_record* d_records,
-                      unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk);
-int occupancy_kerr_batch(int precision, bool adaptive, int variant, int block, int* blocks_per_cu);
+// The Kerr renderer over nframes frame records in device memory, in one
+// persistent launch; d_work holds one cleared counter per frame.
+// precision is an np_precision value; `chunk` is the number of consecutive
+// pixels a lane takes per refill.
+int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
+                int variant, int spp, int sample_offset, float* d_out, int accumulate, float* d_inv,
+                np_trace_counters* d_counters, np_ray_record* d_records,
+                unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk);
+int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu);
 
 // A cleared work-queue counter on the current device (cleared by a kernel
 // on `stream`, so the call may be graph-captured).
 int take_work_slot(hipStream_t stream, unsigned long long** slot);
+
+// Writes one frame record and clears its work counter on `stream` (a
+// kernel, so the call may be graph-captured).
+int write_frame_record(const BatchFrame& rec, BatchFrame* d_dst, unsigned long long* d_counter,
+                       hipStream_t stream);
 
 }  // namespace np

@@ -31,7 +31,7 @@ class RenderParams(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
         ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
-        ("crossing", C.c_int32),
+        ("crossing", C.c_int32), ("estimate_every_step", C.c_int32),
     ]
 
 

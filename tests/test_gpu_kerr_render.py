@@ -17,7 +17,7 @@ from nullpath_amd._lib import ray_record_dtype
 def _run_gpu(p, spp, sample_offset=0):
     n = p.width * p.height
     out = torch.zeros(n * 3, dtype=torch.float32, device="cuda")
-    cnt = torch.zeros(10, dtype=torch.int64, device="cuda")
+    cnt = npa.new_counters()
     rec = torch.zeros(n * spp * 32, dtype=torch.uint8, device="cuda")
     print(f"[gpu] np_render_tile {p.width}x{p.height} spp={spp} integrator={p.integrator} "
           f"spin={p.spin}", flush=True)
@@ -136,6 +136,29 @@ def test_fp32c_matches_fp64_on_disk_workload(oracle, device):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fov", [55.0, 3.0])
+def test_far_field_estimate_skip(device, fov):
+    """Skipping the far-field error estimate (estimate_every_step 0, the
+    default) against estimating on every step (1): the same image bits and
+    per-ray class and crossing count; step counts differ only where a float
+    attempt rejected on rounding noise was re-run in FP64; fewer right-hand
+    sides."""
+    runs = {}
+    for every in (1, 0):
+        p = npa.kerr_disk_params(width=96, height=54, fov=fov, seed=1, precision=1, crossing=1,
+                                 estimate_every_step=every)
+        runs[every] = _run_gpu(p, 1)
+    (i1, c1, r1), (i0, c0, r0) = runs[1], runs[0]
+    assert np.array_equal(i1, i0)
+    assert np.array_equal(r1["cls"], r0["cls"]) and np.array_equal(r1["orders"], r0["orders"])
+    assert np.mean(r1["steps"] == r0["steps"]) > 0.99
+    for k in ("rays", "captured", "escaped", "disk_hits"):
+        assert c0[k] == c1[k], k
+    print(f"[gpu] far-field estimate skip: rhs {c0['rhs_evals']} vs {c1['rhs_evals']}", flush=True)
+    assert c0["rhs_evals"] < 0.85 * c1["rhs_evals"]
+
+
+@pytest.mark.gpu
 def test_fp32c_kernel_variants_agree(device):
     """The 168-VGPR (default), 128-VGPR and compiler-budget kernel variants
     compute the same thing: outputs and records are bitwise identical."""
@@ -169,7 +192,7 @@ def test_batch_equals_per_frame_renders(device, precision, tuning):
     frames = _batch_frames(precision)
     total = sum(f.width * f.height for f in frames)
     out = torch.zeros(3 * total, dtype=torch.float32, device="cuda")
-    cnt = torch.zeros(10, dtype=torch.int64, device="cuda")
+    cnt = npa.new_counters()
     print(f"[gpu] np_render_batch precision={precision} tuning={tuning}", flush=True)
     npa.render_batch(frames, out, spp=2, counters=cnt, tuning=tuning)
     torch.cuda.synchronize()

@@ -24,7 +24,7 @@ def _frames(sizes, spins, thetas, precision):
 def _render(frames, spp, noalloc):
     total = sum(f.width * f.height for f in frames)
     out = torch.full((3 * total,), -1.0, dtype=torch.float32, device="cuda")
-    cnt = torch.zeros(10, dtype=torch.int64, device="cuda")
+    cnt = npa.new_counters()
     rec = torch.zeros(total * spp * 32, dtype=torch.uint8, device="cuda")
     print(f"[gpu] np_render_batch{'_noalloc' if noalloc else ''} frames={len(frames)} "
           f"px={total} spp={spp}", flush=True)
