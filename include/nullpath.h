@@ -92,6 +92,37 @@ int np_render_accumulate(const np_render_params* p, int spp_done, int samples_pe
                          int passes, int tile_h, float* h_accum, float* inv_max);
 
 /*
+ * One progressive pass with the accumulation on the device: adds the colour
+ * sums of samples sample_offset .. sample_offset+spp-1 of every pixel to
+ * d_accum (device, width*height*3 floats, row-major RGB), in the order the
+ * reference's host loop adds each pass's tiles (render.cu:523-580, one
+ * launch per pass instead of per tile; the same per-pixel sample loop).
+ * d_inv_max3 as in np_render_tile.
+ */
+int np_render_pass(const np_render_params* p, int sample_offset, int spp, float* d_accum,
+                   float* d_inv_max3, void* stream);
+
+/*
+ * Progressive passes captured once in a HIP graph and replayed: the graph
+ * holds `passes` passes of `spp` samples each (as np_render_pass) into
+ * d_accum (and d_inv_max3, optional), and its own device-side sample
+ * counter: a replay (np_pass_graph_launch) adds samples next .. next +
+ * passes*spp - 1 and advances the counter, so successive replays continue
+ * the progressive render of render.cu:523-580 (the first replay starts at
+ * first_sample; np_pass_graph_seek moves the counter, stream-ordered).  The
+ * graph contains kernels only (no memset or copy nodes); it keeps its own
+ * frame record and work counter, so replays of one graph must not overlap
+ * each other.  Create on the device the replays run on; destroy after the
+ * last replay has finished.
+ */
+typedef struct np_pass_graph np_pass_graph;
+int np_pass_graph_create(const np_render_params* p, int spp, int passes, float* d_accum,
+                         float* d_inv_max3, int first_sample, np_pass_graph** graph);
+int np_pass_graph_launch(np_pass_graph* graph, void* stream);
+int np_pass_graph_seek(np_pass_graph* graph, int next_sample, void* stream);
+void np_pass_graph_destroy(np_pass_graph* graph);
+
+/*
  * Launch-shape choices of the batch renderer, picked by np_autotune_batch.
  * block_size: 64, 128 or 256 threads; waves_per_simd: FP32C kernel variant
  * (4 = 128-VGPR cap, 3 = 168-VGPR cap, 2 = compiler budget); refill_chunk: consecutive pixels
@@ -128,7 +159,7 @@ int np_render_batch(const np_render_params* frames, int nframes, int spp, int sa
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, kernels and results as np_render_batch; the
  * frame table and the per-frame work counters live in a fixed device array
- * of the library (about 48 KB per GPU, held until the process ends) instead
+ * of the library (about 49 KB per GPU, held until the process ends) instead
  * of a per-call allocation.  At most NP_NOALLOC_MAX_FRAMES frames
  * (NP_ECAPACITY beyond).  The caller still provides d_rgb and the optional
  * d_records / d_counters.  The call copies the frame table into that array

@@ -101,6 +101,51 @@ def new_counters(device="cuda"):
     return torch.zeros(len(COUNTER_FIELDS), dtype=torch.int64, device=device)
 
 
+def render_pass(p: RenderParams, accum, sample_offset=0, spp=1, inv=None, stream=None) -> None:
+    """np_render_pass: add one progressive pass of `spp` samples into the
+    float32 device tensor `accum` (width*height*3)."""
+    if accum.numel() < p.width * p.height * 3 or accum.dtype.itemsize != 4 or not accum.is_contiguous():
+        raise NullPathError("accum must be a contiguous float32 device tensor of width*height*3")
+    check(lib().np_render_pass(p, sample_offset, spp, accum.data_ptr(),
+                               None if inv is None else inv.data_ptr(), _stream_ptr(stream)),
+          "np_render_pass")
+
+
+class PassGraph:
+    """np_pass_graph: `passes` progressive passes of `spp` samples captured in
+    a HIP graph; launch() replays them into `accum`, continuing the sample
+    ids from the graph's device-side counter."""
+
+    def __init__(self, p: RenderParams, accum, spp=1, passes=1, first_sample=0, inv=None):
+        import ctypes as C
+        if accum.numel() < p.width * p.height * 3 or accum.dtype.itemsize != 4:
+            raise NullPathError("accum must be a float32 device tensor of width*height*3")
+        self._accum = accum
+        self._inv = inv
+        h = C.c_void_p()
+        check(lib().np_pass_graph_create(p, spp, passes, accum.data_ptr(),
+                                         None if inv is None else inv.data_ptr(), first_sample, C.byref(h)),
+              "np_pass_graph_create")
+        self._h = h
+
+    def launch(self, stream=None) -> None:
+        check(lib().np_pass_graph_launch(self._h, _stream_ptr(stream)), "np_pass_graph_launch")
+
+    def seek(self, next_sample: int, stream=None) -> None:
+        check(lib().np_pass_graph_seek(self._h, next_sample, _stream_ptr(stream)), "np_pass_graph_seek")
+
+    def close(self) -> None:
+        if self._h:
+            lib().np_pass_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def counters_dict(t) -> dict:
     """Decode a uint64 counter tensor (np_trace_counters, 10 elements) into a dict."""
     vals = t.cpu().tolist()

@@ -156,6 +156,16 @@ def _declare(L: C.CDLL) -> None:
     prp, pt = C.POINTER(RenderParams), C.POINTER(Tuning)
     L.np_render_batch.argtypes = [prp, i32, i32, i32, vp, vp, vp, pt, vp]
     L.np_render_batch.restype = i32
+    L.np_render_pass.argtypes = [prp, i32, i32, vp, vp, vp]
+    L.np_render_pass.restype = i32
+    L.np_pass_graph_create.argtypes = [prp, i32, i32, vp, vp, i32, C.POINTER(vp)]
+    L.np_pass_graph_create.restype = i32
+    L.np_pass_graph_launch.argtypes = [vp, vp]
+    L.np_pass_graph_launch.restype = i32
+    L.np_pass_graph_seek.argtypes = [vp, i32, vp]
+    L.np_pass_graph_seek.restype = i32
+    L.np_pass_graph_destroy.argtypes = [vp]
+    L.np_pass_graph_destroy.restype = None
     L.np_render_batch_noalloc.argtypes = [prp, i32, i32, i32, vp, vp, vp, pt, vp]
     L.np_render_batch_noalloc.restype = i32
     L.np_autotune_batch.argtypes = [prp, i32, i32, i32, pt, pt, C.POINTER(C.c_double), i32,

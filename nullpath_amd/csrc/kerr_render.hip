@@ -1196,7 +1196,7 @@ __device__ __forceinline__ const FrameConst& cold(const BatchFrame* B) {
 template <class S, bool ADAPTIVE, bool INV>
 __device__ __forceinline__ void
 kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int spp,
-          const int sample_offset, float* __restrict__ out_base, const int accumulate,
+          float* __restrict__ out_base, const int accumulate,
           float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
           np_ray_record* __restrict__ records_base, unsigned long long* __restrict__ work,
           const int chunk) {
@@ -1245,7 +1245,7 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
                         const int x = (int)(ap % (uint64_t)W);
                         const int yy = (int)(ap / (uint64_t)W);
                         float n_r, n_th, n_ph, cx, cy;
-                        camera_dir(Fc, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
+                        camera_dir(Fc, x, yy, B->sample0 + (uint32_t)sample, n_r, n_th, n_ph, cx, cy);
                         S::start(Fc, R, n_r, n_th, n_ph, cx, cy);
                         has = true;
                     } else {
@@ -1310,7 +1310,7 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
                         const int x = (int)(ap % (uint64_t)W);
                         const int yy = (int)(ap / (uint64_t)W);
                         float n_r, n_th, n_ph, cx, cy;
-                        camera_dir(Fc, x, yy, (uint32_t)(sample_offset + sample), n_r, n_th, n_ph, cx, cy);
+                        camera_dir(Fc, x, yy, B->sample0 + (uint32_t)sample, n_r, n_th, n_ph, cx, cy);
                         S::start(Fc, R, n_r, n_th, n_ph, cx, cy);
                     } else {
                         float* o = out_base + B->out_off + 3 * pix;
@@ -1367,11 +1367,10 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
 
 #define NP_KERNEL_ARGS                                                                          \
     const BatchFrame* __restrict__ frames, const int nframes, const int spp,                    \
-        const int sample_offset, float* __restrict__ out, const int accumulate,                 \
-        float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,                 \
-        np_ray_record* __restrict__ records, unsigned long long* __restrict__ work, const int chunk
-#define NP_KERNEL_PASS frames, nframes, spp, sample_offset, out, accumulate, inv_max3, counters, \
-        records, work, chunk
+        float* __restrict__ out, const int accumulate, float* __restrict__ inv_max3,            \
+        np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records,          \
+        unsigned long long* __restrict__ work, const int chunk
+#define NP_KERNEL_PASS frames, nframes, spp, out, accumulate, inv_max3, counters, records, work, chunk
 
 // Kernel entry points.  Variant 0 leaves the register budget to the
 // compiler; the FP32C variants W = 3 and W = 4 ask for W waves per SIMD,
@@ -1424,15 +1423,34 @@ __global__ void init_frame_record(const BatchFrame rec, BatchFrame* __restrict__
     }
 }
 
+// The same for a pass whose first sample id lives in device memory: the
+// record takes *next_sample, which then advances by spp (so a captured
+// sequence of passes continues where the previous replay stopped).
+__global__ void init_pass_record(const BatchFrame rec, BatchFrame* __restrict__ dst,
+                                 unsigned long long* __restrict__ counter,
+                                 uint32_t* __restrict__ next_sample, const uint32_t spp) {
+    if (threadIdx.x == 0) {
+        const uint32_t s0 = *next_sample;
+        *dst = rec;
+        dst->sample0 = s0;
+        *counter = 0ull;
+        *next_sample = s0 + spp;
+    }
+}
+
+__global__ void set_u32(uint32_t* __restrict__ p, const uint32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
 }  // namespace
 
 int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
-                int variant, int spp, int sample_offset, float* d_out, int accumulate, float* d_inv,
+                int variant, int spp, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
     const void* fn = select_kernel(precision, adaptive, inv, variant);
-    void* args[] = {&d_frames, &nframes, &spp, &sample_offset, &d_out, &accumulate, &d_inv,
-                    &d_counters, &d_records, &d_work, &chunk};
+    void* args[] = {&d_frames, &nframes, &spp, &d_out, &accumulate, &d_inv, &d_counters, &d_records,
+                    &d_work, &chunk};
     const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
     return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
 }
@@ -1446,6 +1464,18 @@ int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int bloc
 int write_frame_record(const BatchFrame& rec, BatchFrame* d_dst, unsigned long long* d_counter,
                        hipStream_t stream) {
     hipLaunchKernelGGL(init_frame_record, dim3(1), dim3(64), 0, stream, rec, d_dst, d_counter);
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int write_pass_record(const BatchFrame& rec, BatchFrame* d_dst, unsigned long long* d_counter,
+                      uint32_t* d_next_sample, int spp, hipStream_t stream) {
+    hipLaunchKernelGGL(init_pass_record, dim3(1), dim3(64), 0, stream, rec, d_dst, d_counter, d_next_sample,
+                       (uint32_t)spp);
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int set_device_u32(uint32_t* d_p, uint32_t v, hipStream_t stream) {
+    hipLaunchKernelGGL(set_u32, dim3(1), dim3(64), 0, stream, d_p, v);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

@@ -115,6 +115,8 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     rec.pix0 = (uint64_t)y0 * (uint64_t)p->width;
     rec.out_off = 0;
     rec.rec_off = 0;
+    rec.sample0 = (uint32_t)sample_offset;
+    rec.reserved = 0;
     unsigned long long* slot = nullptr;
     BatchFrame* d_rec = nullptr;
     err = take_frame_slot(d, stream, rec, &d_rec, &slot);
@@ -125,8 +127,8 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     int per_cu = 0;
     if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu) != NP_OK) per_cu = 1;
     const int grid = persistent_grid(d, per_cu, block, npix);
-    return launch_kerr(d_rec, 1, p->precision, adaptive, inv, F.variant, spp, sample_offset, d_rgb,
-                       accumulate, d_inv, d_counters, d_records, slot, stream, grid, block, 1);
+    return launch_kerr(d_rec, 1, p->precision, adaptive, inv, F.variant, spp, d_rgb, accumulate, d_inv,
+                       d_counters, d_records, slot, stream, grid, block, 1);
 }
 
 __global__ void zero_u64_n(unsigned long long* p, int n) {
@@ -135,9 +137,9 @@ __global__ void zero_u64_n(unsigned long long* p, int n) {
 
 // The frames of a batch, checked and laid out: one precision and one
 // integrator for all frames, no invariant monitor.
-int batch_frames(const np_render_params* frames, int nframes, int spp, const np_tuning* t,
-                 BatchFrame* out, uint64_t* total_px) {
-    if (!frames || nframes <= 0 || spp < 0) return NP_EINVAL;
+int batch_frames(const np_render_params* frames, int nframes, int spp, int sample_offset,
+                 const np_tuning* t, BatchFrame* out, uint64_t* total_px) {
+    if (!frames || nframes <= 0 || spp < 0 || sample_offset < 0) return NP_EINVAL;
     uint64_t off = 0;
     for (int i = 0; i < nframes; ++i) {
         np_render_params q = frames[i];
@@ -152,6 +154,8 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, const np_
         if (err != NP_OK) return err;
         out[i].npix = (uint64_t)q.width * (uint64_t)q.height;
         out[i].pix0 = 0;
+        out[i].sample0 = (uint32_t)sample_offset;
+        out[i].reserved = 0;
         out[i].out_off = 3 * off;
         out[i].rec_off = off * (uint64_t)spp;
         off += out[i].npix;
@@ -171,7 +175,7 @@ int check_tuning(const np_tuning* tuning, np_tuning* t) {
 // already in device memory.
 int batch_kernel(const np_render_params* frames, int nframes, const BatchFrame* host0,
                  const BatchFrame* d_frames, unsigned long long* d_work, int cus, int spp,
-                 int sample_offset, uint64_t total_px, float* d_rgb, np_ray_record* d_records,
+                 uint64_t total_px, float* d_rgb, np_ray_record* d_records,
                  np_trace_counters* d_counters, const np_tuning& t, hipStream_t stream) {
     hipLaunchKernelGGL(zero_u64_n, dim3(1), dim3(256), 0, stream, d_work, nframes);
     if (hipGetLastError() != hipSuccess) return NP_ELAUNCH;
@@ -185,9 +189,9 @@ int batch_kernel(const np_render_params* frames, int nframes, const BatchFrame* 
     const uint64_t need = (total_px + (uint64_t)t.block_size - 1) / (uint64_t)t.block_size;
     if (need < grid) grid = need;
     if (grid < 1) grid = 1;
-    return launch_kerr(d_frames, nframes, frames[0].precision, adaptive, false, variant, spp,
-                       sample_offset, d_rgb, 0, nullptr, d_counters, d_records, d_work, stream, (int)grid,
-                       t.block_size, t.refill_chunk);
+    return launch_kerr(d_frames, nframes, frames[0].precision, adaptive, false, variant, spp, d_rgb, 0,
+                       nullptr, d_counters, d_records, d_work, stream, (int)grid, t.block_size,
+                       t.refill_chunk);
 }
 
 int batch_launch(const np_render_params* frames, int nframes, int spp, int sample_offset,
@@ -199,7 +203,7 @@ int batch_launch(const np_render_params* frames, int nframes, int spp, int sampl
     if (!frames || nframes <= 0) return NP_EINVAL;
     std::vector<BatchFrame> host((size_t)nframes);
     uint64_t total_px = 0;
-    int err = batch_frames(frames, nframes, spp, &t, host.data(), &total_px);
+    int err = batch_frames(frames, nframes, spp, sample_offset, &t, host.data(), &total_px);
     if (err != NP_OK) return err;
     DeviceState* d = device_state(&err);
     if (!d) return err;
@@ -220,8 +224,8 @@ int batch_launch(const np_render_params* frames, int nframes, int spp, int sampl
         hipStreamSynchronize(stream) != hipSuccess)   // host table released on return
         err = NP_ELAUNCH;
     if (err == NP_OK)
-        err = batch_kernel(frames, nframes, host.data(), d_frames, d_work, d->cus, spp,
-                           sample_offset, total_px, d_rgb, d_records, d_counters, t, stream);
+        err = batch_kernel(frames, nframes, host.data(), d_frames, d_work, d->cus, spp, total_px, d_rgb,
+                           d_records, d_counters, t, stream);
     if (stream_ordered) {
         if (hipFreeAsync(buf, stream) != hipSuccess && err == NP_OK) err = NP_ELAUNCH;
     } else {
@@ -294,7 +298,7 @@ int batch_launch_noalloc(const np_render_params* frames, int nframes, int spp, i
     if (!s) return err;
     std::lock_guard<std::mutex> lk(s->mu);
     uint64_t total_px = 0;
-    err = batch_frames(frames, nframes, spp, &t, s->stage, &total_px);
+    err = batch_frames(frames, nframes, spp, sample_offset, &t, s->stage, &total_px);
     if (err != NP_OK) return err;
     if (total_px == 0 || spp == 0) return NP_OK;
     BatchFrame* d_frames = reinterpret_cast<BatchFrame*>(s->ptr);
@@ -304,8 +308,128 @@ int batch_launch_noalloc(const np_render_params* frames, int nframes, int spp, i
                        stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return NP_ELAUNCH;
-    return batch_kernel(frames, nframes, s->stage, d_frames, d_work, s->cus, spp, sample_offset,
-                        total_px, d_rgb, d_records, d_counters, t, stream);
+    return batch_kernel(frames, nframes, s->stage, d_frames, d_work, s->cus, spp, total_px, d_rgb,
+                        d_records, d_counters, t, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Progressive passes captured in a HIP graph.  The graph owns its frame
+// record, work counter and next-sample counter; each captured pass is two
+// kernels (record + counter set-up, render with accumulation), so replays
+// contain no memset or copy node and continue the sample ids where the
+// previous replay stopped.
+// ---------------------------------------------------------------------------
+struct PassGraph {
+    int device = 0;
+    int spp = 0, passes = 0;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* mem = nullptr;
+    BatchFrame* d_rec = nullptr;
+    unsigned long long* d_work = nullptr;
+    uint32_t* d_next = nullptr;
+};
+
+void destroy_pass_graph(PassGraph* g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->capture) (void)hipStreamDestroy(g->capture);
+    if (g->mem) (void)hipFree(g->mem);
+    delete g;
+}
+
+int create_pass_graph(const np_render_params* p, int spp, int passes, float* d_accum, float* d_inv,
+                      int first_sample, PassGraph** out) {
+    if (!out) return NP_EINVAL;
+    *out = nullptr;
+    if (!p || !d_accum || spp <= 0 || passes <= 0 || first_sample < 0) return NP_EINVAL;
+    if (p->precision != NP_PREC_FP64 && p->precision != NP_PREC_FP32C) return NP_EINVAL;
+    FrameConst F;
+    int err = make_frame(p, &F);
+    if (err != NP_OK) return err;
+    DeviceState* d = device_state(&err);
+    if (!d) return err;
+    PassGraph* g = new PassGraph();
+    g->spp = spp;
+    g->passes = passes;
+    (void)hipGetDevice(&g->device);
+    const size_t rec_bytes = (sizeof(BatchFrame) + 255) & ~(size_t)255;
+    if (hipMalloc(&g->mem, rec_bytes + 256) != hipSuccess) {
+        g->mem = nullptr;
+        destroy_pass_graph(g);
+        return NP_EALLOC;
+    }
+    g->d_rec = static_cast<BatchFrame*>(g->mem);
+    g->d_work = reinterpret_cast<unsigned long long*>(static_cast<char*>(g->mem) + rec_bytes);
+    g->d_next = reinterpret_cast<uint32_t*>(static_cast<char*>(g->mem) + rec_bytes + 8);
+    BatchFrame rec;
+    rec.F = F;
+    rec.npix = (uint64_t)p->width * (uint64_t)p->height;
+    rec.pix0 = 0;
+    rec.out_off = 0;
+    rec.rec_off = 0;
+    rec.sample0 = 0;
+    rec.reserved = 0;
+    const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
+    const bool inv = p->check_inv != 0 && d_inv != nullptr;
+    const int block = 256;
+    int per_cu = 0;
+    if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu) != NP_OK) per_cu = 1;
+    const int grid = persistent_grid(d, per_cu, block, rec.npix);
+    err = NP_OK;
+    if (hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking) != hipSuccess) {
+        g->capture = nullptr;
+        err = NP_ELAUNCH;
+    }
+    if (err == NP_OK) err = set_device_u32(g->d_next, (uint32_t)first_sample, g->capture);
+    if (err == NP_OK && hipStreamSynchronize(g->capture) != hipSuccess) err = NP_ELAUNCH;
+    if (err == NP_OK && hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        err = NP_ELAUNCH;
+    if (err == NP_OK) {
+        int cerr = NP_OK;
+        for (int k = 0; k < passes && cerr == NP_OK; ++k) {
+            cerr = write_pass_record(rec, g->d_rec, g->d_work, g->d_next, spp, g->capture);
+            if (cerr == NP_OK)
+                cerr = launch_kerr(g->d_rec, 1, p->precision, adaptive, inv, F.variant, spp, d_accum, 1,
+                                   inv ? d_inv : nullptr, nullptr, nullptr, g->d_work, g->capture, grid,
+                                   block, 1);
+        }
+        // end the capture in every case, so the stream leaves capture mode
+        if (hipStreamEndCapture(g->capture, &g->graph) != hipSuccess) {
+            g->graph = nullptr;
+            cerr = NP_ELAUNCH;
+        }
+        err = cerr;
+    }
+    if (err == NP_OK) {
+        // every node must be a kernel (no memset or copy nodes in replays)
+        size_t nodes = 0;
+        if (hipGraphGetNodes(g->graph, nullptr, &nodes) != hipSuccess) {
+            err = NP_ELAUNCH;
+        } else {
+            std::vector<hipGraphNode_t> list(nodes);
+            if (nodes && hipGraphGetNodes(g->graph, list.data(), &nodes) != hipSuccess) err = NP_ELAUNCH;
+            for (size_t i = 0; i < nodes && err == NP_OK; ++i) {
+                hipGraphNodeType type;
+                if (hipGraphNodeGetType(list[i], &type) != hipSuccess || type != hipGraphNodeTypeKernel)
+                    err = NP_ELAUNCH;
+            }
+            if (err == NP_OK && nodes != (size_t)(2 * passes)) err = NP_ELAUNCH;
+        }
+    }
+    if (err == NP_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        g->exec = nullptr;
+        err = NP_ELAUNCH;
+    }
+    if (err != NP_OK) {
+        (void)hipGetLastError();
+        destroy_pass_graph(g);
+        return err;
+    }
+    *out = g;
+    return NP_OK;
 }
 
 // Autotuner: times every candidate tuning on a subsampled copy of the batch
@@ -440,6 +564,37 @@ int np_render_tile(const np_render_params* p, int y0, int rows, int spp, int sam
                    np_ray_record* d_records, void* stream) {
     return np::render_launch(p, y0, rows, spp, sample_offset, d_rgb, 0, d_inv_max3, d_counters,
                              d_records, (hipStream_t)stream);
+}
+
+int np_render_pass(const np_render_params* p, int sample_offset, int spp, float* d_accum,
+                   float* d_inv_max3, void* stream) {
+    if (!p || !d_accum || spp <= 0) return NP_EINVAL;
+    return np::render_launch(p, 0, p->height, spp, sample_offset, d_accum, 1, d_inv_max3, nullptr,
+                             nullptr, (hipStream_t)stream);
+}
+
+int np_pass_graph_create(const np_render_params* p, int spp, int passes, float* d_accum,
+                         float* d_inv_max3, int first_sample, np_pass_graph** graph) {
+    np::PassGraph* g = nullptr;
+    const int err = np::create_pass_graph(p, spp, passes, d_accum, d_inv_max3, first_sample, &g);
+    if (graph) *graph = reinterpret_cast<np_pass_graph*>(g);
+    return err;
+}
+
+int np_pass_graph_launch(np_pass_graph* graph, void* stream) {
+    np::PassGraph* g = reinterpret_cast<np::PassGraph*>(graph);
+    if (!g || !g->exec) return NP_EINVAL;
+    return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+int np_pass_graph_seek(np_pass_graph* graph, int next_sample, void* stream) {
+    np::PassGraph* g = reinterpret_cast<np::PassGraph*>(graph);
+    if (!g || next_sample < 0) return NP_EINVAL;
+    return np::set_device_u32(g->d_next, (uint32_t)next_sample, (hipStream_t)stream);
+}
+
+void np_pass_graph_destroy(np_pass_graph* graph) {
+    np::destroy_pass_graph(reinterpret_cast<np::PassGraph*>(graph));
 }
 
 int np_render_accumulate(const np_render_params* p, int spp_done, int samples_per_pass, int passes,

@@ -10,14 +10,16 @@ namespace np {
 
 // One frame of a launch in device memory: its constants, the window of
 // pixels to trace (npix pixels from absolute pixel index pix0 = y0 * width),
-// and the offsets of the window's colours (floats) and ray records in the
-// packed outputs.
+// the offsets of the window's colours (floats) and ray records in the packed
+// outputs, and the id of the first sample traced per pixel.
 struct BatchFrame {
     FrameConst F;
     uint64_t npix;
     uint64_t pix0;
     uint64_t out_off;
     uint64_t rec_off;
+    uint32_t sample0;
+    uint32_t reserved;
 };
 
 // The Kerr renderer over nframes frame records in device memory, in one
@@ -25,7 +27,7 @@ struct BatchFrame {
 // precision is an np_precision value; `chunk` is the number of consecutive
 // pixels a lane takes per refill.
 int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
-                int variant, int spp, int sample_offset, float* d_out, int accumulate, float* d_inv,
+                int variant, int spp, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk);
 int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu);
@@ -38,5 +40,11 @@ int take_work_slot(hipStream_t stream, unsigned long long** slot);
 // kernel, so the call may be graph-captured).
 int write_frame_record(const BatchFrame& rec, BatchFrame* d_dst, unsigned long long* d_counter,
                        hipStream_t stream);
+
+// The same for a progressive pass whose first sample id is read from
+// *d_next_sample, which then advances by spp (all on `stream`, kernels only).
+int write_pass_record(const BatchFrame& rec, BatchFrame* d_dst, unsigned long long* d_counter,
+                      uint32_t* d_next_sample, int spp, hipStream_t stream);
+int set_device_u32(uint32_t* d_p, uint32_t v, hipStream_t stream);
 
 }  // namespace np

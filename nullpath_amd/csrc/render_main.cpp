@@ -1,327 +1,520 @@
-// render_main.cpp -- the `render` command line, a drop-in for the reference
-// renderer's CLI (render.cu:391-647): same flags, defaults, parsing quirks
-// (unknown flags and flags missing their value are ignored; --mode, --tonemap
-// and --bg are case-folded; -h is the height), validation messages, exit
-// codes, progressive passes with PPM checkpoints, and frame animation.
-// Tracing runs through libnullpath.so (np_render_accumulate).
+// render_main.cpp -- the `render` command line: a drop-in for the reference
+// renderer's executable (render.cu:391-647).  Kept from the reference, as its
+// interface: every flag's spelling, default and value type; unknown flags and
+// flags missing their value are ignored; --mode, --tonemap and --bg are
+// case-folded; -h is the height; the validation messages, their order and
+// exit code 1; the "[checkpoint] ..." and "Wrote ..." lines; the PPM bytes
+// (truncating 8-bit tone curve of render.cu:583-607); --frames/--turns/--outpat
+// animation with its one-%d pattern rule.
+//
+// The program is built around libnullpath.so's progressive passes: the
+// accumulation buffer stays in device memory for the whole frame, the passes
+// between two checkpoints are one replay of a captured HIP graph
+// (np_pass_graph_*), and only checkpoints copy the image back.
 //
 // Additive flags (defaults reproduce the reference):
-//   --precision fp64|fp32c   --rtol X --atol X    --max-steps N   --seed N
-//   --disk-rin isco|X        --disk-rout X        (X in units of M)
-//   --emissivity-index q     (emissivity (r/M)^-q; default r^-2 absolute)
-//   --raw-out file.f32       (linear float32 RGB, accum / spp, row-major)
-//   --fp64-radius X          (fp32c: FP64 steps inside X M; default max(30, 1.5 disk-rout))
-//   --waves-per-simd 2|3|4   (fp32c kernel variant: 3 = capped at 168 VGPRs, the default;
-//                             4 = capped at 128 VGPRs; 2 = the compiler's register budget)
+//   --precision fp64|fp32c    --rtol X --atol X    --max-steps N    --seed N
+//   --disk-rin isco|X         --disk-rout X        (X in units of M)
+//   --emissivity-index q      (emissivity (r/M)^-q; default r^-2 absolute)
+//   --raw-out FILE            (linear float32 RGB at the end, accum / spp, row-major)
+//   --state FILE              (at every checkpoint: FILE = float32 accumulation sums,
+//                              FILE.json = size, samples done, parameter signature)
+//   --resume FILE             (continue a single-frame render from a --state file)
+//   --fp64-radius X           (fp32c: FP64 steps inside X M; default max(30, 1.5 disk-rout))
+//   --waves-per-simd 2|3|4    (fp32c kernel variant)
 //   --crossing linear|hermite (disk-crossing radius; default linear as the reference)
+//   --estimate-every-step     (fp32c: no far-field skip of the error estimate)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <functional>
+#include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "nullpath.h"
 
 namespace {
 
-struct Options {
-    int width = 1280;
-    int height = 720;
-    float fov_y_deg = 50.0f;
-    float theta_deg = 90.0f;
-    float mass = 10.0f;
-    float r_cam = 50.0f;
-    const char* out_path = "render.ppm";
-    int samples = 1;
-    int tile_h = 128;
-    int frames = 1;
-    float turns = 1.0f;
-    float phi0_deg = 0.0f;
-    const char* out_pat = "frame_%05d.ppm";
-    float spin = 0.0f;
-    int spp_total = 0;
-    int checkpoint_every = 0;
-    int check_invariants = 0;
-    const char* mode = "balanced";
-    float exposure_ev = 0.0f;
-    float gamma = 2.2f;
-    const char* tonemap = "none";
-    const char* bg = "gradient";
-    float bg_sigma_deg = 1.0f;
-    float bg_brightness = 3.0f;
-    int disk_on = 1;
-    float disk_gain = 1.0f;
-    int disk_max_orders = 1;
-    // extensions
-    const char* precision = "fp64";
-    double rtol = -1.0, atol = -1.0;
-    const char* disk_rin = nullptr;
-    double disk_rout_M = -1.0;
-    float emiss_q = -1.0f;
-    const char* raw_out = nullptr;
-    unsigned seed = 0;
-    int max_steps = 10000;
-    double fp64_radius_M = 0.0;
-    int waves_per_simd = 0;
-    const char* crossing = "linear";
+constexpr float kPi = 3.14159265358979323846f;
+
+// ---------------------------------------------------------------------------
+// Settings, grouped by what they control.
+// ---------------------------------------------------------------------------
+struct Settings {
+    struct {
+        int width = 1280, height = 720, tile_rows = 128;
+    } image;
+    struct {
+        float fov_deg = 50.0f, theta_deg = 90.0f, phi0_deg = 0.0f, r_cam_rs = 50.0f;
+        float mass = 10.0f, spin = 0.0f;
+    } camera;
+    struct {
+        int per_pass = 1, total = 0, checkpoint_every = 0;
+    } samples;
+    struct {
+        int frames = 1;
+        float turns = 1.0f;
+        std::string pattern = "frame_%05d.ppm";
+        std::string single = "render.ppm";
+    } output;
+    struct {
+        std::string mode = "balanced", tonemap = "none", background = "gradient";
+        float exposure_ev = 0.0f, gamma = 2.2f, bg_sigma_deg = 1.0f, bg_brightness = 3.0f;
+        float disk_gain = 1.0f;
+        bool disk = true, invariants = false;
+        int disk_orders = 1;
+    } look;
+    struct {   // additive extensions
+        std::string precision = "fp64", crossing = "linear", disk_rin, raw_out, state, resume;
+        double rtol = -1.0, atol = -1.0, disk_rout_m = -1.0, fp64_radius_m = 0.0;
+        float emissivity = -1.0f;
+        unsigned seed = 0;
+        int max_steps = 10000, waves_per_simd = 0;
+        bool estimate_every_step = false;
+    } ext;
 };
 
-void parse_args(int argc, char** argv, Options& p) {
+// ---------------------------------------------------------------------------
+// Flag table: each flag names the setting it writes; one loop applies them.
+// ---------------------------------------------------------------------------
+using Apply = std::function<void(Settings&, const char*)>;
+
+struct Flag {
+    std::vector<std::string> names;
+    bool takes_value;
+    Apply apply;
+};
+
+int as_int(const char* v) { return (int)std::strtol(v, nullptr, 10); }
+float as_float(const char* v) { return std::strtof(v, nullptr); }
+double as_double(const char* v) { return std::strtod(v, nullptr); }
+
+std::vector<Flag> flag_table() {
+    using S = Settings;
+    std::vector<Flag> t;
+    auto val = [&t](std::vector<std::string> names, Apply a) { t.push_back({std::move(names), true, std::move(a)}); };
+    auto sw = [&t](std::vector<std::string> names, Apply a) { t.push_back({std::move(names), false, std::move(a)}); };
+    val({"--w", "-w"}, [](S& s, const char* v) { s.image.width = as_int(v); });
+    val({"--h", "-h"}, [](S& s, const char* v) { s.image.height = as_int(v); });
+    val({"--tile"}, [](S& s, const char* v) { s.image.tile_rows = as_int(v); });
+    val({"--fov"}, [](S& s, const char* v) { s.camera.fov_deg = as_float(v); });
+    val({"--theta"}, [](S& s, const char* v) { s.camera.theta_deg = as_float(v); });
+    val({"--phi0"}, [](S& s, const char* v) { s.camera.phi0_deg = as_float(v); });
+    val({"--rcam"}, [](S& s, const char* v) { s.camera.r_cam_rs = as_float(v); });
+    val({"--mass"}, [](S& s, const char* v) { s.camera.mass = as_float(v); });
+    val({"--spin"}, [](S& s, const char* v) { s.camera.spin = as_float(v); });
+    val({"--samples", "-s"}, [](S& s, const char* v) { s.samples.per_pass = as_int(v); });
+    val({"--spp-total"}, [](S& s, const char* v) { s.samples.total = as_int(v); });
+    val({"--checkpoint-every"}, [](S& s, const char* v) { s.samples.checkpoint_every = as_int(v); });
+    val({"--frames"}, [](S& s, const char* v) { s.output.frames = as_int(v); });
+    val({"--turns"}, [](S& s, const char* v) { s.output.turns = as_float(v); });
+    val({"--outpat"}, [](S& s, const char* v) { s.output.pattern = v; });
+    val({"--out"}, [](S& s, const char* v) { s.output.single = v; });
+    val({"--mode"}, [](S& s, const char* v) { s.look.mode = v; });
+    val({"--tonemap"}, [](S& s, const char* v) { s.look.tonemap = v; });
+    val({"--bg"}, [](S& s, const char* v) { s.look.background = v; });
+    val({"--exposure"}, [](S& s, const char* v) { s.look.exposure_ev = as_float(v); });
+    val({"--gamma"}, [](S& s, const char* v) { s.look.gamma = as_float(v); });
+    val({"--bg-sigma"}, [](S& s, const char* v) { s.look.bg_sigma_deg = as_float(v); });
+    val({"--bg-bright"}, [](S& s, const char* v) { s.look.bg_brightness = as_float(v); });
+    val({"--disk-gain"}, [](S& s, const char* v) { s.look.disk_gain = as_float(v); });
+    val({"--disk-max-orders"}, [](S& s, const char* v) { s.look.disk_orders = as_int(v); });
+    sw({"--no-disk"}, [](S& s, const char*) { s.look.disk = false; });
+    sw({"--check-invariants"}, [](S& s, const char*) { s.look.invariants = true; });
+    // extensions
+    val({"--precision"}, [](S& s, const char* v) { s.ext.precision = v; });
+    val({"--crossing"}, [](S& s, const char* v) { s.ext.crossing = v; });
+    val({"--rtol"}, [](S& s, const char* v) { s.ext.rtol = as_double(v); });
+    val({"--atol"}, [](S& s, const char* v) { s.ext.atol = as_double(v); });
+    val({"--disk-rin"}, [](S& s, const char* v) { s.ext.disk_rin = v; });
+    val({"--disk-rout"}, [](S& s, const char* v) { s.ext.disk_rout_m = as_double(v); });
+    val({"--emissivity-index"}, [](S& s, const char* v) { s.ext.emissivity = as_float(v); });
+    val({"--raw-out"}, [](S& s, const char* v) { s.ext.raw_out = v; });
+    val({"--state"}, [](S& s, const char* v) { s.ext.state = v; });
+    val({"--resume"}, [](S& s, const char* v) { s.ext.resume = v; });
+    val({"--seed"}, [](S& s, const char* v) { s.ext.seed = (unsigned)std::strtoul(v, nullptr, 10); });
+    val({"--max-steps"}, [](S& s, const char* v) { s.ext.max_steps = as_int(v); });
+    val({"--fp64-radius"}, [](S& s, const char* v) { s.ext.fp64_radius_m = as_double(v); });
+    val({"--waves-per-simd"}, [](S& s, const char* v) { s.ext.waves_per_simd = as_int(v); });
+    sw({"--estimate-every-step"}, [](S& s, const char*) { s.ext.estimate_every_step = true; });
+    return t;
+}
+
+Settings parse(int argc, char** argv) {
+    const std::vector<Flag> table = flag_table();
+    std::unordered_map<std::string, const Flag*> by_name;
+    for (const Flag& f : table)
+        for (const std::string& n : f.names) by_name[n] = &f;
+    Settings s;
     for (int i = 1; i < argc; ++i) {
-        std::string a(argv[i]);
-        auto nextf = [&](float& ref) { if (i + 1 < argc) ref = strtof(argv[++i], nullptr); };
-        auto nextd = [&](double& ref) { if (i + 1 < argc) ref = strtod(argv[++i], nullptr); };
-        auto nexti = [&](int& ref) { if (i + 1 < argc) ref = atoi(argv[++i]); };
-        auto nexts = [&](const char*& ref) { if (i + 1 < argc) ref = argv[++i]; };
-        if (a == "--w" || a == "-w") nexti(p.width);
-        else if (a == "--h" || a == "-h") nexti(p.height);
-        else if (a == "--fov") nextf(p.fov_y_deg);
-        else if (a == "--theta") nextf(p.theta_deg);
-        else if (a == "--mass") nextf(p.mass);
-        else if (a == "--rcam") nextf(p.r_cam);
-        else if (a == "--out") nexts(p.out_path);
-        else if (a == "--samples" || a == "-s") nexti(p.samples);
-        else if (a == "--spp-total") nexti(p.spp_total);
-        else if (a == "--checkpoint-every") nexti(p.checkpoint_every);
-        else if (a == "--spin") nextf(p.spin);
-        else if (a == "--check-invariants") p.check_invariants = 1;
-        else if (a == "--tile") nexti(p.tile_h);
-        else if (a == "--frames") nexti(p.frames);
-        else if (a == "--turns") nextf(p.turns);
-        else if (a == "--phi0") nextf(p.phi0_deg);
-        else if (a == "--outpat") nexts(p.out_pat);
-        else if (a == "--mode" && i + 1 < argc) p.mode = argv[++i];
-        else if (a == "--exposure") nextf(p.exposure_ev);
-        else if (a == "--gamma") nextf(p.gamma);
-        else if (a == "--tonemap" && i + 1 < argc) p.tonemap = argv[++i];
-        else if (a == "--bg" && i + 1 < argc) p.bg = argv[++i];
-        else if (a == "--bg-sigma") nextf(p.bg_sigma_deg);
-        else if (a == "--bg-bright") nextf(p.bg_brightness);
-        else if (a == "--no-disk") p.disk_on = 0;
-        else if (a == "--disk-gain") nextf(p.disk_gain);
-        else if (a == "--disk-max-orders") nexti(p.disk_max_orders);
-        // extensions
-        else if (a == "--precision" && i + 1 < argc) p.precision = argv[++i];
-        else if (a == "--rtol") nextd(p.rtol);
-        else if (a == "--atol") nextd(p.atol);
-        else if (a == "--disk-rin") nexts(p.disk_rin);
-        else if (a == "--disk-rout") nextd(p.disk_rout_M);
-        else if (a == "--emissivity-index") nextf(p.emiss_q);
-        else if (a == "--raw-out") nexts(p.raw_out);
-        else if (a == "--seed" && i + 1 < argc) p.seed = (unsigned)strtoul(argv[++i], nullptr, 10);
-        else if (a == "--max-steps") nexti(p.max_steps);
-        else if (a == "--fp64-radius") nextd(p.fp64_radius_M);
-        else if (a == "--waves-per-simd") nexti(p.waves_per_simd);
-        else if (a == "--crossing" && i + 1 < argc) p.crossing = argv[++i];
+        const auto hit = by_name.find(argv[i]);
+        if (hit == by_name.end()) continue;           // unknown: ignored
+        const Flag& f = *hit->second;
+        if (!f.takes_value) {
+            f.apply(s, nullptr);
+        } else if (i + 1 < argc) {
+            f.apply(s, argv[++i]);                     // the next word, whatever it looks like
+        }                                              // value missing: ignored
     }
+    return s;
 }
 
-bool validate_out_pattern(const char* pat) {
-    if (!pat) return false;
-    int count = 0;
-    for (const char* q = pat; *q; ++q) {
-        if (*q != '%') continue;
-        ++q;
-        if (*q == '\0') return false;
-        if (*q == '%') continue;
-        if (*q == '-') ++q;
-        while (*q >= '0' && *q <= '9') ++q;
-        if (*q != 'd') return false;
-        count++;
-    }
-    return count == 1;
+std::string folded(std::string v) {
+    std::transform(v.begin(), v.end(), v.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+    return v;
 }
 
-bool write_ppm(const char* path, int w, int h, const std::vector<unsigned char>& rgb) {
-    FILE* f = std::fopen(path, "wb");
+// Validation rules in the order the reference applies them; the first one
+// that fails ends the program with its message and status 1.
+struct Rule {
+    std::function<bool(const Settings&)> ok;
+    const char* message;
+};
+
+const std::vector<Rule>& rules() {
+    static const std::vector<Rule> r = {
+        {[](const Settings& s) { return s.image.width > 0 && s.image.height > 0; }, "Invalid image dimensions."},
+        {[](const Settings& s) { return s.image.tile_rows > 0; }, "Invalid tile height."},
+        {[](const Settings& s) { return s.samples.per_pass > 0; }, "Invalid samples per pixel."},
+        {[](const Settings& s) { return s.samples.total >= 0; }, "Invalid total spp."},
+        {[](const Settings& s) { return s.output.frames > 0; }, "Invalid frame count."},
+        {[](const Settings& s) { return s.camera.mass > 0.0f && s.camera.r_cam_rs > 0.0f; },
+         "Invalid mass or camera radius."},
+        {[](const Settings& s) { return std::fabs(s.camera.spin) < 1.0f; }, "Spin must satisfy |a/M| < 1."},
+        {[](const Settings& s) { return s.camera.fov_deg > 0.0f && s.camera.fov_deg < 179.0f; },
+         "Invalid FOV; expected (0, 179) degrees."},
+        {[](const Settings& s) {
+             const std::string p = folded(s.ext.precision);
+             return p == "fp64" || p == "fp32c";
+         },
+         "Invalid --precision; expected fp64 or fp32c."},
+        {[](const Settings& s) {
+             const std::string c = folded(s.ext.crossing);
+             return c == "linear" || c == "hermite";
+         },
+         "Invalid --crossing; expected linear or hermite."},
+        {[](const Settings& s) { return s.ext.resume.empty() || s.output.frames == 1; },
+         "--resume applies to single-frame renders."},
+    };
+    return r;
+}
+
+// An --outpat must contain exactly one integer conversion: '%', an optional
+// '-', optional digits, 'd'; "%%" is a literal percent sign.
+bool one_int_conversion(const std::string& pat) {
+    int conversions = 0;
+    size_t i = 0;
+    while (i < pat.size()) {
+        if (pat[i] != '%') {
+            ++i;
+            continue;
+        }
+        if (i + 1 == pat.size()) return false;
+        if (pat[i + 1] == '%') {
+            i += 2;
+            continue;
+        }
+        size_t j = i + 1;
+        if (pat[j] == '-') ++j;
+        while (j < pat.size() && std::isdigit((unsigned char)pat[j])) ++j;
+        if (j == pat.size() || pat[j] != 'd') return false;
+        ++conversions;
+        i = j + 1;
+    }
+    return conversions == 1;
+}
+
+// Exposure, optional Reinhard curve, gamma, clamp, truncation to 8 bits --
+// the float operations of render.cu:583-607 in the same order.
+struct ToneCurve {
+    float exposure_scale = 1.0f, inv_gamma = 1.0f;
+    bool reinhard = false, gamma_on = false;
+
+    ToneCurve(const Settings& s) {
+        exposure_scale = std::pow(2.0f, s.look.exposure_ev);
+        reinhard = folded(s.look.tonemap) == "reinhard";
+        const float g = s.look.gamma <= 0.0f ? 1.0f : s.look.gamma;
+        gamma_on = std::fabs(g - 1.0f) > 1e-6f;
+        inv_gamma = 1.0f / std::fmax(1e-6f, g);
+    }
+    unsigned char operator()(float v) const {
+        v *= exposure_scale;
+        if (reinhard) v = v / (1.0f + v);
+        if (gamma_on) v = std::pow(std::fmax(v, 0.0f), inv_gamma);
+        v = std::fmin(std::fmax(v, 0.0f), 1.0f);
+        return (unsigned char)(255.0f * v);
+    }
+};
+
+bool save_ppm(const std::string& path, int w, int h, const std::vector<unsigned char>& pixels) {
+    std::ofstream f(path, std::ios::binary);
     if (!f) return false;
-    std::fprintf(f, "P6\n%d %d\n255\n", w, h);
-    const size_t wrote = std::fwrite(rgb.data(), 1, rgb.size(), f);
-    std::fclose(f);
-    return wrote == rgb.size();
+    std::ostringstream head;
+    head << "P6\n" << w << ' ' << h << "\n255\n";
+    const std::string hs = head.str();
+    f.write(hs.data(), (std::streamsize)hs.size());
+    f.write(reinterpret_cast<const char*>(pixels.data()), (std::streamsize)pixels.size());
+    return (bool)f;
 }
 
-std::string lower(const char* s) {
-    std::string o(s ? s : "");
-    for (auto& c : o) c = (char)std::tolower((unsigned char)c);
-    return o;
+bool save_floats(const std::string& path, const float* v, size_t n) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.write(reinterpret_cast<const char*>(v), (std::streamsize)(n * sizeof(float)));
+    return (bool)f;
+}
+
+// FNV-1a over the bytes of the render parameters up to the last field (the
+// tail padding is not part of the value): a resumed render must trace
+// exactly the rays the saved one did.
+uint64_t signature(const np_render_params& p) {
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&p);
+    const size_t n = offsetof(np_render_params, estimate_every_step) + sizeof(p.estimate_every_step);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// One frame's progressive render, accumulated on the device.
+// ---------------------------------------------------------------------------
+class ProgressiveFrame {
+public:
+    ProgressiveFrame(const np_render_params& p, const Settings& s) : p_(p), s_(s), curve_(s) {
+        n_ = (size_t)p.width * (size_t)p.height * 3;
+    }
+    ~ProgressiveFrame() {
+        if (graph_) np_pass_graph_destroy(graph_);
+        if (d_accum_) (void)hipFree(d_accum_);
+        if (d_inv_) (void)hipFree(d_inv_);
+    }
+
+    // Renders into `path`; false after printing the reason.
+    bool run(const std::string& path) {
+        const int per = std::max(1, s_.samples.per_pass);
+        const int total = s_.samples.total > 0 ? s_.samples.total : s_.samples.per_pass;
+        const int passes = (total + per - 1) / per;
+        const int ckpt = s_.samples.checkpoint_every;
+        if (hipMalloc(&d_accum_, n_ * sizeof(float)) != hipSuccess ||
+            hipMalloc(&d_inv_, 3 * sizeof(float)) != hipSuccess ||
+            hipMemset(d_inv_, 0, 3 * sizeof(float)) != hipSuccess)
+            return fail("Render failed: device memory");
+        int pass = 0;   // passes done
+        if (!s_.ext.resume.empty()) {
+            if (!load_state(s_.ext.resume, per, &pass)) return false;
+            if (pass >= passes) return write_checkpoint(path, total, true);
+        } else if (hipMemset(d_accum_, 0, n_ * sizeof(float)) != hipSuccess) {
+            return fail("Render failed: device memory");
+        }
+        // full passes are replayed from a captured graph, group by group up
+        // to the next checkpoint; a final short pass runs on its own
+        const int full = total / per;
+        const int group = ckpt > 0 ? ckpt : std::max(1, full);
+        while (pass < passes) {
+            const int next_ckpt = (ckpt > 0) ? std::min(passes, (pass / ckpt + 1) * ckpt) : passes;
+            int err = NP_OK;
+            if (pass + group <= std::min(full, next_ckpt) && (pass % group) == 0) {
+                if (!graph_) err = np_pass_graph_create(&p_, per, group, d_accum_, inv_ptr(), pass * per, &graph_);
+                if (err == NP_OK) err = np_pass_graph_seek(graph_, pass * per, nullptr);
+                if (err == NP_OK) err = np_pass_graph_launch(graph_, nullptr);
+                pass += group;
+            } else {
+                const int spp = std::min(per, total - pass * per);
+                err = np_render_pass(&p_, pass * per, spp, d_accum_, inv_ptr(), nullptr);
+                pass += 1;
+            }
+            if (err == NP_OK && hipDeviceSynchronize() != hipSuccess) err = NP_ELAUNCH;
+            if (err != NP_OK) return fail(std::string("Render failed: ") + np_status_string(err));
+            const bool checkpoint = pass == passes || (ckpt > 0 && pass % ckpt == 0);
+            if (checkpoint && !write_checkpoint(path, std::min(total, pass * per), pass == passes)) return false;
+        }
+        return true;
+    }
+
+private:
+    float* inv_ptr() { return p_.check_inv ? d_inv_ : nullptr; }
+
+    bool fail(const std::string& what) {
+        std::fprintf(stderr, "%s\n", what.c_str());
+        return false;
+    }
+
+    bool write_checkpoint(const std::string& path, int spp_done, bool last) {
+        std::vector<float> accum(n_);
+        float inv[3] = {0.0f, 0.0f, 0.0f};
+        if (hipMemcpy(accum.data(), d_accum_, n_ * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+            (p_.check_inv && hipMemcpy(inv, d_inv_, sizeof(inv), hipMemcpyDeviceToHost) != hipSuccess))
+            return fail("Render failed: copy back");
+        const float scale = 1.0f / (float)spp_done;
+        std::vector<unsigned char> bytes(n_);
+        for (size_t i = 0; i < n_; ++i) bytes[i] = curve_(accum[i] * scale);
+        if (!save_ppm(path, p_.width, p_.height, bytes)) return fail("Failed to write " + path);
+        if (p_.check_inv)
+            std::printf("[checkpoint] %s @ %d spp  H|max=%.3e  dE/E|max=%.3e  dL/L|max=%.3e\n", path.c_str(),
+                        spp_done, inv[0], inv[1], inv[2]);
+        else
+            std::printf("[checkpoint] %s @ %d spp\n", path.c_str(), spp_done);
+        if (!s_.ext.state.empty() && !save_state(s_.ext.state, accum, spp_done)) return false;
+        if (last && !s_.ext.raw_out.empty()) {
+            std::vector<float> lin(n_);
+            for (size_t i = 0; i < n_; ++i) lin[i] = accum[i] * scale;
+            if (!save_floats(s_.ext.raw_out, lin.data(), n_)) return fail("Failed to write " + s_.ext.raw_out);
+        }
+        return true;
+    }
+
+    bool save_state(const std::string& path, const std::vector<float>& accum, int spp_done) {
+        if (!save_floats(path, accum.data(), accum.size())) return fail("Failed to write " + path);
+        std::ofstream j(path + ".json");
+        j << "{\"width\": " << p_.width << ", \"height\": " << p_.height << ", \"spp_done\": " << spp_done
+          << ", \"signature\": \"" << std::hex << signature(p_) << std::dec << "\"}\n";
+        if (!j) return fail("Failed to write " + path + ".json");
+        return true;
+    }
+
+    // Restores the sums and samples of a --state file; the render continues
+    // at the next pass (sample ids continue, so the result equals an
+    // uninterrupted render).
+    bool load_state(const std::string& path, int per, int* pass) {
+        std::ifstream j(path + ".json");
+        std::stringstream text;
+        text << j.rdbuf();
+        const std::string js = text.str();
+        auto field = [&js](const char* key) -> std::string {
+            const std::string k = std::string("\"") + key + "\": ";
+            const size_t at = js.find(k);
+            if (at == std::string::npos) return "";
+            size_t b = at + k.size(), e = b;
+            if (b < js.size() && js[b] == '"') e = js.find('"', ++b);
+            else
+                while (e < js.size() && (std::isdigit((unsigned char)js[e]) || js[e] == '-')) ++e;
+            return e == std::string::npos ? "" : js.substr(b, e - b);
+        };
+        std::ostringstream want;
+        want << std::hex << signature(p_);
+        const int done = std::atoi(field("spp_done").c_str());
+        if (!j || std::atoi(field("width").c_str()) != p_.width ||
+            std::atoi(field("height").c_str()) != p_.height || field("signature") != want.str())
+            return fail("Cannot resume from " + path + ": it belongs to a different render");
+        if (done <= 0 || done % per != 0)
+            return fail("Cannot resume from " + path + ": its samples do not end on a pass of this render");
+        std::vector<float> accum(n_);
+        std::ifstream f(path, std::ios::binary);
+        f.read(reinterpret_cast<char*>(accum.data()), (std::streamsize)(n_ * sizeof(float)));
+        if (!f || f.gcount() != (std::streamsize)(n_ * sizeof(float)))
+            return fail("Cannot resume from " + path + ": short file");
+        if (hipMemcpy(d_accum_, accum.data(), n_ * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail("Render failed: device memory");
+        *pass = done / per;
+        return true;
+    }
+
+    np_render_params p_;
+    const Settings& s_;
+    ToneCurve curve_;
+    size_t n_ = 0;
+    float* d_accum_ = nullptr;
+    float* d_inv_ = nullptr;
+    np_pass_graph* graph_ = nullptr;
+};
+
+// The library's render description of these settings (render.cu:467-521
+// conversions: degrees to radians in float, mode to integrator and camera).
+np_render_params to_params(Settings& s) {
+    auto rad = [](float d) { return d * kPi / 180.0f; };
+    np_render_params p;
+    np_render_params_default(&p);
+    p.width = s.image.width;
+    p.height = s.image.height;
+    p.fov_y_deg = s.camera.fov_deg;
+    p.theta_cam = rad(s.camera.theta_deg);
+    p.mass = s.camera.mass;
+    p.spin = s.camera.spin;
+    p.r_cam_Rs = s.camera.r_cam_rs;
+    if (folded(s.look.mode) == "accurate") {
+        p.integrator = NP_INTEGRATOR_ADAPTIVE;
+        p.camera = NP_CAMERA_ZAMO;
+        s.look.invariants = true;
+    }
+    p.bg_mode = folded(s.look.background) == "point" ? NP_BG_POINT : NP_BG_GRADIENT;
+    p.bg_sigma_rad = s.look.bg_sigma_deg * kPi / 180.0f;
+    p.bg_peak = s.look.bg_brightness;
+    p.disk_on = s.look.disk ? 1 : 0;
+    p.disk_gain = s.look.disk_gain;
+    p.disk_max_orders = std::max(1, s.look.disk_orders);
+    p.check_inv = s.look.invariants ? 1 : 0;
+    // extensions
+    p.max_steps = s.ext.max_steps;
+    p.precision = folded(s.ext.precision) == "fp32c" ? NP_PREC_FP32C : NP_PREC_FP64;
+    if (s.ext.rtol > 0.0) p.rtol = s.ext.rtol;
+    if (s.ext.atol >= 0.0) p.atol = s.ext.atol;
+    p.seed = s.ext.seed;
+    const double M = (double)s.camera.mass;
+    p.fp64_radius = s.ext.fp64_radius_m > 0.0 ? s.ext.fp64_radius_m * M : 0.0;
+    p.waves_per_simd = s.ext.waves_per_simd;
+    p.crossing = folded(s.ext.crossing) == "hermite" ? NP_CROSSING_HERMITE : NP_CROSSING_LINEAR;
+    p.estimate_every_step = s.ext.estimate_every_step ? 1 : 0;
+    if (!s.ext.disk_rin.empty())
+        p.disk_rin = folded(s.ext.disk_rin) == "isco" ? np_isco_radius((double)s.camera.spin) * M
+                                                       : std::strtod(s.ext.disk_rin.c_str(), nullptr) * M;
+    if (s.ext.disk_rout_m > 0.0) p.disk_rout = s.ext.disk_rout_m * M;
+    if (s.ext.emissivity >= 0.0f) {
+        p.emiss_q = s.ext.emissivity;
+        p.emiss_m_units = 1;
+    }
+    return p;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
-    Options o;
-    parse_args(argc, argv, o);
-    // validation, in the reference's order and words (render.cu:431-465)
-    if (o.width <= 0 || o.height <= 0) { std::fprintf(stderr, "Invalid image dimensions.\n"); return 1; }
-    if (o.tile_h <= 0) { std::fprintf(stderr, "Invalid tile height.\n"); return 1; }
-    if (o.tile_h > o.height) o.tile_h = o.height;
-    if (o.samples <= 0) { std::fprintf(stderr, "Invalid samples per pixel.\n"); return 1; }
-    if (o.spp_total < 0) { std::fprintf(stderr, "Invalid total spp.\n"); return 1; }
-    if (o.frames <= 0) { std::fprintf(stderr, "Invalid frame count.\n"); return 1; }
-    if (o.mass <= 0.0f || o.r_cam <= 0.0f) { std::fprintf(stderr, "Invalid mass or camera radius.\n"); return 1; }
-    if (std::fabs(o.spin) >= 1.0f) { std::fprintf(stderr, "Spin must satisfy |a/M| < 1.\n"); return 1; }
-    if (o.fov_y_deg <= 0.0f || o.fov_y_deg >= 179.0f) {
-        std::fprintf(stderr, "Invalid FOV; expected (0, 179) degrees.\n");
-        return 1;
-    }
-    const std::string prec = lower(o.precision);
-    if (prec != "fp64" && prec != "fp32c") {
-        std::fprintf(stderr, "Invalid --precision; expected fp64 or fp32c.\n");
-        return 1;
-    }
-    const std::string crossing = lower(o.crossing);
-    if (crossing != "linear" && crossing != "hermite") {
-        std::fprintf(stderr, "Invalid --crossing; expected linear or hermite.\n");
-        return 1;
-    }
+    Settings s = parse(argc, argv);
+    for (const Rule& r : rules())
+        if (!r.ok(s)) {
+            std::fprintf(stderr, "%s\n", r.message);
+            return 1;
+        }
+    s.image.tile_rows = std::min(s.image.tile_rows, s.image.height);
+    np_render_params p = to_params(s);
 
-    // mode mapping (render.cu:467-474)
-    const std::string mode = lower(o.mode);
-    int integrator = NP_INTEGRATOR_FIXED, camera = NP_CAMERA_STATIC;
-    if (mode == "accurate") {
-        integrator = NP_INTEGRATOR_ADAPTIVE;
-        camera = NP_CAMERA_ZAMO;
-        o.check_invariants = 1;
-    }
-
-    int device_count = 0;
-    if (hipGetDeviceCount(&device_count) != hipSuccess || device_count <= 0) {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices <= 0) {
         std::fprintf(stderr, "No HIP devices found.\n");
         return 1;
     }
     (void)hipSetDevice(0);
 
-    const float pi_f = 3.14159265358979323846f;
-    auto deg2rad = [pi_f](float d) { return d * pi_f / 180.0f; };
-
-    np_render_params p;
-    np_render_params_default(&p);
-    p.width = o.width;
-    p.height = o.height;
-    p.fov_y_deg = o.fov_y_deg;
-    p.theta_cam = deg2rad(o.theta_deg);
-    p.mass = o.mass;
-    p.spin = o.spin;
-    p.r_cam_Rs = o.r_cam;
-    p.integrator = integrator;
-    p.camera = camera;
-    p.bg_mode = (lower(o.bg) == "point") ? NP_BG_POINT : NP_BG_GRADIENT;
-    p.bg_sigma_rad = o.bg_sigma_deg * pi_f / 180.0f;
-    p.bg_peak = o.bg_brightness;
-    p.disk_on = o.disk_on ? 1 : 0;
-    p.disk_gain = o.disk_gain;
-    p.disk_max_orders = std::max(1, o.disk_max_orders);
-    p.check_inv = o.check_invariants;
-    p.max_steps = o.max_steps;
-    p.precision = (prec == "fp32c") ? NP_PREC_FP32C : NP_PREC_FP64;
-    if (o.rtol > 0.0) p.rtol = o.rtol;
-    if (o.atol >= 0.0) p.atol = o.atol;
-    p.seed = o.seed;
-    const double M = (double)o.mass;
-    p.fp64_radius = o.fp64_radius_M > 0.0 ? o.fp64_radius_M * M : 0.0;
-    p.waves_per_simd = o.waves_per_simd;
-    p.crossing = (crossing == "hermite") ? NP_CROSSING_HERMITE : NP_CROSSING_LINEAR;
-    p.disk_rin = 3.0 * 2.0 * M;
-    p.disk_rout = 20.0 * 2.0 * M;
-    if (o.disk_rin) {
-        p.disk_rin = (lower(o.disk_rin) == "isco") ? np_isco_radius((double)o.spin) * M
-                                                  : strtod(o.disk_rin, nullptr) * M;
+    const float phi0 = s.camera.phi0_deg * kPi / 180.0f;
+    if (s.output.frames <= 1) {
+        p.phi_cam = phi0;
+        ProgressiveFrame frame(p, s);
+        if (!frame.run(s.output.single)) return 1;
+        std::printf("Wrote %s\n", s.output.single.c_str());
+        return 0;
     }
-    if (o.disk_rout_M > 0.0) p.disk_rout = o.disk_rout_M * M;
-    if (o.emiss_q >= 0.0f) {
-        p.emiss_q = o.emiss_q;
-        p.emiss_m_units = 1;
+    if (!one_int_conversion(s.output.pattern)) {
+        std::fprintf(stderr, "Invalid --outpat (expected a single %%d-like specifier).\n");
+        return 1;
     }
-
-    const int tonemap_mode = (lower(o.tonemap) == "reinhard") ? 1 : 0;
-    const float exposure_scale = powf(2.0f, o.exposure_ev);
-    const float gamma = (o.gamma <= 0.0f ? 1.0f : o.gamma);
-
-    auto render_one = [&](float phi_cam, const char* path) -> bool {
-        p.phi_cam = phi_cam;
-        const int total_spp = (o.spp_total > 0 ? o.spp_total : o.samples);
-        const int batch = std::max(1, o.samples);
-        const int passes = (total_spp + batch - 1) / batch;
-        std::vector<float> accum((size_t)o.width * o.height * 3, 0.0f);
-        int spp_done = 0;
-        float inv_max[3] = {0, 0, 0};
-        for (int k = 0; k < passes; ++k) {
-            const int pass_spp = std::min(batch, total_spp - spp_done);
-            const int err = np_render_accumulate(&p, spp_done, pass_spp, 1, o.tile_h, accum.data(),
-                                                 o.check_invariants ? inv_max : nullptr);
-            if (err != NP_OK) {
-                std::fprintf(stderr, "Render failed: %s\n", np_status_string(err));
-                return false;
-            }
-            spp_done += pass_spp;
-            const bool do_ckpt = (k == passes - 1) ||
-                                 (o.checkpoint_every > 0 && ((k + 1) % o.checkpoint_every == 0));
-            if (!do_ckpt) continue;
-            // tone mapping of render.cu:583-607 (truncating 8-bit conversion)
-            std::vector<unsigned char> out((size_t)o.width * o.height * 3);
-            const float inv = 1.0f / (float)spp_done;
-            for (size_t i = 0, n = out.size() / 3; i < n; ++i) {
-                float c[3];
-                for (int ch = 0; ch < 3; ++ch) {
-                    float v = accum[3 * i + ch] * inv;
-                    v *= exposure_scale;
-                    if (tonemap_mode == 1) v = v / (1.0f + v);
-                    if (std::fabs(gamma - 1.0f) > 1e-6f) v = powf(std::fmax(v, 0.0f), 1.0f / std::fmax(1e-6f, gamma));
-                    c[ch] = std::fmin(std::fmax(v, 0.0f), 1.0f);
-                }
-                out[3 * i + 0] = (unsigned char)(255.0f * c[0]);
-                out[3 * i + 1] = (unsigned char)(255.0f * c[1]);
-                out[3 * i + 2] = (unsigned char)(255.0f * c[2]);
-            }
-            if (!write_ppm(path, o.width, o.height, out)) {
-                std::fprintf(stderr, "Failed to write %s\n", path);
-                return false;
-            }
-            if (o.check_invariants)
-                std::printf("[checkpoint] %s @ %d spp  H|max=%.3e  dE/E|max=%.3e  dL/L|max=%.3e\n", path,
-                            spp_done, inv_max[0], inv_max[1], inv_max[2]);
-            else
-                std::printf("[checkpoint] %s @ %d spp\n", path, spp_done);
-            if (o.raw_out && k == passes - 1) {
-                FILE* f = std::fopen(o.raw_out, "wb");
-                bool ok = f != nullptr;
-                for (size_t i = 0; ok && i < accum.size(); ++i) {
-                    const float v = accum[i] * inv;
-                    ok = std::fwrite(&v, sizeof(float), 1, f) == 1;
-                }
-                if (f) std::fclose(f);
-                if (!ok) {
-                    std::fprintf(stderr, "Failed to write %s\n", o.raw_out);
-                    return false;
-                }
-            }
-        }
-        return true;
-    };
-
-    const float phi0 = deg2rad(o.phi0_deg);
-    if (o.frames <= 1) {
-        if (!render_one(phi0, o.out_path)) return 1;
-        std::printf("Wrote %s\n", o.out_path);
-    } else {
-        if (!validate_out_pattern(o.out_pat)) {
-            std::fprintf(stderr, "Invalid --outpat (expected a single %%d-like specifier).\n");
-            return 1;
-        }
-        for (int i = 0; i < o.frames; ++i) {
-            const float t = (float)i / (float)o.frames;
-            const float phi = phi0 + 2.0f * pi_f * o.turns * t;
-            char path[512];
-            std::snprintf(path, sizeof(path), o.out_pat, i);
-            if (!render_one(phi, path)) return 1;
-        }
-        std::printf("Wrote %d frames matching pattern %s\n", o.frames, o.out_pat);
+    for (int i = 0; i < s.output.frames; ++i) {
+        // azimuth of frame i, in the reference's float operation order
+        const float t = (float)i / (float)s.output.frames;
+        p.phi_cam = phi0 + 2.0f * kPi * s.output.turns * t;
+        char path[512];
+        std::snprintf(path, sizeof(path), s.output.pattern.c_str(), i);
+        ProgressiveFrame frame(p, s);
+        if (!frame.run(path)) return 1;
     }
+    std::printf("Wrote %d frames matching pattern %s\n", s.output.frames, s.output.pattern.c_str());
     return 0;
 }

@@ -29,6 +29,11 @@ def run(*args, cwd=None):
     (["--fov", "179"], "Invalid FOV; expected (0, 179) degrees."),
     (["--fov", "0"], "Invalid FOV; expected (0, 179) degrees."),
     (["--precision", "fp16"], "Invalid --precision; expected fp64 or fp32c."),
+    (["--crossing", "cubic"], "Invalid --crossing; expected linear or hermite."),
+    (["--resume", "s.f32", "--frames", "2"], "--resume applies to single-frame renders."),
+    # validation order: the first failing rule reports
+    (["--w", "0", "--tile", "0", "--spin", "2"], "Invalid image dimensions."),
+    (["--tile", "-1", "--spin", "2"], "Invalid tile height."),
 ])
 def test_validation_messages(args, msg):
     if not os.path.exists(RENDER):
@@ -94,3 +99,163 @@ def test_cli_end_to_end_against_oracle(oracle, device, tmp_path):
     lin = np.fromfile(raw, dtype=np.float32).reshape(H * W, 3)
     rel = np.abs(lin - acc / np.float32(3)) / np.maximum(np.abs(acc / 3), 1e-6)
     assert np.median(rel) < 1e-6
+
+
+def _ppm(path, W, H):
+    data = open(path, "rb").read()
+    header = f"P6\n{W} {H}\n255\n".encode()
+    assert data.startswith(header)
+    return np.frombuffer(data[len(header):], dtype=np.uint8).reshape(H, W, 3)
+
+
+def _oracle_accum(oracle, p, total, per):
+    """The CLI's progressive accumulation (passes of `per` samples, pass sums
+    added in order) restated with the oracle."""
+    import nullpath_amd as npa
+    q = oracle.default_params()
+    for name, _ in npa.RenderParams._fields_:
+        setattr(q, name, getattr(p, name))
+    ys, xs = np.mgrid[0:p.height, 0:p.width]
+    acc = np.zeros((p.width * p.height, 3), np.float32)
+    done = 0
+    while done < total:
+        k = min(per, total - done)
+        rgb, info = oracle.render_pixels(q, xs.ravel(), ys.ravel(), done, k)
+        acc += rgb
+        done += k
+    return acc, info
+
+
+def _close(img, ref):
+    diff = np.abs(img.astype(int) - ref.astype(int))
+    assert np.mean(diff == 0) > 0.99 and diff.max() <= 2, (np.mean(diff == 0), diff.max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["point_ring", "reinhard", "no_disk"])
+def test_cli_looks_against_oracle(oracle, device, tmp_path, case):
+    """--bg point (the photon-ring point source, render.cu:313-323,346-353),
+    --tonemap reinhard with --exposure and --gamma (render.cu:583-607) and
+    --no-disk, each against the oracle's image through the same tone curve."""
+    import nullpath_amd as npa
+    W, H = 32, 20
+    out = tmp_path / f"{case}.ppm"
+    args = ["--w", str(W), "-h", str(H), "--spin", "0.7", "--theta", "70", "--fov", "25", "--rcam", "20",
+            "--samples", "2", "--disk-max-orders", "2", "--out", str(out)]
+    kw = dict(width=W, height=H, spin=0.7, theta_cam=npa.deg2rad_f32(70.0), fov_y_deg=25.0, r_cam_Rs=20.0,
+              disk_max_orders=2)
+    tone = {}
+    if case == "point_ring":
+        args += ["--bg", "POINT", "--bg-sigma", "2", "--bg-bright", "4"]
+        kw.update(bg_mode=1, bg_sigma_rad=npa.deg2rad_f32(2.0), bg_peak=4.0)
+    elif case == "reinhard":
+        args += ["--tonemap", "Reinhard", "--exposure", "1.5", "--gamma", "1.0"]
+        tone = dict(exposure=1.5, gamma=1.0, reinhard=True)
+    else:
+        args += ["--no-disk"]
+        kw.update(disk_on=0)
+    print(f"[gpu] render CLI {case}", flush=True)
+    r = run(*args)
+    assert r.returncode == 0, r.stderr
+    acc, info = _oracle_accum(oracle, npa.default_params(**kw), 2, 2)
+    _close(_ppm(out, W, H), _tonemap(acc, 2, **tone).reshape(H, W, 3))
+    if case == "point_ring":
+        assert (info["cls"] == 1).any()      # captured rays take the ring weight
+    if case == "no_disk":
+        assert (info["orders"] > 0).any()    # disk crossings exist but are not shaded
+
+
+@pytest.mark.gpu
+def test_cli_animation_frames(oracle, device, tmp_path):
+    """--frames/--turns/--outpat (render.cu:622-638): one file per frame named
+    by the pattern, the camera azimuth phi0 + 2 pi turns i/frames, the
+    summary line; a pattern without exactly one %d conversion is refused."""
+    import nullpath_amd as npa
+    W, H = 24, 16
+    pat = str(tmp_path / "f%%_%03d.ppm")
+    print("[gpu] render CLI animation", flush=True)
+    r = run("--w", str(W), "-h", str(H), "--spin", "0.5", "--fov", "30", "--rcam", "25", "--frames", "3",
+            "--turns", "0.5", "--phi0", "10", "--outpat", pat)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == f"Wrote 3 frames matching pattern {pat}"
+    f32 = np.float32
+    for i in range(3):
+        path = tmp_path / f"f%_{i:03d}.ppm"
+        assert f"[checkpoint] {path} @ 1 spp" in r.stdout
+        phi = f32(npa.deg2rad_f32(10.0)) + f32(f32(f32(2.0) * f32(npa.PI_F)) * f32(0.5)) * (f32(i) / f32(3))
+        acc, _ = _oracle_accum(oracle, npa.default_params(width=W, height=H, spin=0.5, fov_y_deg=30.0,
+                                                          r_cam_Rs=25.0, phi_cam=float(phi)), 1, 1)
+        _close(_ppm(path, W, H), _tonemap(acc, 1).reshape(H, W, 3))
+    for bad in ("frame.ppm", "f_%d_%d.ppm", "f_%s.ppm", "f_%5", "f_%"):
+        r = run("--w", "8", "-h", "8", "--frames", "2", "--outpat", str(tmp_path / bad))
+        assert r.returncode == 1 and "Invalid --outpat" in r.stderr, bad
+
+
+@pytest.mark.gpu
+def test_cli_invariant_monitor_against_oracle(oracle, device, tmp_path):
+    """--check-invariants (and --mode accurate, which implies it and selects the
+    adaptive integrator with the ZAMO camera): the checkpoint line's |H| max,
+    dE/E max and dL/L max (render.cu:233-244, 327-345) against the oracle's
+    maxima over the same rays."""
+    import re
+    import nullpath_amd as npa
+    W, H = 24, 16
+    for mode, kw in (("balanced", {}), ("accurate", dict(integrator=1, camera=1, rtol=1e-10))):
+        out = tmp_path / f"inv_{mode}.ppm"
+        print(f"[gpu] render CLI invariants {mode}", flush=True)
+        args = ["--w", str(W), "-h", str(H), "--spin", "0.8", "--theta", "80", "--fov", "20", "--rcam", "15",
+                "--samples", "2", "--disk-max-orders", "2", "--mode", mode, "--out", str(out)]
+        if mode == "balanced":
+            args.append("--check-invariants")
+        r = run(*args)
+        assert r.returncode == 0, r.stderr
+        m = re.search(r"H\|max=(\S+)  dE/E\|max=(\S+)  dL/L\|max=(\S+)", r.stdout)
+        assert m, r.stdout
+        got = [float(v) for v in m.groups()]
+        p = npa.default_params(width=W, height=H, spin=0.8, theta_cam=npa.deg2rad_f32(80.0), fov_y_deg=20.0,
+                               r_cam_Rs=15.0, disk_max_orders=2, check_inv=1, **kw)
+        acc, info = _oracle_accum(oracle, p, 2, 2)
+        want = info["inv"].max(axis=0).astype(np.float64)
+        print(f"[gpu] invariants {mode}: cli {got} oracle {want.tolist()}", flush=True)
+        # p_t and p_phi are conserved exactly by both integrators
+        assert got[1] == 0.0 and got[2] == 0.0 and want[1] == 0.0 and want[2] == 0.0
+        # |H| (the constraint drift of the integrator) equals the oracle's to
+        # the four digits printed
+        assert want[0] > 0.0
+        assert abs(got[0] - want[0]) <= 1e-3 * want[0], (got[0], want[0])
+        _close(_ppm(out, W, H), _tonemap(acc, 2).reshape(H, W, 3))
+
+
+@pytest.mark.gpu
+def test_cli_resume_equals_uninterrupted_render(device, tmp_path):
+    """--state writes the float32 accumulation and a sidecar at every
+    checkpoint; --resume continues from it: the result is byte for byte the
+    uninterrupted render (sample ids continue, passes add in the same order).
+    A state of other parameters is refused."""
+    import json
+    W, H = 24, 16
+    base = ["--w", str(W), "-h", str(H), "--spin", "0.9", "--fov", "20", "--rcam", "15", "--samples", "2"]
+    st = tmp_path / "state.f32"
+    print("[gpu] render CLI resume", flush=True)
+    r1 = run(*base, "--spp-total", "4", "--checkpoint-every", "1", "--state", str(st),
+             "--out", str(tmp_path / "a.ppm"))
+    assert r1.returncode == 0, r1.stderr
+    meta = json.loads((tmp_path / "state.f32.json").read_text())
+    assert meta["spp_done"] == 4 and meta["width"] == W and meta["height"] == H
+    r2 = run(*base, "--spp-total", "4", "--state", str(st), "--out", str(tmp_path / "b.ppm"))
+    assert r2.returncode == 0, r2.stderr
+    # interrupted after 2 of 6 samples, then resumed to 6
+    r3 = run(*base, "--spp-total", "2", "--state", str(st), "--out", str(tmp_path / "c.ppm"))
+    assert r3.returncode == 0 and json.loads((tmp_path / "state.f32.json").read_text())["spp_done"] == 2
+    r4 = run(*base, "--spp-total", "6", "--resume", str(st), "--out", str(tmp_path / "d.ppm"),
+             "--raw-out", str(tmp_path / "d.f32"))
+    assert r4.returncode == 0, r4.stderr
+    assert "[checkpoint]" in r4.stdout and "@ 6 spp" in r4.stdout
+    r5 = run(*base, "--spp-total", "6", "--out", str(tmp_path / "e.ppm"), "--raw-out", str(tmp_path / "e.f32"))
+    assert r5.returncode == 0, r5.stderr
+    assert (tmp_path / "d.ppm").read_bytes() == (tmp_path / "e.ppm").read_bytes()
+    assert (tmp_path / "d.f32").read_bytes() == (tmp_path / "e.f32").read_bytes()
+    assert (tmp_path / "a.ppm").read_bytes() == (tmp_path / "b.ppm").read_bytes()
+    r6 = run(*base[:-1], "3", "--spin", "0.5", "--spp-total", "6", "--resume", str(st),
+             "--out", str(tmp_path / "x.ppm"))
+    assert r6.returncode == 1 and "different render" in r6.stderr
