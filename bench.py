@@ -55,8 +55,13 @@ FP32C_HBM_BYTES_1080P4 = (282358.5625 * 2.0 + 772341.625 * 1.0) * 1024.0
 #   fp32c FP64 steps: the FP64 Carter right-hand side with its polynomial
 #     sincos is 105 flops; per attempt 12 stage inputs and sums (12 x 18),
 #     3 updates (3 x 10) and the error norm (40).
+#   fp64c (the FP32C path's formulas and step control in double): the FP64
+#     Carter right-hand side, 105 flops; per attempt 10 stage inputs and sums
+#     (10 x 18), 3 updates (3 x 10) and the error norm (40); without the
+#     estimate 7 x 18 + 2 x 10 + 8.
 FLOPS = {
     "fp64": {"rhs": 200, "attempt": 3 * 196 + 29},
+    "fp64c": {"rhs": 105, "attempt": 10 * 18 + 3 * 10 + 40, "attempt_lazy": 7 * 18 + 2 * 10 + 8},
     "fp32c": {"rhs": 70, "attempt": 10 * 18 + 26 + 2 * 56 + 90, "attempt_lazy": 7 * 18 + 2 * 56 + 8,
               "rhs64": 105, "attempt64": 12 * 18 + 3 * 10 + 40},
 }
@@ -67,6 +72,9 @@ def valu_flops(c: dict, prec: str):
     fl = FLOPS[prec]
     if prec == "fp64":
         return 0, c["rhs_evals"] * fl["rhs"] + c["attempts"] * fl["attempt"]
+    if prec == "fp64c":
+        lazy = c.get("attempts_lazy", 0)
+        return 0, c["rhs_evals"] * fl["rhs"] + (c["attempts"] - lazy) * fl["attempt"] + lazy * fl["attempt_lazy"]
     lazy = c.get("attempts_lazy", 0)
     f32 = (c["rhs_evals"] - c["rhs_evals_f64"]) * fl["rhs"] + \
         (c["attempts"] - c["attempts_f64"] - lazy) * fl["attempt"] + lazy * fl["attempt_lazy"]
@@ -85,7 +93,9 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=3000,
                     help="pixels (x spp rays) in the CPU-baseline sample; 0 skips it")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--precisions", default="fp64,fp32c")
+    ap.add_argument("--precisions", default="fp64,fp64c,fp32c",
+                    help="fp64 = the reference's formulas in double; fp64c = the FP32C path's formulas and "
+                         "step control in double; fp32c = the headline")
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed steps of the FP64 comparison leg (no warmup; one 1080p x4 FP64 frame "
                          "takes ~18 s, so it does not follow --steps/--warmup)")
@@ -135,7 +145,7 @@ def render_case(prec: str, crossing: str, width: int, height: int, spp: int, ste
     from nullpath_amd._lib import ray_record_dtype
 
     dev = torch.device("cuda:0")
-    pid = 1 if prec == "fp32c" else 0
+    pid = {"fp64": 0, "fp32c": 1, "fp64c": 2}[prec]
     cr = 1 if crossing == "hermite" else 0
     npix = width * height
     rays = npix * spp
@@ -698,6 +708,10 @@ def main() -> int:
     }
     for prec, r in res.items():
         line["config"][prec] = case_summary(r)
+    if "fp64c" in res and "fp32c" in res:
+        a, b = res["fp64c"], res["fp32c"]
+        line["config"]["speedup_fp32c_over_fp64c"] = (b["rays"] / b["wall"]) / (a["rays"] / a["wall"])
+        line["config"]["fp32c_vs_fp64c"] = compare(a, b)
     if "fp64" in res and "fp32c" in res:
         a, b = res["fp64"], res["fp32c"]
         line["config"]["speedup_fp32c_over_fp64"] = (b["rays"] / b["wall"]) / (a["rays"] / a["wall"])

@@ -26,7 +26,11 @@ extern "C" {
 enum np_integrator { NP_INTEGRATOR_FIXED = 0, NP_INTEGRATOR_ADAPTIVE = 1 };
 enum np_camera     { NP_CAMERA_STATIC = 0, NP_CAMERA_ZAMO = 1 };
 enum np_bg         { NP_BG_GRADIENT = 0, NP_BG_POINT = 1 };
-enum np_precision  { NP_PREC_FP64 = 0, NP_PREC_FP32C = 1 };
+/* FP64: the reference's formulas in double.  FP32C: float-compensated
+ * stepping with the Carter-form field (FP64 where float cannot decide).
+ * FP64_CARTER: a comparator, the FP32C path's formulas and step control all
+ * in double, so that FP32C against it measures the arithmetic type alone. */
+enum np_precision  { NP_PREC_FP64 = 0, NP_PREC_FP32C = 1, NP_PREC_FP64_CARTER = 2 };
 /* Radius of an equatorial crossing between two accepted states: LINEAR is
  * the reference's interpolation in theta (render.cu:211-226), whose error
  * (~h^2 r'') depends on where the adaptive steps fall; HERMITE solves the

@@ -71,6 +71,50 @@ def build_lib(incremental: bool = False) -> str:
     return LIB
 
 
+def device_functions(obj: str) -> tuple[list[str], list[str]]:
+    """(kernels, other functions) in the gfx950 code object of a compiled
+    object file: every kernel has a `<name>.kd` descriptor symbol; a function
+    without one is a device function the compiler kept out of line."""
+    import tempfile
+    llvm = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "llvm", "bin")
+    with tempfile.TemporaryDirectory() as tmp:
+        fb, elf = os.path.join(tmp, "fatbin"), os.path.join(tmp, "dev.elf")
+        subprocess.run(["objcopy", f"--dump-section=.hip_fatbin={fb}", obj, os.path.join(tmp, "x.o")],
+                       check=True, capture_output=True)
+        subprocess.run([os.path.join(llvm, "clang-offload-bundler"), "--type=o", f"--input={fb}",
+                        f"--output={elf}", f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}", "--unbundle"],
+                       check=True, capture_output=True)
+        out = subprocess.run([os.path.join(llvm, "llvm-readelf"), "-s", "-W", elf], check=True,
+                             capture_output=True, text=True).stdout
+    funcs, descriptors = set(), set()
+    for line in out.splitlines():
+        f = line.split()
+        if len(f) >= 8 and f[3] == "FUNC":
+            funcs.add(f[7])
+        elif len(f) >= 8 and f[3] == "OBJECT" and f[7].endswith(".kd"):
+            descriptors.add(f[7][:-3])
+    return sorted(funcs & descriptors), sorted(funcs - descriptors)
+
+
+def check_no_device_calls() -> None:
+    """No kernel of the library calls a device function: large numerics are
+    force-inlined, because an out-of-line function larger than the branch
+    range gets long-branch sequences that reuse its return-address registers
+    (the cause of a memory-aperture fault this build has had).  Raises when
+    a code object holds a non-kernel function."""
+    bad = {}
+    for src in LIB_SOURCES:
+        if not src.endswith(".hip"):
+            continue
+        kernels, others = device_functions(os.path.join(OBJ_DIR, src + ".o"))
+        if not kernels:
+            raise RuntimeError(f"{src}: no kernels found in its gfx950 code object")
+        if others:
+            bad[src] = others
+    if bad:
+        raise RuntimeError(f"device functions kept out of line (kernels must inline them): {bad}")
+
+
 def build_cli() -> str:
     """The `render` command line, linked against libnullpath.so next to it."""
     os.makedirs(BIN_DIR, exist_ok=True)
@@ -84,6 +128,7 @@ def build_cli() -> str:
 
 def build_all(incremental: bool = False) -> None:
     build_lib(incremental)
+    check_no_device_calls()
     build_cli()
 
 

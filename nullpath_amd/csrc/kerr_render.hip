@@ -1169,6 +1169,178 @@ struct StepF32C {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// FP64 comparator with the FP32C path's formulas (NP_PREC_FP64_CARTER): the
+// Carter-form field kerr_rhs_c64 with the polynomial sincos_d, plain stage
+// sums, and the FP32C step control (step doubling with the reference's
+// constants, the far-field estimate skip under a wave-uniform full step),
+// all in double.  It exists to time the precision alone: FP32C against this
+// path differs only in the arithmetic type.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void step_doubling_c64_lazy(const FrameConst& F, const Y5& y, const Deriv& k1,
+                                                       const double E, const double L, const double h,
+                                                       const bool full_step, Y5& full, Y5& half1,
+                                                       Y5& half2) {
+    const double h2 = 0.5 * h;
+    Deriv S = k1, kp = k1;
+    Y5 b = y;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+        if (j < 3 && !full_step) continue;
+        const double c = (j <= 1) ? 0.5 * h : (j == 2) ? h : (j == 5 || j == 9) ? h2 : 0.5 * h2;
+        const double cc = (j == 6) ? 0.0 : c;
+        const Deriv k = kerr_rhs_c64(F, b.r + cc * kp.r, b.th + cc * kp.th, b.pr + cc * kp.pr,
+                                     b.pth + cc * kp.pth, E, L);
+        if (j == 6) {
+            S = k;
+        } else {
+            const double w = (j == 2 || j == 5 || j == 9) ? 1.0 : 2.0;
+            S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
+        }
+        kp = k;
+        if (j == 2 || j == 5 || j == 9) {
+            const double h6 = (j == 2 ? h : h2) / 6.0;
+            const Y5& base = (j == 9) ? half1 : y;
+            Y5 o;
+            o.r = base.r + h6 * S.r;
+            o.th = base.th + h6 * S.th;
+            o.ph = base.ph + h6 * S.ph;
+            o.pr = base.pr + h6 * S.pr;
+            o.pth = base.pth + h6 * S.pth;
+            if (j == 2) {
+                full = o;
+                S = k1;
+                kp = k1;
+            } else if (j == 5) {
+                half1 = o;
+                b = o;
+            } else {
+                half2 = o;
+            }
+        }
+    }
+}
+
+struct StepC64 {
+    struct Ray : RayBase {
+        Y5 y;
+        Deriv k1;
+        double h_adapt, h, qb;
+        bool k1_valid, skip;
+    };
+
+    __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
+                                 float cam_x, float cam_y) {
+        init_momentum(F, n_r, n_th, n_ph, R.pt, R.y.pr, R.y.pth, R.pph);
+        R.y.r = F.r_cam;
+        R.y.th = F.th_cam;
+        R.y.ph = F.ph_cam;
+        R.h_adapt = 0.0;
+        R.h = 0.0;
+        R.k1_valid = false;
+        R.skip = false;
+        double st, ct;
+        sincos_d(F.th_cam, st, ct);
+        const double E = -R.pt, L = R.pph, a2E2 = F.a * F.a * E * E;
+        const double Q = R.y.pth * R.y.pth + ct * ct * (L * L / fmax(st * st, 1e-300) - a2E2);
+        R.qb = sqrt(fmax(Q + a2E2, 0.0)) * (1.0 + 1e-6);
+        base_start(F, R, cam_x, cam_y);
+    }
+
+    __device__ static double r(const Ray& R) { return R.y.r; }
+    __device__ static double th(const Ray& R) { return R.y.th; }
+    __device__ static double pr(const Ray& R) { return R.y.pr; }
+    __device__ static double pth(const Ray& R) { return R.y.pth; }
+
+    __device__ static void cross_derivs(const FrameConst& F, Ray& R, double& h, double& dr0,
+                                        double& dth0, double& dr1, double& dth1) {
+        h = R.h;
+        dr0 = R.k1.r;
+        dth0 = R.k1.th;
+        R.k1 = kerr_rhs_c64(F, R.y.r, R.y.th, R.y.pr, R.y.pth, -R.pt, R.pph);
+        R.k1_valid = true;
+        dr1 = R.k1.r;
+        dth1 = R.k1.th;
+    }
+
+    template <bool ADAPTIVE>
+    __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& /*rhs64*/,
+                                  int& lazy_att) {
+        const double E = -R.pt, L = R.pph;
+        const double Rs = F.Rs_d;
+        if (!ADAPTIVE) {
+            const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
+            const double h = (double)STEP0 * scale;
+            double k1r, k1th;
+            R.y = rk4_c64(F, R.y, E, L, h, k1r, k1th);
+            R.k1.r = k1r;
+            R.k1.th = k1th;
+            R.k1_valid = false;
+            R.h = h;
+            rhs_count += 4;
+            const bool ok = R.y.r > F.r_plus + 1e-9 && R.y.th > 1e-9 && R.y.th < NP_PI_D - 1e-9;
+            return ok ? 1 : 2;
+        }
+        if (R.attempt == 0) {
+            const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
+            const double h_base = fmin(fmax((double)STEP0 * scale, HMIN), HMAX);
+            const double h = (R.h_adapt > 0.0) ? R.h_adapt : h_base;
+            R.h = fmin(fmax(h, HMIN), HMAX);
+        }
+        const double h = R.h;
+        if (!R.k1_valid) {
+            R.k1 = kerr_rhs_c64(F, R.y.r, R.y.th, R.y.pr, R.y.pth, E, L);
+            rhs_count += 1;
+            R.k1_valid = true;
+        }
+        const bool far = h == HMAX && R.y.r >= (double)F.r_lazyf;
+        bool lazy = false;
+        if (far && R.skip) {
+            const double rm = R.y.r - F.M;
+            const double dth = h * R.qb / (rm * rm) + 1e-6;
+            lazy = R.y.th - dth > 1e-9 + 1e-6 && R.y.th + dth < NP_PI_D - 1e-9 - 1e-6;
+        }
+        R.skip = lazy;
+        const bool full_step = __ballot(!lazy) != 0;
+        Y5 full, half1, half2;
+        step_doubling_c64_lazy(F, R.y, R.k1, E, L, h, full_step, full, half1, half2);
+        rhs_count += full_step ? 10 : 7;
+        lazy_att = full_step ? 0 : 1;
+        const double lim = NP_PI_D - 1e-9;
+        const bool ok2 = half1.r > F.r_plus + 1e-12 && half1.th > 1e-9 && half1.th < lim && half2.r > F.r_plus + 1e-12 &&
+                         half2.th > 1e-9 && half2.th < lim;
+        double h_try = h, fac = 0.0, e = 0.0;
+        if (lazy && ok2) {
+            fac = 2.0;
+        } else if (!lazy && ok2 && full.r > F.r_plus + 1e-12 && full.th > 1e-9 && full.th < lim) {
+            const double rt = F.rtol, at = F.atol;
+            e = fmax(fmax(err1(full.r, half2.r, at + rt * fmax(fabs(full.r), fabs(half2.r))),
+                          err1(full.th, half2.th, at + rt * fmax(fabs(full.th), fabs(half2.th)))),
+                     fmax(err1(full.ph, half2.ph, at + rt * fmax(fabs(full.ph), fabs(half2.ph))),
+                          fmax(err1(full.pr, half2.pr, at + rt * fmax(fabs(full.pr), fabs(half2.pr))),
+                               err1(full.pth, half2.pth, at + rt * fmax(fabs(full.pth), fabs(half2.pth))))));
+            if (e <= 1.0) fac = fmin(2.0, SAFETY * pow(fmax(1e-12, e), -0.2));
+            else h_try *= fmax(0.1, SAFETY * pow(fmax(1e-12, e), -0.25));
+        } else {
+            h_try *= 0.5;
+        }
+        if (fac > 0.0) {
+            R.y = half2;
+            R.k1_valid = false;
+            R.h_adapt = fmin(fmax(h_try * fac, HMIN), HMAX);
+            R.attempt = 0;
+            if (!lazy) R.skip = F.lazy && far && e < (double)kLazyError;
+            return 1;
+        }
+        R.skip = false;
+        R.h = fmax(h_try, HMIN);
+        R.attempt += 1;
+        if (R.h <= HMIN * 1.01 || R.attempt >= 8) return 2;
+        return 0;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Persistent renderer: work item = pixel (or a chunk of consecutive pixels),
 // lane-level refill, one integrator attempt per loop iteration.
@@ -1412,6 +1584,8 @@ const void* kernel_ptr_f32c(bool inv, int variant) {
 const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
     if (precision == NP_PREC_FP32C)
         return adaptive ? kernel_ptr_f32c<true>(inv, variant) : kernel_ptr_f32c<false>(inv, variant);
+    if (precision == NP_PREC_FP64_CARTER)
+        return adaptive ? kernel_ptr<StepC64, true>(inv) : kernel_ptr<StepC64, false>(inv);
     return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
 }
 

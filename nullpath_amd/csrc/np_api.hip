@@ -100,7 +100,7 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
                   float* d_rgb, int accumulate, float* d_inv, np_trace_counters* d_counters,
                   np_ray_record* d_records, hipStream_t stream) {
     if (!p || !d_rgb || spp < 0 || rows < 0 || y0 < 0 || sample_offset < 0) return NP_EINVAL;
-    if (p->precision != NP_PREC_FP64 && p->precision != NP_PREC_FP32C) return NP_EINVAL;
+    if (p->precision < NP_PREC_FP64 || p->precision > NP_PREC_FP64_CARTER) return NP_EINVAL;
     FrameConst F;
     int err = make_frame(p, &F);
     if (err != NP_OK) return err;
@@ -144,7 +144,7 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, int sampl
     for (int i = 0; i < nframes; ++i) {
         np_render_params q = frames[i];
         if (q.precision != frames[0].precision || q.integrator != frames[0].integrator) return NP_EINVAL;
-        if (q.precision != NP_PREC_FP64 && q.precision != NP_PREC_FP32C) return NP_EINVAL;
+        if (q.precision < NP_PREC_FP64 || q.precision > NP_PREC_FP64_CARTER) return NP_EINVAL;
         if (q.check_inv) return NP_EINVAL;
         if (t) {
             q.waves_per_simd = t->waves_per_simd;
@@ -345,7 +345,7 @@ int create_pass_graph(const np_render_params* p, int spp, int passes, float* d_a
     if (!out) return NP_EINVAL;
     *out = nullptr;
     if (!p || !d_accum || spp <= 0 || passes <= 0 || first_sample < 0) return NP_EINVAL;
-    if (p->precision != NP_PREC_FP64 && p->precision != NP_PREC_FP32C) return NP_EINVAL;
+    if (p->precision < NP_PREC_FP64 || p->precision > NP_PREC_FP64_CARTER) return NP_EINVAL;
     FrameConst F;
     int err = make_frame(p, &F);
     if (err != NP_OK) return err;

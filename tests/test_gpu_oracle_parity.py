@@ -82,14 +82,14 @@ ZOOMS = {"disk": (96, 54, 1.6), "photon_ring": (80, 45, 0.8)}
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("zoom", sorted(ZOOMS))
-@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("precision", [1, 0, 2])
 def test_kerr_still_vs_oracle(oracle, device, zoom, precision):
     w, h, fov = ZOOMS[zoom]
     p = npa.kerr_disk_params(width=w, height=h, fov=fov, seed=1, precision=precision, crossing=1)
     g_rgb, g_rec = _gpu_frame(p, 1)
     o_rgb, o_info = _oracle_frame(oracle, p, 1, ("still", zoom))
     ndisk, nband = check_against_oracle(p, 1, g_rgb, g_rec, o_rgb, o_info,
-                                        f"{'fp32c' if precision else 'fp64'} {zoom}")
+                                        f"{['fp64', 'fp32c', 'fp64-carter'][precision]} {zoom}")
     assert ndisk > (1000 if zoom == "disk" else 300)
     if zoom == "photon_ring":
         assert (o_info["orders"] >= 2).sum() > 50
