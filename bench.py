@@ -34,10 +34,10 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # HBM bytes of one FP32C render launch on the 1920x1080x4 frame (with ray
 # records), measured with rocprofv3 FETCH_SIZE and WRITE_SIZE and scaled by
 # what the counters read for a 256 MiB device copy (FETCH x2.00, WRITE x1.00;
-# profiles/kerr_render_f32c_1920x1080x4_hbm_traffic.txt): 578.3 MB read +
-# 790.9 MB written.  The algorithmic bytes are the 24.9 MB image and 265.4 MB
-# of ray records; the rest is register-spill scratch of the persistent lanes.
-FP32C_HBM_BYTES_1080P4 = (282358.5625 * 2.0 + 772341.625 * 1.0) * 1024.0
+# profiles/kerr_render_1920x1080x4_kernel_trace.txt): 121.1 MB read + 667.7 MB
+# written.  The algorithmic bytes are the 24.9 MB image and 265.4 MB of ray
+# records; the rest is register-spill scratch of the persistent lanes.
+FP32C_HBM_BYTES_1080P4 = (59138.4 * 2.0 + 652010.8 * 1.0) * 1024.0
 
 # Algorithmic operation counts per unit (add, sub, mul, div, sqrt, rcp = 1;
 # fma = 2; library sin/cos/pow counted apart), derived from the kernel source:
@@ -65,6 +65,34 @@ FLOPS = {
     "fp32c": {"rhs": 70, "attempt": 10 * 18 + 26 + 2 * 56 + 90, "attempt_lazy": 7 * 18 + 2 * 56 + 8,
               "rhs64": 105, "attempt64": 12 * 18 + 3 * 10 + 40},
 }
+
+
+# Schwarzschild FP64 sweep (schw.hip schw_sweep_kernel): the right-hand side
+# is 33 flops (r^2, r^3, 1 - Rs/r, the five derivative terms and the force
+# sum) plus one sincos; a step is 4 right-hand sides, 3 stage inputs (30),
+# the step law (2), the stage combination and update (36) and the unwrapped
+# delta-phi (2).
+SCHW_FLOPS_PER_STEP = 4 * 33 + 30 + 2 + 36 + 2
+
+# Differentiable render (adjoint.hip): the r-scaled Carter field is 53 flops
+# and its vector-Jacobian product 258 (counted with a counting scalar on the
+# host: tests/support/flop_count.cpp, pinned by tests/test_flop_counts.py).
+# Forward FP32C step: 4 fields + 3 stage inputs (24) + the stage combination
+# (20 + 2) + 4 float-float updates of 10 + reading the float-float state (4).
+# Backward step: the segment's forward recompute (one forward step) + 4 float
+# stores + the RK4 adjoint (3 fields, 4 products, 76 for stage inputs and
+# adjoint updates).
+ADJ_RHS, ADJ_VJP = 53, 258
+ADJ_FWD_STEP = 4 * ADJ_RHS + 24 + 22 + 4 * 10 + 4
+ADJ_BWD_STEP = ADJ_FWD_STEP + 4 + 3 * ADJ_RHS + 4 * ADJ_VJP + 76
+
+# GEOMU batch (geomu.hip geomu_camera_kernel), data dependent (Laguerre and
+# Carlson duplication counts): FP64 flops per ray measured on the 2^20-ray
+# workload as SQ_INSTS_VALU_FLOPS_FP64 x 64 x the lane utilisation
+# (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)), i.e. the flops the
+# active lanes executed, division and square-root sequences included
+# (profiles/geomu_2p20_counters.txt).
+GEOMU_FLOPS_PER_RAY = 7456
 
 
 def valu_flops(c: dict, prec: str):
@@ -118,6 +146,8 @@ def parse():
     ap.add_argument("--geokerr-driver-only", type=int, default=0,
                     help="only time the whole GEOKERR driver (MUFILL and USEGEOR runs) on this many "
                          "geodesics and exit")
+    ap.add_argument("--only", default="", choices=["", "schw", "geomu", "adjoint", "batch"],
+                    help="only time this side workload (with its CPU baseline unless --cpu-pixels 0) and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (a reference "
                          "kernel of known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) "
@@ -238,14 +268,33 @@ def compare(a: dict, b: dict) -> dict:
     }
 
 
+def host_threads() -> int:
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    return min(16, ncpu)
+
+
+def roofline(flops: float, seconds: float, prec: str, kernel: str) -> dict:
+    peak = PEAK_TFLOPS[prec]
+    achieved = flops / seconds / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "kernel": kernel, "flops_per_launch": flops}
+
+
 def bench_adjoint(args) -> dict:
     """Gradients/s of the differentiable disk render: 512x512, 4096 fixed
     RK4 steps per ray (FP32-compensated), checkpoint every 64 steps; loss
     sum((I - I_target)^2) with the target at (a=0.7, theta=60 deg, q=3,
     gain=1) and the gradient taken at (0.6, 65 deg, 2.5, 0.8).  One gradient
-    = one forward + one backward."""
+    = one forward + one backward.  Beside it: the rooflines of the forward
+    and adjoint kernels (algorithmic FP32 flops from the steps each ray ran,
+    read back from the workspace) and the CPU baseline, the FP64
+    forward-mode oracle gradient on a sample of pixels."""
     import math
 
+    import numpy as np
     import torch
 
     from nullpath_amd import adjoint as A
@@ -268,29 +317,62 @@ def bench_adjoint(args) -> dict:
     one()
     torch.cuda.synchronize()
     steps = max(1, args.steps)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
     t0 = time.perf_counter()
     for k in range(steps):
         evs[k][0].record(stream)
         A.forward(start, A.PREC_FP32C, image=img, ws=ws)
         evs[k][1].record(stream)
         torch.sub(img, tgt, out=gimg).mul_(2.0)
-        A.backward(start, gimg, ws, grad4=g4)
         evs[k][2].record(stream)
+        A.backward(start, gimg, ws, grad4=g4)
+        evs[k][3].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    fwd = [a.elapsed_time(b) for a, b, _ in evs]
-    tot = [a.elapsed_time(c) for a, _, c in evs]
-    return {
+    fwd = [a.elapsed_time(b) for a, b, _, _ in evs]
+    bwd = [c.elapsed_time(d) for _, _, c, d in evs]
+    tot = [a.elapsed_time(d) for a, _, _, d in evs]
+    # steps each ray ran (forward) and the steps the adjoint walks back (from
+    # the last counted crossing), from the workspace's per-ray records
+    nrays = n * n
+    nck = (base.n_steps + base.ckpt_every - 1) // base.ckpt_every
+    meta = ws.buf[nck * 8 * nrays * 4:(nck * 8 + 6) * nrays * 4].view(torch.int32).view(6, nrays).cpu().numpy()
+    term, ncross = meta[0].astype(np.int64), meta[1]
+    last = np.where(ncross > 0, meta[1 + np.clip(ncross, 1, 3), np.arange(nrays)], -1)
+    w = (gimg.view(nrays, 3).double() * torch.tensor([1.0, 0.75, 0.45], dtype=torch.float64,
+                                                        device=gimg.device)).sum(dim=1).cpu().numpy()
+    bsteps = np.where((ncross > 0) & (w != 0.0), last + 1, 0).sum()
+    fl_f, fl_b = float(term.sum()) * ADJ_FWD_STEP, float(bsteps) * ADJ_BWD_STEP
+    res = {
         "workload": f"{n}x{n}, 4096 fixed RK4 steps/ray FP32-comp, checkpoint every 64",
         "grads_per_s": steps / wall,
         "ms_per_grad": wall * 1e3 / steps,
         "forward_ms": fwd,
+        "backward_ms": bwd,
         "forward_backward_ms": tot,
         "grad": g4.cpu().tolist(),
         "workspace_bytes": ws.nbytes,
+        "forward_steps": int(term.sum()), "adjoint_steps": int(bsteps),
+        "roofline_forward": roofline(fl_f, sum(fwd) / len(fwd) / 1e3, "fp32", "adj_forward_f32"),
+        "roofline_backward": roofline(fl_b, sum(bwd) / len(bwd) / 1e3, "fp32", "adj_backward_f32"),
     }
-
+    if args.cpu_pixels > 0:
+        from oracle import np_oracle as O
+        m = 16384
+        rng = np.random.default_rng(7)
+        sel = rng.choice(nrays, size=m, replace=False)
+        q = O.adjoint_params()
+        for name, _ in O.AdjointParams._fields_:
+            setattr(q, name, getattr(start, name))
+        th = host_threads()
+        t1 = time.perf_counter()
+        O.adj_pixels(q, sel % n, sel // n, nthreads=th)
+        dt = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": 1.0 / (dt * nrays / m), "unit": "gradients/s", "cores": th,
+                               "kind": "port",
+                               "sample": f"FP64 forward-mode oracle gradient of {m} seeded random pixels of the "
+                                         f"{n}x{n} image ({dt:.1f} s), scaled to the whole image"}
+    return res
 
 def batch_frames(nframes: int = 64, seed: int = 4096):
     """The uneven multi-frame workload: nframes frames, resolution drawn from
@@ -310,7 +392,7 @@ def batch_frames(nframes: int = 64, seed: int = 4096):
             for i, (r, a, t) in enumerate(zip(res, spins, thetas))]
 
 
-def bench_batch(spp: int = 2) -> dict:
+def bench_batch(spp: int = 2, cpu_pixels_per_frame: int = 0) -> dict:
     """One persistent launch over the 64-frame uneven batch, after the
     autotuner picked block size, refill chunk, FP32C kernel variant and
     FP64 radius on a 1/8-resolution copy; beside it the same frames as 64
@@ -356,7 +438,7 @@ def bench_batch(spp: int = 2) -> dict:
     n_gt = int((relpix > 1e-5).sum().item())
     f32, f64 = valu_flops(c, "fp32c")
     achieved = (f32 + 2.0 * f64) / (batch_ms / 1e3) / 1e12
-    return {
+    res = {
         "workload": f"{len(frames)} frames (256^2..1024^2, a in {{0,0.5,0.9,0.998}}, theta U[5,89] deg), "
                     f"{spp} spp, r_cam=50M fov 50deg adaptive rtol 1e-8, FP32C, one persistent launch",
         "rays": total * spp,
@@ -374,21 +456,45 @@ def bench_batch(spp: int = 2) -> dict:
         "steps_per_ray_mean": c["steps"] / max(1, c["rays"]),
         "fp64_attempt_frac": c["attempts_f64"] / max(1, c["attempts"]),
         "valu_peak_frac": achieved / PEAK_TFLOPS["fp32"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS["fp32"], "kernel": "kerr_render (FP32C, tuned variant)",
+                     "note": "FP64 flops count twice (FP32-equivalent)"},
     }
+    if cpu_pixels_per_frame > 0:
+        import numpy as np
+
+        from oracle import np_oracle as O
+        rng = np.random.default_rng(99)
+        th = host_threads()
+        t1 = time.perf_counter()
+        for f in frames:
+            q = O.default_params()
+            for name, _ in npa.RenderParams._fields_:
+                setattr(q, name, getattr(f, name))
+            sel = rng.choice(f.width * f.height, size=cpu_pixels_per_frame, replace=False)
+            O.render_pixels(q, sel % f.width, sel // f.width, 0, spp, nthreads=th, with_info=False)
+        dt = time.perf_counter() - t1
+        m = cpu_pixels_per_frame * len(frames) * spp
+        res["cpu_baseline"] = {"value": m / dt, "unit": "rays/s", "cores": th, "kind": "port",
+                               "sample": f"{cpu_pixels_per_frame} seeded random pixels x {spp} spp of each of the "
+                                         f"{len(frames)} frames ({m} rays, {dt:.1f} s), FP64 oracle, OpenMP"}
+    return res
 
 
-def bench_schwarzschild(n: int) -> dict:
+def bench_schwarzschild(n: int, cpu_rays: int = 0) -> dict:
     """Plain-method FP64 Schwarzschild sweep: M = 10, n equatorial photons
     from r0 = 1000 M, b uniform on [0.5 M, 15 M] (seed 2024), captured at
     r < 2.01 M, escaped at r > r0 outbound, photon sphere |b - 3 sqrt(3) M| <
-    1e-3 M.  Timed with HIP events on the launch stream."""
+    1e-3 M.  Timed with HIP events on the launch stream; the oracle's sweep
+    on cpu_rays of the same impact parameters as the CPU baseline."""
     import numpy as np
     import torch
 
     import nullpath_amd as npa
     M = 10.0
     rng = np.random.default_rng(2024)
-    b = torch.tensor(rng.uniform(0.5 * M, 15.0 * M, n), dtype=torch.float64, device="cuda")
+    bh = rng.uniform(0.5 * M, 15.0 * M, n)
+    b = torch.tensor(bh, dtype=torch.float64, device="cuda")
     total = torch.zeros(1, dtype=torch.int64, device="cuda")
     npa.schw_sweep(b[:256], M=M)
     torch.cuda.synchronize()
@@ -400,15 +506,26 @@ def bench_schwarzschild(n: int) -> dict:
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     c = np.bincount(cls.cpu().numpy(), minlength=4)
-    return {
+    nsteps = int(total.item())
+    res = {
         "workload": f"Schwarzschild M=10, {n} equatorial photons from r0=1000M, b~U[0.5M,15M], FP64 RK4",
         "rays_per_s": n / (ms / 1e3), "kernel_ms": ms,
-        "steps_per_s": int(total.item()) / (ms / 1e3),
+        "steps_per_s": nsteps / (ms / 1e3),
         "escaped": int(c[0]), "captured": int(c[1]), "photon_sphere": int(c[2]), "stepcap": int(c[3]),
+        "roofline": roofline(float(nsteps) * SCHW_FLOPS_PER_STEP, ms / 1e3, "fp64", "schw_sweep_kernel"),
     }
+    if cpu_rays > 0:
+        from oracle import np_oracle as O
+        th = host_threads()
+        t1 = time.perf_counter()
+        O.schw_sweep(M, 1000.0 * M, 2.01 * M, 1e-3 * M, 10 ** 7, bh[:cpu_rays], nthreads=th)
+        dt = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": cpu_rays / dt, "unit": "rays/s", "cores": th, "kind": "port",
+                               "sample": f"the first {cpu_rays} of the {n} impact parameters ({dt:.1f} s), "
+                                         "FP64 oracle sweep, OpenMP"}
+    return res
 
-
-def bench_geomu(n: int) -> dict:
+def bench_geomu(n: int, cpu_rays: int = 0) -> dict:
     """GEOKERR GEOMU batch: n rays with alpha, beta ~ U[-12, 12] M, a ~
     U[0, 0.998], mu0 ~ U[0.05, 0.99], target u ~ U(0, u_+) (seed 2025), from
     u0 = 1e-4, FP64 with Carlson RF; timed with HIP events."""
@@ -443,7 +560,7 @@ def bench_geomu(n: int) -> dict:
     ms_mixed = e0.elapsed_time(e1)
     same = (cm == ncase) & (tm == tpm)
     dmu = (mm - muf).abs()
-    return {
+    res = {
         "workload": f"GEOMU {n} rays, alpha,beta~U[-12,12]M, a~U[0,0.998], mu0~U[0.05,0.99], u~U(0,u+), FP64",
         "rays_per_s": n / (ms / 1e3), "kernel_ms": ms,
         "ncase_counts": [int(v) for v in nc],
@@ -456,6 +573,20 @@ def bench_geomu(n: int) -> dict:
             "muf_abs_diff_gt_1e-10_frac": float((dmu > 1e-10).double().mean().item()),
         },
     }
+    if GEOMU_FLOPS_PER_RAY is not None:
+        res["roofline"] = roofline(GEOMU_FLOPS_PER_RAY * n, ms / 1e3, "fp64", "geomu_camera_kernel<false>")
+        res["roofline"]["note"] = "flops per ray measured by the FP64 flop counter on active lanes"
+    if cpu_rays > 0:
+        from oracle import np_oracle as O
+        host = [x[:cpu_rays].cpu().numpy() for x in args]
+        th = host_threads()
+        t1 = time.perf_counter()
+        O.geomu_batch(*host, 1e-4, nthreads=th)
+        dt = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": cpu_rays / dt, "unit": "rays/s", "cores": th, "kind": "port",
+                               "sample": f"the first {cpu_rays} of the {n} rays ({dt:.1f} s), FP64 oracle GEOMU "
+                                         "restatement, OpenMP"}
+    return res
 
 
 def geokerr_cases(n: int, seed: int = 2026):
@@ -647,6 +778,14 @@ def main() -> int:
     if args.geomu_only > 0:
         print(json.dumps(bench_geomu(args.geomu_only)))
         return 0
+    if args.only:
+        cpu = args.cpu_pixels > 0
+        run = {"schw": lambda: bench_schwarzschild(100000, 20000 if cpu else 0),
+               "geomu": lambda: bench_geomu(1 << 20, (1 << 20) if cpu else 0),
+               "adjoint": lambda: bench_adjoint(args),
+               "batch": lambda: bench_batch(2, 256 if cpu else 0)}[args.only]
+        print(json.dumps(run()))
+        return 0
     if args.geokerr_only > 0:
         print(json.dumps(bench_geokerr(args.geokerr_only, 4096, 0)))
         return 0
@@ -728,9 +867,10 @@ def main() -> int:
     if args.adjoint:
         line["config"]["adjoint"] = bench_adjoint(args)
     if args.side_workloads:
-        line["config"]["uneven_batch"] = bench_batch()
-        line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000)
-        line["config"]["geomu_batch"] = bench_geomu(1 << 20)
+        cpu = args.cpu_pixels > 0
+        line["config"]["uneven_batch"] = bench_batch(2, 256 if cpu else 0)
+        line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000, 20000 if cpu else 0)
+        line["config"]["geomu_batch"] = bench_geomu(1 << 20, (1 << 20) if cpu else 0)
         line["config"]["geokerr_eval_lambda"] = bench_geokerr(256, 4096, 64 if args.cpu_pixels > 0 else 0)
         line["config"]["geokerr_driver"] = bench_geokerr_driver(131072, 32 if args.cpu_pixels > 0 else 0)
 
