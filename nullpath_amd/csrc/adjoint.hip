@@ -16,13 +16,17 @@
 // (a, theta_obs, q, gain).  Pixel colour = C summed x (1, 0.75, 0.45).
 //
 // Forward: FP32-compensated state (float-float r, theta, p_r, p_theta),
-// checkpointed every ckpt_every steps into the workspace.  Backward: per
-// ray, from its last crossing down to step 0, each checkpoint segment is
-// recomputed forward (float stage inputs stored in the workspace) and the
-// discrete RK4 adjoint is applied step by step with the hand-written
-// vector-Jacobian product of the right-hand side; crossing terms enter the
-// adjoint of the two states that bracket the crossing.  Parameter
-// gradients collect the initial-state sensitivity (camera tetrad
+// checkpointed every ckpt_every steps into the workspace, with the two
+// states around every counted crossing.  Backward: a prep pass evaluates
+// each crossing term's partials (they do not depend on the adjoint state)
+// and sorts the rays with adjoint work by the number of checkpoint
+// segments they walk back; persistent waves then take the sorted rays
+// longest first and, per ray, from its last crossing down to step 0,
+// recompute each checkpoint segment forward (float stage inputs stored in
+// the workspace) and apply the discrete RK4 adjoint step by step with the
+// hand-written vector-Jacobian product of the right-hand side; crossing
+// terms enter the adjoint of the two states that bracket the crossing.
+// Parameter gradients collect the initial-state sensitivity (camera tetrad
 // derivatives from the host), the E, L and a dependence of every
 // right-hand side, and the explicit dependence of C.
 #include <hip/hip_runtime.h>
@@ -321,30 +325,69 @@ __device__ __forceinline__ bool crosses(const AdjConst& K, double th0, double th
     return ((d0 > 0 && d1 < 0) || (d0 < 0 && d1 > 0)) && r1 > K.rp + 1e-3;
 }
 
-// Workspace layout (SoA, coalesced per lane):
+// Workspace layout (SoA, one column per ray or per work-list slot; every
+// section starts on a 256-byte boundary):
 //   ck   [nck][8][nrays] float     checkpoints (hi, lo of r, theta, p_r, p_theta)
-//   meta [6][nrays] int32          term step, crossings, crossing steps x3, unused
-//   scr  [ck+1][4][nrays] float    backward recompute (float stage inputs)
+//   meta [7][nrays] int32          term step, crossings, crossing steps x3,
+//                                  work key (-1: no adjoint work), work list
+//   xs   [3][4][nrays] double      per crossing: r and theta - pi/2 of the two
+//                                  states around it (r0, r1, d0, d1)
+//   co   [8][nrays] double         initial-state partials d(p_r, p_theta, E, L)/d(a, theta_obs)
+//   el   [2][nrays] float          E, L as the float stepping uses them
+//   inj  [3][4][nrays] float       per crossing: adjoint injected into the later
+//                                  state (r, theta) and into the earlier one
+//   scr  [ck+1][4][nrays] float    backward recompute (float stage inputs),
+//                                  one column per work-list slot
+//   ctl  uint32 bucket[kBuckets], total, pad; uint64 fetch
+constexpr int kBuckets = 512;
+
+struct Ctl {
+    uint32_t bucket[kBuckets];
+    uint32_t total, pad;
+    unsigned long long fetch;
+};
+
 struct WS {
     float* ck;
     int32_t* meta;
+    double* xs;
+    double* co;
+    float* el;
+    float* inj;
     float* scr;
+    Ctl* ctl;
 };
 
-__host__ __device__ inline WS ws_layout(void* base, int64_t nrays, int nck, int ck) {
-    WS w;
-    char* p = (char*)base;
-    w.ck = (float*)p;
-    p += (size_t)nck * 8 * nrays * sizeof(float);
-    w.meta = (int32_t*)p;
-    p += (size_t)6 * nrays * sizeof(int32_t);
-    w.scr = (float*)p;
-    return w;
+// byte offsets of the sections and the total size
+struct WSOffsets {
+    size_t ck, meta, xs, co, el, inj, scr, ctl, end;
+};
+
+__host__ __device__ inline size_t ws_align(size_t n) { return (n + 255) & ~(size_t)255; }
+
+__host__ __device__ inline WSOffsets ws_offsets(int64_t nrays, int nck, int ck) {
+    const size_t n = (size_t)nrays;
+    WSOffsets o;
+    o.ck = 0;
+    o.meta = ws_align(o.ck + (size_t)nck * 8 * n * sizeof(float));
+    o.xs = ws_align(o.meta + 7 * n * sizeof(int32_t));
+    o.co = ws_align(o.xs + 12 * n * sizeof(double));
+    o.el = ws_align(o.co + 8 * n * sizeof(double));
+    o.inj = ws_align(o.el + 2 * n * sizeof(float));
+    o.scr = ws_align(o.inj + 12 * n * sizeof(float));
+    o.ctl = ws_align(o.scr + (size_t)(ck + 1) * 4 * n * sizeof(float));
+    o.end = o.ctl + sizeof(Ctl);
+    return o;
 }
 
-int64_t ws_bytes(int64_t nrays, int nck, int ck) {
-    return (int64_t)nrays * ((int64_t)nck * 8 * 4 + 6 * 4 + (int64_t)(ck + 1) * 4 * 4);
+__host__ __device__ inline WS ws_layout(void* base, int64_t nrays, int nck, int ck) {
+    const WSOffsets o = ws_offsets(nrays, nck, ck);
+    char* p = (char*)base;
+    return WS{(float*)(p + o.ck), (int32_t*)(p + o.meta), (double*)(p + o.xs), (double*)(p + o.co),
+              (float*)(p + o.el), (float*)(p + o.inj), (float*)(p + o.scr), (Ctl*)(p + o.ctl)};
 }
+
+int64_t ws_bytes(int64_t nrays, int nck, int ck) { return (int64_t)ws_offsets(nrays, nck, ck).end; }
 
 // ---------------- forward kernels ----------------
 __global__ void __launch_bounds__(256)
@@ -379,6 +422,8 @@ adj_forward_f32(const AdjConst K, float* __restrict__ image, void* __restrict__ 
                 const double al = d0 / (d0 - d1);
                 double lr;
                 isum += disk_term(K, r0 + al * (r1 - r0), in.E, in.L, lr).v;
+                double* x = ws.xs + (size_t)ncross * 4 * nrays + ray;
+                x[0 * nrays] = r0; x[1 * nrays] = r1; x[2 * nrays] = d0; x[3 * nrays] = d1;
                 cs[ncross++] = n;
             }
             s = t;
@@ -392,7 +437,6 @@ adj_forward_f32(const AdjConst K, float* __restrict__ image, void* __restrict__ 
         ws.meta[2 * nrays + ray] = cs[0];
         ws.meta[3 * nrays + ray] = cs[1];
         ws.meta[4 * nrays + ray] = cs[2];
-        ws.meta[5 * nrays + ray] = 0;
         const float I = (float)isum;
         image[3 * ray + 0] = 1.0f * I;
         image[3 * ray + 1] = 0.75f * I;
@@ -465,9 +509,29 @@ __device__ __forceinline__ CState<float> step_vjp(const AdjConst& K, const CStat
     return out;
 }
 
+// Work key of a ray that walks back `segs` checkpoint segments: the
+// segment count itself when it fits the buckets, else scaled onto them.
+__device__ __forceinline__ int work_key(int segs, int nck) {
+    return nck <= kBuckets ? segs - 1 : (int)((int64_t)(segs - 1) * kBuckets / nck);
+}
+
+__global__ void __launch_bounds__(kBuckets) adj_clear(double* __restrict__ grad4, Ctl* __restrict__ ctl) {
+    if (threadIdx.x < 4) grad4[threadIdx.x] = 0.0;
+    ctl->bucket[threadIdx.x] = 0u;
+}
+
+// Per ray: the crossing terms of the loss, which do not depend on the
+// adjoint state (their partials in r, theta of the two bracketing states,
+// scaled by the pixel weight, for the backward to inject; their E, L, a,
+// q, gain partials folded into the gradient here), the initial-state
+// partials the backward contracts with at the end, and the ray's work key
+// counted in the bucket histogram.
 __global__ void __launch_bounds__(256)
-adj_backward_f32(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws_base,
-                 double* __restrict__ grad4) {
+adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws_base,
+         double* __restrict__ grad4) {
+    __shared__ uint32_t hist[kBuckets];
+    for (int t = threadIdx.x; t < kBuckets; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
     const int64_t nrays = (int64_t)K.W * K.H;
     const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
     double g_a = 0.0, g_th = 0.0, g_q = 0.0, g_gain = 0.0;
@@ -475,82 +539,48 @@ adj_backward_f32(const AdjConst K, const float* __restrict__ gimg, void* __restr
          ray += (int64_t)gridDim.x * blockDim.x) {
         const int ncross = ws.meta[1 * nrays + ray];
         const double wpix = 1.0 * gimg[3 * ray + 0] + 0.75 * gimg[3 * ray + 1] + 0.45 * gimg[3 * ray + 2];
-        if (ncross == 0 || wpix == 0.0) continue;
-        const int x = (int)(ray % K.W), y = (int)(ray / K.W);
-        float n_r, n_th, n_ph;
-        adj_camera_dir(K, x, y, n_r, n_th, n_ph);
-        const Init in = adj_init(K, n_r, n_th, n_ph);
-        const float E = (float)in.E, L = (float)in.L;
-        int cs[3];
-        cs[0] = ws.meta[2 * nrays + ray];
-        cs[1] = ws.meta[3 * nrays + ray];
-        cs[2] = ws.meta[4 * nrays + ray];
-        const int n_last = cs[ncross - 1];
-        CState<float> lam{0.0f, 0.0f, 0.0f, 0.0f};
-        float gE = 0.0f, gL = 0.0f, ga = 0.0f;
-        double gE_x = 0.0, gL_x = 0.0, ga_x = 0.0;
-        int ci = ncross - 1;
-        for (int seg = n_last / K.ck; seg >= 0; --seg) {
-            const int n0 = seg * K.ck;
-            // recompute the segment: states n0 .. n0 + ck (float stage inputs)
-            const float* c = ws.ck + (size_t)seg * 8 * nrays + ray;
-            FFState s;
-            s.r.hi = c[0 * nrays]; s.r.lo = c[1 * nrays]; s.th.hi = c[2 * nrays]; s.th.lo = c[3 * nrays];
-            s.pr.hi = c[4 * nrays]; s.pr.lo = c[5 * nrays]; s.pth.hi = c[6 * nrays]; s.pth.lo = c[7 * nrays];
-            const int jmax = min(K.ck - 1, n_last - n0);
-            for (int j = 0; j <= jmax + 1; ++j) {
-                float* o = ws.scr + (size_t)j * 4 * nrays + ray;
-                // keep the double view of r, theta for the crossing partials
-                o[0 * nrays] = ffp_f(s.r);
-                o[1 * nrays] = ffp_f(s.th);
-                o[2 * nrays] = ffp_f(s.pr);
-                o[3 * nrays] = ffp_f(s.pth);
-                if (j <= jmax) s = step_f32(K, s, E, L);
+        int key = -1;
+        if (ncross > 0 && wpix != 0.0) {
+            const int x = (int)(ray % K.W), y = (int)(ray / K.W);
+            float n_r, n_th, n_ph;
+            adj_camera_dir(K, x, y, n_r, n_th, n_ph);
+            const Init in = adj_init(K, n_r, n_th, n_ph);
+            double gE_x = 0.0, gL_x = 0.0;
+            for (int c = 0; c < ncross; ++c) {
+                const double* xc = ws.xs + (size_t)c * 4 * nrays + ray;
+                const double r0 = xc[0], r1 = xc[nrays], d0 = xc[2 * nrays], d1 = xc[3 * nrays];
+                // r_x = r0 + al (r1 - r0), al = d0 / (d0 - d1)
+                const double D = d0 - d1;
+                const double al = d0 / D;
+                double lr;
+                const D4 C = disk_term(K, r0 + al * (r1 - r0), in.E, in.L, lr);
+                const double dr = wpix * C.r;
+                float* q = ws.inj + (size_t)c * 4 * nrays + ray;
+                q[0 * nrays] = (float)(dr * al);
+                q[1 * nrays] = (float)(dr * (r1 - r0) * (d0 / (D * D)));
+                q[2 * nrays] = (float)(dr * (1.0 - al));
+                q[3 * nrays] = (float)(dr * (r1 - r0) * (-d1 / (D * D)));
+                gE_x += wpix * C.E;
+                gL_x += wpix * C.L;
+                g_a += wpix * C.a;
+                g_q += wpix * (-lr) * C.v;
+                g_gain += wpix * C.v / K.gain;
             }
-            for (int j = jmax; j >= 0; --j) {
-                const int n = n0 + j;
-                const float* yj = ws.scr + (size_t)j * 4 * nrays + ray;
-                const CState<float> y0{yj[0], yj[nrays], yj[2 * nrays], yj[3 * nrays]};
-                CState<float> extra{0.0f, 0.0f, 0.0f, 0.0f};
-                if (ci >= 0 && cs[ci] == n) {
-                    const float* y1p = ws.scr + (size_t)(j + 1) * 4 * nrays + ray;
-                    const double r0 = y0.r, r1 = y1p[0];
-                    const double d0 = (double)y0.th - NP_PI_D * 0.5, d1 = (double)y1p[nrays] - NP_PI_D * 0.5;
-                    const double D = d0 - d1;
-                    const double al = d0 / D;
-                    const double rx = r0 + al * (r1 - r0);
-                    double lr;
-                    const D4 C = disk_term(K, rx, in.E, in.L, lr);
-                    const double dr = wpix * C.r;
-                    // r_x = r0 + al (r1 - r0), al = d0 / (d0 - d1)
-                    extra.r = (float)(dr * (1.0 - al));
-                    extra.th = (float)(dr * (r1 - r0) * (-d1 / (D * D)));
-                    lam.r += (float)(dr * al);
-                    lam.th += (float)(dr * (r1 - r0) * (d0 / (D * D)));
-                    gE_x += wpix * C.E;
-                    gL_x += wpix * C.L;
-                    ga_x += wpix * C.a;
-                    g_q += wpix * (-lr) * C.v;
-                    g_gain += wpix * C.v / K.gain;
-                    --ci;
-                }
-                lam = step_vjp(K, y0, E, L, lam, gE, gL, ga);
-                lam.r += extra.r;
-                lam.th += extra.th;
-            }
+            g_a += gE_x * in.dE[0] + gL_x * in.dL[0];
+            g_th += gE_x * in.dE[1] + gL_x * in.dL[1];
+            double* co = ws.co + ray;
+            co[0 * nrays] = in.dpr[0]; co[1 * nrays] = in.dpr[1];
+            co[2 * nrays] = in.dpth[0]; co[3 * nrays] = in.dpth[1];
+            co[4 * nrays] = in.dE[0]; co[5 * nrays] = in.dE[1];
+            co[6 * nrays] = in.dL[0]; co[7 * nrays] = in.dL[1];
+            ws.el[0 * nrays + ray] = (float)in.E;
+            ws.el[1 * nrays + ray] = (float)in.L;
+            const int n_last = ws.meta[(1 + ncross) * nrays + ray];
+            key = work_key(n_last / K.ck + 1, K.nck);
+            atomicAdd(&hist[key], 1u);
         }
-        // initial state: y0 = (r_cam, theta_obs, p_r, p_th), E, L from the tetrad
-        const double GE = (double)gE + gE_x, GL = (double)gL + gL_x;
-        for (int k = 0; k < 2; ++k) {
-            const double dth0 = (k == 1) ? 1.0 : 0.0;
-            const double v = (double)lam.th * dth0 + (double)lam.pr * in.dpr[k] + (double)lam.pth * in.dpth[k] +
-                             GE * in.dE[k] + GL * in.dL[k];
-            if (k == 0) g_a += v;
-            else g_th += v;
-        }
-        g_a += (double)ga + ga_x;
+        ws.meta[5 * nrays + ray] = key;
     }
-    // wave reduction, then one atomic per wave and component
     for (int off = 32; off > 0; off >>= 1) {
         g_a += __shfl_xor(g_a, off);
         g_th += __shfl_xor(g_th, off);
@@ -563,10 +593,120 @@ adj_backward_f32(const AdjConst K, const float* __restrict__ gimg, void* __restr
         atomicAdd(&grad4[2], g_q);
         atomicAdd(&grad4[3], g_gain);
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kBuckets; t += blockDim.x)
+        if (hist[t]) atomicAdd(&ws.ctl->bucket[t], hist[t]);
 }
 
-__global__ void zero4(double* p) {
-    if (threadIdx.x < 4) p[threadIdx.x] = 0.0;
+// Bucket counts -> first list slot of each bucket, the longest work first
+// (bucket kBuckets-1 at slot 0); one block of kBuckets threads.
+__global__ void __launch_bounds__(kBuckets) adj_scan(Ctl* __restrict__ ctl) {
+    __shared__ uint32_t s[kBuckets];
+    const int t = threadIdx.x;
+    const uint32_t own = ctl->bucket[kBuckets - 1 - t];
+    s[t] = own;
+    __syncthreads();
+    for (int off = 1; off < kBuckets; off <<= 1) {
+        const uint32_t v = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    ctl->bucket[kBuckets - 1 - t] = s[t] - own;
+    if (t == kBuckets - 1) {
+        ctl->total = s[t];
+        ctl->fetch = 0ull;
+    }
+}
+
+__global__ void __launch_bounds__(256) adj_scatter(const AdjConst K, void* __restrict__ ws_base) {
+    const int64_t nrays = (int64_t)K.W * K.H;
+    const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
+    for (int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ray < nrays;
+         ray += (int64_t)gridDim.x * blockDim.x) {
+        const int key = ws.meta[5 * nrays + ray];
+        if (key >= 0) ws.meta[6 * nrays + atomicAdd(&ws.ctl->bucket[key], 1u)] = ray;
+    }
+}
+
+// The adjoint walk of the rays on the work list: persistent waves take 64
+// consecutive slots at a time (longest work first, so a wave's lanes walk
+// back nearly the same number of segments and the short rays fill the
+// tail).  The recompute buffer is indexed by list slot, so a wave's
+// stores and loads there are coalesced.
+__global__ void __launch_bounds__(256)
+adj_backward_f32(const AdjConst K, void* __restrict__ ws_base, double* __restrict__ grad4) {
+    const int64_t nrays = (int64_t)K.W * K.H;
+    const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
+    const uint32_t total = ws.ctl->total;
+    double g_a = 0.0, g_th = 0.0;
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane_id() == 0) base = atomicAdd(&ws.ctl->fetch, 64ull);
+        base = __shfl(base, 0);
+        if (base >= total) break;
+        const int64_t i = (int64_t)base + lane_id();
+        if (i >= total) continue;
+        const int64_t ray = ws.meta[6 * nrays + i];
+        const int ncross = ws.meta[1 * nrays + ray];
+        int cs[3];
+        cs[0] = ws.meta[2 * nrays + ray];
+        cs[1] = ws.meta[3 * nrays + ray];
+        cs[2] = ws.meta[4 * nrays + ray];
+        const int n_last = cs[ncross - 1];
+        const float E = ws.el[ray], L = ws.el[nrays + ray];
+        CState<float> lam{0.0f, 0.0f, 0.0f, 0.0f};
+        float gE = 0.0f, gL = 0.0f, ga = 0.0f;
+        int ci = ncross - 1;
+        for (int seg = n_last / K.ck; seg >= 0; --seg) {
+            const int n0 = seg * K.ck;
+            // recompute the segment: states n0 .. n0 + ck (float stage inputs)
+            const float* c = ws.ck + (size_t)seg * 8 * nrays + ray;
+            FFState s;
+            s.r.hi = c[0 * nrays]; s.r.lo = c[1 * nrays]; s.th.hi = c[2 * nrays]; s.th.lo = c[3 * nrays];
+            s.pr.hi = c[4 * nrays]; s.pr.lo = c[5 * nrays]; s.pth.hi = c[6 * nrays]; s.pth.lo = c[7 * nrays];
+            const int jmax = min(K.ck - 1, n_last - n0);
+            for (int j = 0; j <= jmax; ++j) {
+                float* o = ws.scr + (size_t)j * 4 * nrays + i;
+                o[0 * nrays] = ffp_f(s.r);
+                o[1 * nrays] = ffp_f(s.th);
+                o[2 * nrays] = ffp_f(s.pr);
+                o[3 * nrays] = ffp_f(s.pth);
+                if (j < jmax) s = step_f32(K, s, E, L);
+            }
+            for (int j = jmax; j >= 0; --j) {
+                const int n = n0 + j;
+                const float* yj = ws.scr + (size_t)j * 4 * nrays + i;
+                const CState<float> y0{yj[0], yj[nrays], yj[2 * nrays], yj[3 * nrays]};
+                float xr = 0.0f, xth = 0.0f;
+                if (ci >= 0 && cs[ci] == n) {
+                    const float* q = ws.inj + (size_t)ci * 4 * nrays + ray;
+                    lam.r += q[0];
+                    lam.th += q[nrays];
+                    xr = q[2 * nrays];
+                    xth = q[3 * nrays];
+                    --ci;
+                }
+                lam = step_vjp(K, y0, E, L, lam, gE, gL, ga);
+                lam.r += xr;
+                lam.th += xth;
+            }
+        }
+        // initial state: y0 = (r_cam, theta_obs, p_r, p_th), E, L from the tetrad
+        const double* co = ws.co + ray;
+        g_a += (double)lam.pr * co[0] + (double)lam.pth * co[2 * nrays] + (double)gE * co[4 * nrays] +
+               (double)gL * co[6 * nrays] + (double)ga;
+        g_th += (double)lam.th + (double)lam.pr * co[nrays] + (double)lam.pth * co[3 * nrays] +
+                (double)gE * co[5 * nrays] + (double)gL * co[7 * nrays];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        g_a += __shfl_xor(g_a, off);
+        g_th += __shfl_xor(g_th, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&grad4[0], g_a);
+        atomicAdd(&grad4[1], g_th);
+    }
 }
 
 int grid_for(int64_t n) {
@@ -576,6 +716,16 @@ int grid_for(int64_t n) {
     const int64_t want = (n + 255) / 256;
     const int64_t cap = (int64_t)cus * 32;
     return (int)(want < cap ? (want < 1 ? 1 : want) : cap);
+}
+
+// persistent grid of the backward: every CU filled at the kernel's occupancy
+int backward_grid() {
+    int dev = 0, cus = 256, per_cu = 1;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adj_backward_f32, 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return cus * per_cu;
 }
 
 }  // namespace
@@ -617,9 +767,15 @@ int np_adjoint_backward(const np_adjoint_params* p, const float* d_grad_image, v
     if (err != NP_OK) return err;
     if (!d_grad_image || !d_workspace || !d_grad4) return NP_EINVAL;
     const int64_t nrays = (int64_t)K.W * K.H;
-    hipLaunchKernelGGL(np::zero4, dim3(1), dim3(64), 0, (hipStream_t)stream, d_grad4);
-    hipLaunchKernelGGL(np::adj_backward_f32, dim3(np::grid_for(nrays)), dim3(256), 0, (hipStream_t)stream,
-                       K, d_grad_image, d_workspace, d_grad4);
+    const hipStream_t st = (hipStream_t)stream;
+    np::Ctl* ctl = np::ws_layout(d_workspace, nrays, K.nck, K.ck).ctl;
+    const dim3 grid(np::grid_for(nrays));
+    hipLaunchKernelGGL(np::adj_clear, dim3(1), dim3(np::kBuckets), 0, st, d_grad4, ctl);
+    hipLaunchKernelGGL(np::adj_prep, grid, dim3(256), 0, st, K, d_grad_image, d_workspace, d_grad4);
+    hipLaunchKernelGGL(np::adj_scan, dim3(1), dim3(np::kBuckets), 0, st, ctl);
+    hipLaunchKernelGGL(np::adj_scatter, grid, dim3(256), 0, st, K, d_workspace);
+    hipLaunchKernelGGL(np::adj_backward_f32, dim3(np::backward_grid()), dim3(256), 0, st, K, d_workspace,
+                       d_grad4);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

@@ -14,6 +14,8 @@
 
 #include <cmath>
 
+#include "np_math.hpp"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define NP_HD __host__ __device__
@@ -28,10 +30,7 @@ struct CState {
     Real r, th, pr, pth;
 };
 
-NP_HD inline void carter_sincos(float x, float& s, float& c) {
-    s = sinf(x);
-    c = cosf(x);
-}
+NP_HD inline void carter_sincos(float x, float& s, float& c) { sincos_f(x, s, c); }
 NP_HD inline void carter_sincos(double x, double& s, double& c) {
     s = sin(x);
     c = cos(x);

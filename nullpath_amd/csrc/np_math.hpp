@@ -55,4 +55,33 @@ NP_HD inline void sincos_d(double x, double& s, double& c) {
     c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
 }
 
+// sin and cos of a float angle of moderate size (|x| < 1e3; the Carter
+// field calls it with polar angles in (0, pi)): the same Cody-Waite
+// reduction by pi/2, here with a three-part float split of pi/2 applied by
+// fused multiply-adds, then the Taylor series of sin through x^9 and of cos
+// through x^10 on [-pi/4, pi/4] (truncation below 3e-9 relative).  Like
+// sincos_d it has no large-argument path, which keeps the register
+// footprint of the float field small.
+NP_HD inline void sincos_f(float x, float& s, float& c) {
+    const float k = rintf(x * 0.63661977236758134f);                     // 2/pi
+    float r = fmaf(-k, 1.5707963705062866f, x);                           // float(pi/2)
+    r = fmaf(-k, -4.371138828673793e-08f, r);                             // pi/2 - float(pi/2)
+    r = fmaf(-k, -1.7151245100058819e-15f, r);                             // the rest, rounded
+    const float r2 = r * r;
+    float ps = 1.0f / 362880.0f;                    //  1/9!
+    ps = ps * r2 - 1.0f / 5040.0f;                  // -1/7!
+    ps = ps * r2 + 1.0f / 120.0f;                   //  1/5!
+    ps = ps * r2 - 1.0f / 6.0f;                     // -1/3!
+    const float sr = r + r * (r2 * ps);
+    float pc = 1.0f / 3628800.0f;                   //  1/10!
+    pc = pc * r2 - 1.0f / 40320.0f;                 // -1/8!
+    pc = pc * r2 + 1.0f / 720.0f;                   //  1/6!
+    pc = pc * r2 - 1.0f / 24.0f;                    // -1/4!
+    pc = pc * r2 + 0.5f;                            //  1/2!
+    const float cr = 1.0f - r2 * pc;
+    const int q = (int)k & 3;
+    s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
+    c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
+}
+
 }  // namespace np
