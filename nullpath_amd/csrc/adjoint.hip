@@ -619,13 +619,29 @@ __global__ void __launch_bounds__(kBuckets) adj_scan(Ctl* __restrict__ ctl) {
     }
 }
 
+// Each ray with work takes the next slot of its bucket.  Neighbouring rays
+// mostly share a key, so a wave claims the slots of all its lanes with one
+// key in one atomic (one per distinct key in the wave).
 __global__ void __launch_bounds__(256) adj_scatter(const AdjConst K, void* __restrict__ ws_base) {
     const int64_t nrays = (int64_t)K.W * K.H;
     const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
-    for (int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ray < nrays;
-         ray += (int64_t)gridDim.x * blockDim.x) {
-        const int key = ws.meta[5 * nrays + ray];
-        if (key >= 0) ws.meta[6 * nrays + atomicAdd(&ws.ctl->bucket[key], 1u)] = ray;
+    const uint64_t below = (1ull << lane_id()) - 1ull;
+    // the loop bound is uniform over the wave (whole blocks), so every lane
+    // takes part in the ballots
+    for (int64_t b0 = blockIdx.x * (int64_t)blockDim.x; b0 < nrays; b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ray = b0 + threadIdx.x;
+        const int key = ray < nrays ? ws.meta[5 * nrays + ray] : -1;
+        uint64_t pending = __ballot(key >= 0);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const int lk = __shfl(key, leader);
+            const uint64_t peers = __ballot(key == lk) & pending;
+            uint32_t base = 0;
+            if (lane_id() == leader) base = atomicAdd(&ws.ctl->bucket[lk], (uint32_t)__popcll(peers));
+            base = __shfl(base, leader);
+            if (key == lk) ws.meta[6 * nrays + base + (uint32_t)__popcll(peers & below)] = (int32_t)ray;
+            pending &= ~peers;
+        }
     }
 }
 

@@ -1,6 +1,6 @@
 // Operation count of the Carter-form field and its vector-Jacobian product
 // (nullpath_amd/csrc/np_carter.hpp) with a counting scalar type: add, sub,
-// mul, div and sqrt count one flop each; sin and cos are counted apart;
+// mul, div, reciprocal and sqrt count one flop each; sin and cos are counted apart;
 // max/min, negation and comparisons are not flops.  The adjoint benchmark's
 // per-step flop figures are built from these counts.
 #include <cmath>
@@ -32,6 +32,7 @@ inline void carter_sincos(Counted x, Counted& s, Counted& c) {
     c = Counted(std::cos(x.v));
 }
 inline Counted carter_fmax(Counted a, Counted b) { return Counted(a.v > b.v ? a.v : b.v); }
+inline Counted carter_rcp(Counted x) { return op(1.0 / x.v); }
 }  // namespace cnt
 
 #include "../../nullpath_amd/csrc/np_carter.hpp"

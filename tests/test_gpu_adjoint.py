@@ -7,6 +7,13 @@ contraction on the GPU vs separately rounded oracle arithmetic, amplified on
 near-critical rays); FP32C forward vs FP64 forward 1e-5 relative on 99% of
 disk pixels; gradients 1e-4 relative (per component, relative to the
 gradient vector's largest component).
+
+At the benchmark's full size (512x512, 4096 steps) the loss is not smooth
+in spin and theta_obs: a few hundred rays next to the critical curve change
+their crossings under tiny parameter changes, and central differences do not
+converge there.  The full-size test therefore compares the adjoint and
+per-pixel differences of the FP64 forward on the pixels where differences at
+two step sizes agree, and checks that those are nearly all of the disk.
 """
 import math
 
@@ -136,3 +143,57 @@ def test_autograd_function_matches_direct_backward(device):
     g_direct, _ = _adjoint_grad(n)
     g_torch, _ = _adjoint_grad(n, torch_function=True)
     np.testing.assert_allclose(g_torch, g_direct, rtol=1e-6, atol=0)
+
+
+def test_adjoint_full_size_smooth_pixels(device):
+    """512x512, 4096 steps, checkpoint every 64: the adjoint gradient of
+    sum over smooth pixels of (I - I_target)^2 (dLoss/dImage zeroed on the
+    others) against the same sum of per-pixel central differences of the
+    FP64 forward, h = 1e-6 (a pixel is smooth when the differences at h and
+    4h agree to 1e-5 relative in both spin and theta_obs); q and gain
+    (which do not move the rays) over all pixels."""
+    import torch
+
+    from nullpath_amd import adjoint as A
+    n = 512
+    base = _params(n)
+    assert base.n_steps == 4096 and base.ckpt_every == 64
+    print(f"[adjoint] full size {n}x{n}, {base.n_steps} steps: adjoint vs per-pixel FD", flush=True)
+    ws = A.Workspace(base)
+    T = _fp64_isum(A.with_params4(base, TARGET.values()))
+    S = _fp64_isum(A.with_params4(base, START.values()))
+    x0 = np.array(list(START.values()))
+
+    def fd(k, h):
+        xp, xm = x0.copy(), x0.copy()
+        xp[k] += h
+        xm[k] -= h
+        return (_fp64_isum(A.with_params4(base, xp)) - _fp64_isum(A.with_params4(base, xm))) / (2 * h)
+
+    d = {(k, h): fd(k, h) for k in range(4) for h in (1e-6, 4e-6)}
+    smooth = np.ones(n * n, bool)
+    for k in (0, 1):
+        a, b = d[(k, 1e-6)], d[(k, 4e-6)]
+        floor = 1e-9 * np.median(np.abs(a[a != 0]))
+        smooth &= np.abs(a - b) <= 1e-5 * np.maximum(np.abs(a), np.abs(b)) + floor
+    disk = S > 0
+    print(f"[adjoint] disk pixels {disk.sum()}, not smooth {(~smooth).sum()}", flush=True)
+    assert (~smooth).sum() <= 0.02 * disk.sum()
+    w = 2.0 * (S - T)
+    p = A.with_params4(base, START.values())
+    img = A.forward(p, A.PREC_FP32C, ws=ws)
+
+    def adjoint(mask):
+        g = np.where(mask, w, 0.0)
+        gi = torch.from_numpy(np.repeat(g[:, None], 3, 1) * WEIGHTS[None, :]).float().to(img.device)
+        out = A.backward(p, gi.reshape(-1).contiguous(), ws)
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    W2 = (WEIGHTS ** 2).sum()
+    g_s = adjoint(smooth)
+    ref_s = np.array([(w * d[(k, 1e-6)] * W2)[smooth].sum() for k in range(4)])
+    assert _rel(g_s[:2], ref_s[:2]) <= 1e-4, (g_s, ref_s)
+    g_all = adjoint(np.ones(n * n, bool))
+    ref_qg = np.array([(w * d[(k, 1e-6)] * W2).sum() for k in (2, 3)])
+    assert _rel(g_all[2:], ref_qg) <= 1e-5, (g_all, ref_qg)
