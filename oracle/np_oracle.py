@@ -455,6 +455,32 @@ def geor(u0, mu0, muf, a, l, q2, tpm, su=1.0, sm=1.0, uf_in=-1.0, tpr_in=0) -> d
     return {k: getattr(o, k) for k, _ in GeorOut._fields_ if k != "pad_"}
 
 
+class GkVars(C.Structure):
+    """orc_gk_vars: GEOKERR's variables shared along one geodesic."""
+    _fields_ = [("ncase", C.c_int32), ("pad_", C.c_int32)] + [
+        (k, C.c_double) for k in ("u1", "u2", "u3", "u4", "h1", "iu0", "rffu0", "rffu1", "i1mu", "i3mu",
+                                  "rffmu1", "rffmu2", "rffmu3", "rdc", "rjc", "tu01", "tu02", "tu03", "tu04",
+                                  "tmu1", "tmu3", "phimu1", "phimu3")]
+
+
+def gk_geor(v: GkVars, u0, uf, mu0, muf, a, l, q2, imu, tpm, tpr, su, sm, pht=0, firstpt=1) -> dict:
+    """orc_gk_geor: GEOR with the Fortran's by-reference arguments and its
+    FIRSTPT/PHT switches; v carries the shared variables between calls."""
+    L = _glib()
+    if not getattr(L, "_gk_geor_declared", False):
+        d, pd, pi = C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.orc_gk_geor.argtypes = [C.POINTER(GkVars), pd, pd, d, d, d, d, d, pd, C.c_int32, pi, d, d,
+                                  C.c_int, C.c_int, pi]
+        L.orc_gk_geor.restype = None
+        L._gk_geor_declared = True
+    U0, UF, IMU = C.c_double(u0), C.c_double(uf), C.c_double(imu)
+    TPR, W = C.c_int32(tpr), C.c_int32(0)
+    L.orc_gk_geor(C.byref(v), C.byref(U0), C.byref(UF), mu0, muf, a, l, q2, C.byref(IMU), int(tpm), C.byref(TPR),
+                  su, sm, int(pht), int(firstpt), C.byref(W))
+    return {"u0": U0.value, "uf": UF.value, "imu": IMU.value, "tpr": TPR.value, "warned": W.value,
+            "ncase": v.ncase, "iu0": v.iu0}
+
+
 def geor_batch(u0, mu0, muf, a, l, q2, tpm, su, sm, uf_in=None, nthreads=0):
     """orc_geor_batch over arrays of n: returns (uf, imu, tpr, ncase)."""
     arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (u0, mu0, muf, a, l, q2, su, sm)]

@@ -14,9 +14,10 @@ mathematics that does not restate it:
 * SNCNDN against mpmath's Jacobi elliptic functions;
 * ZROOTS against numpy.roots.
 
-Parity unpinned (only a Fortran build could pin it): the exact WARNING
-input edits of the forbidden-region branches and ZROOTS root ordering when
-two roots tie in real part.
+The forbidden-region WARNING edits and the equal-root cubic are pinned in
+test_oracle_geokerr_edges.py (Mino-time identity on the edited inputs).
+Parity unpinned (only a Fortran build could pin it): ZROOTS root ordering
+when two roots tie in real part.
 """
 import math
 

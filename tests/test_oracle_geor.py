@@ -18,9 +18,9 @@ Inputs are physical: M(mu0) >= 0, U(u0) >= 0, u0 below the horizon's
 u+, and uf on the side of u0 that su points to (before the first radial
 turning point, or after it for tpr = 1).
 
-Parity unpinned: the equal-root cubic branch (a measure-zero set of
-constants) and the forbidden-region input edits, which physical inputs do
-not reach.
+The equal-root cubic branch (a measure-zero set of constants), row 2 with
+FIRSTPT off and the forbidden-region input edits, which these physical
+inputs do not reach, are pinned in test_oracle_geokerr_edges.py.
 """
 import numpy as np
 import pytest
