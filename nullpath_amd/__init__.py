@@ -169,15 +169,23 @@ def carlson_batch(kind: str, x, y, z=None, p=None, out=None, stream=None):
     return out
 
 
-def geomu_camera_batch(alpha, beta, spin, mu0, uf, u0, stream=None):
-    """GEOMU from screen coordinates (np_geomu_camera_batch); float64 device
-    tensors in, (muf, tpm, ncase, iu) device tensors out."""
+def _geomu_outputs(n, device, out, count):
     import torch
+    if out is not None:
+        assert len(out) == count and all(o.numel() == n for o in out)
+        return out
+    muf = torch.empty(n, dtype=torch.float64, device=device)
+    tpm = torch.empty(n, dtype=torch.int32, device=device)
+    rest = [torch.empty_like(tpm)] + [torch.empty_like(muf)] + [torch.empty_like(tpm)] * (count - 4)
+    return (muf, tpm, *rest)
+
+
+def geomu_camera_batch(alpha, beta, spin, mu0, uf, u0, stream=None, out=None):
+    """GEOMU from screen coordinates (np_geomu_camera_batch); float64 device
+    tensors in, (muf, tpm, ncase, iu) device tensors out (written into `out`
+    when given: float64, int32, int32, float64 tensors of n)."""
     n = alpha.numel()
-    muf = torch.empty(n, dtype=torch.float64, device=alpha.device)
-    iu = torch.empty_like(muf)
-    tpm = torch.empty(n, dtype=torch.int32, device=alpha.device)
-    ncase = torch.empty_like(tpm)
+    muf, tpm, ncase, iu = _geomu_outputs(n, alpha.device, out, 4)
     code = lib().np_geomu_camera_batch(alpha.data_ptr(), beta.data_ptr(), spin.data_ptr(),
                                        mu0.data_ptr(), uf.data_ptr(), float(u0), n, muf.data_ptr(),
                                        tpm.data_ptr(), ncase.data_ptr(), iu.data_ptr(),
@@ -186,18 +194,13 @@ def geomu_camera_batch(alpha, beta, spin, mu0, uf, u0, stream=None):
     return muf, tpm, ncase, iu
 
 
-def geomu_camera_batch_mixed(alpha, beta, spin, mu0, uf, u0, stream=None):
+def geomu_camera_batch_mixed(alpha, beta, spin, mu0, uf, u0, stream=None, out=None):
     """GEOMU from screen coordinates with the quartic's roots in mixed
     precision (np_geomu_camera_batch_mixed); returns (muf, tpm, ncase, iu,
     fp64_roots) device tensors, fp64_roots = 1 where the FP64 root finder
-    ran."""
-    import torch
+    ran (written into `out` when given)."""
     n = alpha.numel()
-    muf = torch.empty(n, dtype=torch.float64, device=alpha.device)
-    iu = torch.empty_like(muf)
-    tpm = torch.empty(n, dtype=torch.int32, device=alpha.device)
-    ncase = torch.empty_like(tpm)
-    fb = torch.empty_like(tpm)
+    muf, tpm, ncase, iu, fb = _geomu_outputs(n, alpha.device, out, 5)
     code = lib().np_geomu_camera_batch_mixed(alpha.data_ptr(), beta.data_ptr(), spin.data_ptr(),
                                              mu0.data_ptr(), uf.data_ptr(), float(u0), n, muf.data_ptr(),
                                              tpm.data_ptr(), ncase.data_ptr(), iu.data_ptr(), fb.data_ptr(),

@@ -196,3 +196,33 @@ def test_geor_warning_edit(oracle, case):
         iu_q = _iu(a, l, q2, edited, o["uf"])
         assert abs(iu_q - imu_q) <= 1e-8 * imu_q, (su, o, iu_q, imu_q)
         assert (o["uf"] - float(edited)) * su > 0, (su, o)
+
+
+def test_geokerr_camera_invariants(oracle):
+    """(alpha, beta) on the observer's sky at mu0 -> (L, Q2) with E = 1
+    (Bardeen 1973): pinned by what a wrong mapping breaks rather than by its
+    formula: at a = 0 spherical symmetry makes the total impact parameter
+    L^2 + Q2 = alpha^2 + beta^2 for every mu0; an equatorial observer
+    (mu0 = 0) sees L = -alpha, Q2 = beta^2; in general the polar potential
+    at the observer, Theta(mu0) = Q2 + a^2 mu0^2 - L^2 mu0^2 / (1 - mu0^2),
+    is beta^2 and |L| = |alpha| sqrt(1 - mu0^2).  sm follows the sign of
+    beta; values below 1e-5 (1e-10 for Q2) are set to zero as the driver
+    does."""
+    rng = np.random.default_rng(7)
+    for _ in range(500):
+        al, be = rng.uniform(-12, 12, 2)
+        mu0 = rng.uniform(0.01, 0.99)
+        a = rng.uniform(0.05, 0.99)
+        l, q2, aa, su, sm = oracle.geokerr_camera(al, be, mu0, 0.0)
+        assert aa == 0.0 and su == 1.0
+        assert abs(l * l + q2 - (al * al + be * be)) <= 1e-12 * (al * al + be * be)
+        l, q2, aa, su, sm = oracle.geokerr_camera(al, be, 0.0, a)
+        assert l == -al and abs(q2 - be * be) <= 1e-15 * be * be and aa == a
+        l, q2, aa, su, sm = oracle.geokerr_camera(al, be, mu0, a)
+        theta = q2 + a * a * mu0 ** 2 - l * l * mu0 ** 2 / (1 - mu0 ** 2)
+        assert abs(theta - be * be) <= 1e-12 * max(be * be, 1.0)
+        assert abs(abs(l) - abs(al) * math.sqrt(1 - mu0 ** 2)) <= 1e-14 * abs(al)
+        assert np.sign(l) == -np.sign(al)
+        assert sm == (1.0 if be >= 0 else -1.0)
+    l, q2, aa, _, _ = oracle.geokerr_camera(5e-6, 1e-6, 0.3, 5e-6)
+    assert l == 0.0 and q2 == 0.0 and aa == 0.0
