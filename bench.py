@@ -579,6 +579,11 @@ def bench_geomu(n: int, cpu_rays: int = 0) -> dict:
     if GEOMU_FLOPS_PER_RAY is not None:
         res["roofline"] = roofline(GEOMU_FLOPS_PER_RAY * n, ms / 1e3, "fp64", "geomu_camera_kernel<false>")
         res["roofline"]["note"] = "flops per ray measured by the FP64 flop counter on active lanes"
+        # the same work priced the same way: the compared mode solves the same
+        # rays, its float root phase standing in for FP64 work
+        res["mixed_roots"]["roofline"] = roofline(GEOMU_FLOPS_PER_RAY * n, ms_mixed / 1e3, "fp64",
+                                                  "geomu_camera_kernel<true>")
+        res["mixed_roots"]["roofline"]["note"] = "the FP64 kernel's flops per ray, FP64 peak"
     if cpu_rays > 0:
         from oracle import np_oracle as O
         host = [x[:cpu_rays].cpu().numpy() for x in args]

@@ -530,21 +530,41 @@ __device__ __forceinline__ cplxf operator*(cplxf a, cplxf b) {
     return mkf(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 __device__ __forceinline__ cplxf operator*(float s, cplxf a) { return mkf(s * a.re, s * a.im); }
-__device__ __forceinline__ float cabs_f(cplxf a) { return hypotf(a.re, a.im); }
+// The float phase only seeds the FP64 polish, so on the device it uses the
+// hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, ~1 ulp)
+// instead of the IEEE division and sqrtf/hypotf sequences; a host build
+// (the CPU tests) uses the IEEE operations.  |z| as sqrt(re^2 + im^2): the
+// quartic's coefficients and roots here stay far from float's range limits.
+__device__ __forceinline__ float rcp_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+__device__ __forceinline__ float sqrt_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return std::sqrt(x);
+#endif
+}
+__device__ __forceinline__ float cabs_f(cplxf a) { return sqrt_f(a.re * a.re + a.im * a.im); }
 __device__ __forceinline__ cplxf operator/(cplxf a, cplxf b) {
     if (fabsf(b.re) >= fabsf(b.im)) {
-        const float r = b.im / b.re, d = b.re + b.im * r;
-        return mkf((a.re + a.im * r) / d, (a.im - a.re * r) / d);
+        const float r = b.im * rcp_f(b.re), id = rcp_f(b.re + b.im * r);
+        return mkf((a.re + a.im * r) * id, (a.im - a.re * r) * id);
     }
-    const float r = b.re / b.im, d = b.re * r + b.im;
-    return mkf((a.re * r + a.im) / d, (a.im * r - a.re) / d);
+    const float r = b.re * rcp_f(b.im), id = rcp_f(b.re * r + b.im);
+    return mkf((a.re * r + a.im) * id, (a.im * r - a.re) * id);
 }
 __device__ __forceinline__ cplxf csqrt_f(cplxf z) {
     if (z.re == 0.0f && z.im == 0.0f) return mkf(0.0f, z.im);
     const float m = cabs_f(z);
-    const float t = sqrtf(0.5f * (m + fabsf(z.re)));
-    if (z.re >= 0.0f) return mkf(t, z.im / (2.0f * t));
-    return mkf(fabsf(z.im) / (2.0f * t), copysignf(t, z.im));
+    const float t = sqrt_f(0.5f * (m + fabsf(z.re)));
+    const float h = 0.5f * rcp_f(t);
+    if (z.re >= 0.0f) return mkf(t, z.im * h);
+    return mkf(fabsf(z.im) * h, copysignf(t, z.im));
 }
 
 // LAGUER's iteration in float (same step control; float round-off bound).

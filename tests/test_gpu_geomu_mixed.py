@@ -6,8 +6,12 @@ to the oracle in test_gpu_geokerr) on the GEOKERR batch distribution
 u ~ U(0, u+), from u0 = 1e-4).
 
 The polished roots are FP64 roots; they may differ from the FP64 path's in
-the last bits, so the Table-1 case and mu turning points must agree and
-mu_f agree to 1e-10 (absolute; |mu| <= 1)."""
+the last bits.  Asserted: the Table-1 case and mu turning points agree on at
+least 99.99 % of rays, and mu_f to 1e-10 (absolute; |mu| <= 1) on at least
+99.99 % (the benchmark's 2^20 rays measure no disagreement at all); the rays
+handed to the FP64 root finder give the FP64 kernel's case exactly.  Against
+the oracle, the statements of the FP64 kernel's own oracle test: identical
+case, I_u to 1e-12, turning points on 99.99 %, mu_f to 1e-9 where they agree."""
 import numpy as np
 import pytest
 import torch
@@ -44,3 +48,19 @@ def test_mixed_roots_match_fp64_geomu(device, seed):
     # the fallback rays ran the FP64 root finder: identical results there
     sel = fb == 1
     assert np.array_equal(c64[sel], cm[sel])
+
+
+@pytest.mark.gpu
+def test_mixed_roots_match_oracle(oracle, device):
+    n = 1 << 16
+    args = _rays(n, 31)
+    host = [x.cpu().numpy() for x in args]
+    print(f"[gpu] np_geomu_camera_batch_mixed vs oracle n={n}", flush=True)
+    g_muf, g_tpm, g_nc, g_iu, fb = [x.cpu().numpy() for x in npa.geomu_camera_batch_mixed(*args, 1e-4)]
+    o_muf, o_tpm, o_nc, o_iu = oracle.geomu_batch(*host, 1e-4)
+    assert np.array_equal(g_nc, o_nc)
+    assert np.all(np.abs(g_iu - o_iu) <= 1e-12 * np.abs(o_iu) + 1e-15)
+    same_tpm = g_tpm == o_tpm
+    assert same_tpm.mean() > 0.9999
+    good = same_tpm & np.isfinite(o_muf)
+    assert np.all(np.abs(g_muf[good] - o_muf[good]) <= 1e-9), np.abs(g_muf[good] - o_muf[good]).max()
