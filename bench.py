@@ -276,6 +276,24 @@ def host_threads() -> int:
     return min(16, ncpu)
 
 
+# CPU baselines of the side workloads run about this long: whole passes over
+# a sample (or over the workload itself) until the time is reached.
+CPU_SAMPLE_S = 8.0
+
+
+def cpu_for(seconds: float, fn):
+    """Call fn(k) for k = 0, 1, ... (each call returns the units it did)
+    until `seconds` have passed; returns (units, calls, elapsed s)."""
+    t0 = time.perf_counter()
+    units, k = 0, 0
+    while True:
+        units += fn(k)
+        k += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            return units, k, dt
+
+
 def roofline(flops: float, seconds: float, prec: str, kernel: str) -> dict:
     peak = PEAK_TFLOPS[prec]
     achieved = flops / seconds / 1e12
@@ -358,7 +376,7 @@ def bench_adjoint(args) -> dict:
     }
     if args.cpu_pixels > 0:
         from oracle import np_oracle as O
-        m = 16384
+        m = 65536
         rng = np.random.default_rng(7)
         sel = rng.choice(nrays, size=m, replace=False)
         q = O.adjoint_params()
@@ -588,12 +606,15 @@ def bench_geomu(n: int, cpu_rays: int = 0) -> dict:
         from oracle import np_oracle as O
         host = [x[:cpu_rays].cpu().numpy() for x in args]
         th = host_threads()
-        t1 = time.perf_counter()
-        O.geomu_batch(*host, 1e-4, nthreads=th)
-        dt = time.perf_counter() - t1
-        res["cpu_baseline"] = {"value": cpu_rays / dt, "unit": "rays/s", "cores": th, "kind": "port",
-                               "sample": f"the first {cpu_rays} of the {n} rays ({dt:.1f} s), FP64 oracle GEOMU "
-                                         "restatement, OpenMP"}
+
+        def one(k):
+            O.geomu_batch(*host, 1e-4, nthreads=th)
+            return cpu_rays
+
+        done, calls, dt = cpu_for(CPU_SAMPLE_S, one)
+        res["cpu_baseline"] = {"value": done / dt, "unit": "rays/s", "cores": th, "kind": "port",
+                               "sample": f"the first {cpu_rays} of the {n} rays, {calls} passes ({dt:.1f} s), "
+                                         "FP64 oracle GEOMU restatement, OpenMP"}
     return res
 
 
@@ -653,12 +674,19 @@ def bench_geokerr(n: int, s: int, cpu_geodesics: int) -> dict:
         from oracle import np_oracle as O
         m = min(cpu_geodesics, n)
         threads = min(16, len(os.sched_getaffinity(0)))
-        t0 = time.perf_counter()
-        O.geokerr_eval_lambda_batch(u0[:m], mu0[:m], uf[:m], a[:m], lz[:m], q2[:m], np.zeros(m), 0.0, 0.1, s,
-                                    nthreads=threads)
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": m * s / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-                               "sample": f"first {m} of the {n} geodesics, {s} samples each"}
+
+        def one(k):
+            i0 = (k * m) % n
+            sl = slice(i0, min(i0 + m, n))
+            cnt = sl.stop - sl.start
+            O.geokerr_eval_lambda_batch(u0[sl], mu0[sl], uf[sl], a[sl], lz[sl], q2[sl], np.zeros(cnt), 0.0, 0.1,
+                                        s, nthreads=threads)
+            return cnt * s
+
+        done, calls, dt = cpu_for(CPU_SAMPLE_S, one)
+        res["cpu_baseline"] = {"value": done / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+                               "sample": f"{calls} chunks of {m} of the {n} geodesics in turn ({done // s} "
+                                         f"geodesics, {s} samples each, {dt:.1f} s)"}
     return res
 
 
@@ -719,16 +747,22 @@ def bench_geokerr_driver(n: int, cpu_geodesics: int) -> dict:
 
             from oracle import np_oracle as O
             m = min(cpu_geodesics, n)
-            g = np.zeros(m, O.GK_ARGS_DTYPE)
-            for k in ("u0", "uf", "uout", "mu0", "muf", "a", "l", "q2", "beta", "su", "sm", "tpm", "tpr"):
-                g[k] = rays[k][:m].cpu().numpy()
             threads = min(16, len(os.sched_getaffinity(0)))
-            t0 = time.perf_counter()
-            ref = O.geokerr_full_batch(opts, g, nthreads=threads)
-            dt = time.perf_counter() - t0
-            entry["cpu_baseline"] = {"value": int((ref["tpri"] >= 0).sum()) / dt, "unit": "samples/s",
-                                     "cores": threads, "kind": "port",
-                                     "sample": f"first {m} of the {n} geodesics"}
+            host = {k: rays[k].cpu().numpy() for k in ("u0", "uf", "uout", "mu0", "muf", "a", "l", "q2", "beta",
+                                                          "su", "sm", "tpm", "tpr")}
+
+            def one(k, opts=opts, host=host):
+                i0 = (k * m) % n
+                g = np.zeros(min(m, n - i0), O.GK_ARGS_DTYPE)
+                for key, v in host.items():
+                    g[key] = v[i0:i0 + len(g)]
+                ref = O.geokerr_full_batch(opts, g, nthreads=threads)
+                return int((ref["tpri"] >= 0).sum())
+
+            done, calls, dt = cpu_for(CPU_SAMPLE_S, one)
+            entry["cpu_baseline"] = {"value": done / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+                                     "sample": f"{calls} chunks of {m} of the {n} geodesics in turn ({done} samples, "
+                                               f"{dt:.1f} s)"}
         out[name] = entry
     out["workload"] = (f"GEOKERR driver (np_geokerr_batch), {n} camera geodesics from u0 = 1e-4: MUFILL "
                        "(NUP 512, NPTS 64, KEXT 2) and USEGEOR (NUP 512), PHIT on, FP64")
@@ -788,10 +822,10 @@ def main() -> int:
         return 0
     if args.only:
         cpu = args.cpu_pixels > 0
-        run = {"schw": lambda: bench_schwarzschild(100000, 20000 if cpu else 0),
+        run = {"schw": lambda: bench_schwarzschild(100000, 100000 if cpu else 0),
                "geomu": lambda: bench_geomu(1 << 20, (1 << 20) if cpu else 0),
                "adjoint": lambda: bench_adjoint(args),
-               "batch": lambda: bench_batch(2, 256 if cpu else 0)}[args.only]
+               "batch": lambda: bench_batch(2, 1024 if cpu else 0)}[args.only]
         print(json.dumps(run()))
         return 0
     if args.geokerr_only > 0:
@@ -876,11 +910,11 @@ def main() -> int:
         line["config"]["adjoint"] = bench_adjoint(args)
     if args.side_workloads:
         cpu = args.cpu_pixels > 0
-        line["config"]["uneven_batch"] = bench_batch(2, 256 if cpu else 0)
-        line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000, 20000 if cpu else 0)
+        line["config"]["uneven_batch"] = bench_batch(2, 1024 if cpu else 0)
+        line["config"]["schwarzschild_sweep"] = bench_schwarzschild(100000, 100000 if cpu else 0)
         line["config"]["geomu_batch"] = bench_geomu(1 << 20, (1 << 20) if cpu else 0)
         line["config"]["geokerr_eval_lambda"] = bench_geokerr(256, 4096, 64 if args.cpu_pixels > 0 else 0)
-        line["config"]["geokerr_driver"] = bench_geokerr_driver(131072, 32 if args.cpu_pixels > 0 else 0)
+        line["config"]["geokerr_driver"] = bench_geokerr_driver(131072, 64 if args.cpu_pixels > 0 else 0)
 
     if args.cpu_pixels > 0 and "fp64" in res:
         line["cpu_baseline"] = cpu_baseline(args, res["fp64"])

@@ -384,13 +384,17 @@ void np_free_lookup_table(GeodesicLookupTable* table);
  * launched from r0 with E = 1, L = b[i], p_r < 0 from H = 0.  A ray is
  * captured when r < r_capture (or the integrator stops at the horizon),
  * escaped when r > r0 with p_r > 0, class 2 when |b - 3 sqrt(3) M| <
- * ps_band, class 3 when max_steps run out.  Outputs per ray: class, total
- * unwrapped delta-phi, steps (optional); d_total_steps (optional, one u64)
- * accumulates the step count.
+ * ps_band (0 turns the band off), class 3 when max_steps run out.  Outputs
+ * per ray: class, total unwrapped delta-phi; optional: steps, the turning
+ * radius r_min (for escaped rays the largest root of r^3 - b^2 r + b^2 Rs,
+ * NaN otherwise) and the largest |H| over the accepted steps (H = 1/2
+ * g^{mu nu} p_mu p_nu, the accuracy plan's invariant monitor,
+ * docs/PLAN.md:38-43); d_total_steps (optional, one u64) accumulates the
+ * step count.
  */
 int np_schw_sweep(double M, double r0, double r_capture, double ps_band, int64_t max_steps,
                   const double* d_b, int64_t n, int32_t* d_cls, double* d_dphi, int64_t* d_steps,
-                  uint64_t* d_total_steps, void* stream);
+                  double* d_rmin, double* d_hmax, uint64_t* d_total_steps, void* stream);
 
 /*
  * Differentiable disk render (no counterpart in the reference, which has

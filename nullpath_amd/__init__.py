@@ -373,9 +373,10 @@ def geokerr_port_stream(u0, uf, mu0, a, lz, q2, lam0, dlam, s, symbol="geokerr_c
 
 
 def schw_sweep(b, M=10.0, r0=None, r_capture=None, ps_band=None, max_steps=2_000_000,
-               stream=None, total_steps=None):
+               stream=None, total_steps=None, diagnostics=False):
     """FP64 plain-method Schwarzschild sweep (np_schw_sweep) over a float64
-    device tensor of impact parameters; returns (cls, dphi, steps)."""
+    device tensor of impact parameters; returns (cls, dphi, steps), and
+    with diagnostics=True also (r_min, |H|max) per ray."""
     import torch
     r0 = 1000.0 * M if r0 is None else r0
     r_capture = 2.01 * M if r_capture is None else r_capture
@@ -384,11 +385,13 @@ def schw_sweep(b, M=10.0, r0=None, r_capture=None, ps_band=None, max_steps=2_000
     cls = torch.empty(n, dtype=torch.int32, device=b.device)
     dphi = torch.empty(n, dtype=torch.float64, device=b.device)
     steps = torch.empty(n, dtype=torch.int64, device=b.device)
+    rmin = torch.empty(n, dtype=torch.float64, device=b.device) if diagnostics else None
+    hmax = torch.empty(n, dtype=torch.float64, device=b.device) if diagnostics else None
     code = lib().np_schw_sweep(M, r0, r_capture, ps_band, max_steps, b.data_ptr(), n,
-                               cls.data_ptr(), dphi.data_ptr(), steps.data_ptr(),
+                               cls.data_ptr(), dphi.data_ptr(), steps.data_ptr(), _dptr(rmin), _dptr(hmax),
                                _dptr(total_steps), _stream_ptr(stream))
     check(code, "np_schw_sweep")
-    return cls, dphi, steps
+    return (cls, dphi, steps, rmin, hmax) if diagnostics else (cls, dphi, steps)
 
 
 def schw_precompute(b, mass=10.0, r_start=1000.0, stream=None):

@@ -147,7 +147,7 @@ def _declare(L: C.CDLL) -> None:
     f = C.c_float
     L.np_schw_precompute.argtypes = [f, f, vp, i64, vp, vp, vp, vp]
     L.np_schw_precompute.restype = i32
-    L.np_schw_sweep.argtypes = [d, d, d, d, i64, vp, i64, vp, vp, vp, vp, vp]
+    L.np_schw_sweep.argtypes = [d, d, d, d, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp]
     L.np_schw_sweep.restype = i32
     L.np_precompute_geodesics.argtypes = [f, i32, C.POINTER(LookupTable)]
     L.np_precompute_geodesics.restype = i32

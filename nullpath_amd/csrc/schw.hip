@@ -20,6 +20,7 @@
 #include "bh_types.h"
 #include "np_device.hpp"
 #include "np_kernels.hpp"
+#include "np_math.hpp"
 #include "nullpath.h"
 
 namespace np {
@@ -166,6 +167,9 @@ struct GD {
     double r, th, ph, pr, pth;   // t is never read; p_t, p_phi are constants
 };
 
+// The right-hand side of bh_schwarzschild.h:124-295 in FP64: the same
+// terms, with the powers of 1/r, 1/A and 1/sin^2 formed from one reciprocal
+// each instead of a division per term, and the polynomial sincos_d.
 __device__ __forceinline__ GD schw_rhs_d(const GD& s, double Rs, double pt, double pph) {
     GD d;
     const double r = s.r;
@@ -174,22 +178,51 @@ __device__ __forceinline__ GD schw_rhs_d(const GD& s, double Rs, double pt, doub
         return d;
     }
     double sin_t, cos_t;
-    sincos(s.th, &sin_t, &cos_t);
-    const double r2 = r * r;
-    const double r3 = r2 * r;
-    const double A = fmax(1e-6, 1.0 - Rs / r);
+    sincos_d(s.th, sin_t, cos_t);
+    const double ir = 1.0 / r;
+    const double ir2 = ir * ir;
+    const double ir3 = ir2 * ir;
+    const double A = fmax(1e-6, 1.0 - Rs * ir);
+    const double iA = 1.0 / A;
     d.r = A * s.pr;
-    d.th = s.pth / r2;
+    d.th = s.pth * ir2;
     const double sin2 = fmax(1e-6, sin_t * sin_t);
-    d.ph = pph / (r2 * sin2);
-    const double dgtt = Rs / (r2 * A * A);
-    const double dgrr = Rs / r2;
-    const double dgthth = -2.0 / r3;
-    const double dgphph = -2.0 / (r3 * sin2);
+    const double is2 = 1.0 / sin2;
+    d.ph = pph * ir2 * is2;
+    const double dgtt = Rs * ir2 * iA * iA;
+    const double dgrr = Rs * ir2;
+    const double dgthth = -2.0 * ir3;
+    const double dgphph = -2.0 * ir3 * is2;
     d.pr = -0.5 * (dgtt * pt * pt + dgrr * s.pr * s.pr + dgthth * s.pth * s.pth + dgphph * pph * pph);
     const double sin_safe = (fabs(sin_t) < 1e-6) ? (sin_t >= 0.0 ? 1e-6 : -1e-6) : sin_t;
-    d.pth = (cos_t / (r2 * sin2 * sin_safe)) * pph * pph;
+    d.pth = cos_t * ir2 * is2 / sin_safe * pph * pph;
     return d;
+}
+
+// H = 1/2 g^{mu nu} p_mu p_nu of the Schwarzschild metric at the state.
+__device__ __forceinline__ double schw_hamiltonian_d(const GD& s, double Rs, double pt, double pph) {
+    double sin_t, cos_t;
+    sincos_d(s.th, sin_t, cos_t);
+    const double ir2 = 1.0 / (s.r * s.r);
+    const double A = 1.0 - Rs / s.r;
+    return 0.5 * (-pt * pt / A + A * s.pr * s.pr + ir2 * s.pth * s.pth + ir2 * pph * pph / (sin_t * sin_t));
+}
+
+// Turning radius of an equatorial photon with impact parameter b: the
+// largest root of r^3 - b^2 r + b^2 Rs = 0 (exists for b >= b_crit =
+// (3 sqrt 3 / 2) Rs), by the trigonometric form of the cubic's roots,
+// r = (2b/sqrt 3) cos(arccos(-(3 sqrt 3 / 2) Rs / b) / 3), polished by two
+// Newton steps; NaN below b_crit.
+__device__ __forceinline__ double schw_turning_radius(double b, double Rs) {
+    const double x = -2.598076211353316 * Rs / b;   // -(3 sqrt 3 / 2) Rs / b
+    if (!(x >= -1.0)) return NAN;
+    double r = 1.1547005383792515 * b * cos(acos(x) / 3.0);   // 2 / sqrt 3
+    for (int i = 0; i < 2; ++i) {
+        const double f = (r * r - b * b) * r + b * b * Rs;
+        const double fp = 3.0 * r * r - b * b;
+        if (fp != 0.0) r -= f / fp;
+    }
+    return r;
 }
 
 __device__ __forceinline__ GD axpy_d(const GD& y, double c, const GD& k) {
@@ -199,16 +232,20 @@ __device__ __forceinline__ GD axpy_d(const GD& y, double c, const GD& k) {
     return o;
 }
 
+// DIAG: also the turning radius and the largest |H| per ray (a separate
+// instantiation, so the plain sweep keeps its register budget).
+template <bool DIAG>
 __global__ void __launch_bounds__(256)
 schw_sweep_kernel(const double M, const double r0, const double r_cap, const double ps_band,
                   const int64_t max_steps, const double* __restrict__ b_in, const int64_t n,
                   int32_t* __restrict__ cls_out, double* __restrict__ dphi_out,
-                  int64_t* __restrict__ steps_out, unsigned long long* __restrict__ work,
+                  int64_t* __restrict__ steps_out, double* __restrict__ rmin_out,
+                  double* __restrict__ hmax_out, unsigned long long* __restrict__ work,
                   uint64_t* __restrict__ total_steps) {
     const double Rs = 2.0 * M;
     const double pt = -1.0;
     GD s;
-    double pph = 0.0, b = 0.0, total = 0.0, prev_phi = 0.0;
+    double pph = 0.0, b = 0.0, total = 0.0, prev_phi = 0.0, hmax = 0.0;
     int64_t step = 0;
     uint64_t idx = 0, lane_steps = 0;
     bool has = false, done = false;
@@ -227,6 +264,7 @@ schw_sweep_kernel(const double M, const double r0, const double r_cap, const dou
                     s.pr = -sqrt(fmax(0.0, pr2));
                     total = 0.0;
                     prev_phi = 0.0;
+                    hmax = 0.0;
                     step = 0;
                     has = true;
                 } else {
@@ -256,6 +294,9 @@ schw_sweep_kernel(const double M, const double r0, const double r_cap, const dou
             total += dphi;
             prev_phi = s.ph;
             step += 1;
+            if constexpr (DIAG) {
+                if (hmax_out && ok) hmax = fmax(hmax, fabs(schw_hamiltonian_d(s, Rs, pt, pph)));
+            }
             int st = -1;
             if (!ok || s.r < r_cap) st = 1;
             else if (s.r > r0 && s.pr > 0.0) st = 0;
@@ -265,6 +306,10 @@ schw_sweep_kernel(const double M, const double r0, const double r_cap, const dou
                 cls_out[idx] = st;
                 dphi_out[idx] = total;
                 if (steps_out) steps_out[idx] = step;
+                if constexpr (DIAG) {
+                    if (rmin_out) rmin_out[idx] = st == 0 ? schw_turning_radius(b, Rs) : NAN;
+                    if (hmax_out) hmax_out[idx] = hmax;
+                }
                 lane_steps += (uint64_t)step;
                 has = false;
             }
@@ -300,7 +345,7 @@ int np_schw_precompute(float mass, float r_start, const float* d_b, int64_t n, f
 
 int np_schw_sweep(double M, double r0, double r_capture, double ps_band, int64_t max_steps,
                   const double* d_b, int64_t n, int32_t* d_cls, double* d_dphi, int64_t* d_steps,
-                  uint64_t* d_total_steps, void* stream) {
+                  double* d_rmin, double* d_hmax, uint64_t* d_total_steps, void* stream) {
     if (n < 0 || !(M > 0.0) || !(r0 > 2.0 * M) || max_steps <= 0 ||
         (n > 0 && (!d_b || !d_cls || !d_dphi)))
         return NP_EINVAL;
@@ -308,14 +353,19 @@ int np_schw_sweep(double M, double r0, double r_capture, double ps_band, int64_t
     unsigned long long* slot = nullptr;
     int err = np::take_work_slot((hipStream_t)stream, &slot);
     if (err != NP_OK) return err;
+    const bool diag = d_rmin || d_hmax;
+    const void* fn = diag ? (const void*)np::schw_sweep_kernel<true> : (const void*)np::schw_sweep_kernel<false>;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, np::schw_sweep_kernel, 256, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess || per_cu <= 0)
         per_cu = 1;
-    hipLaunchKernelGGL(np::schw_sweep_kernel, dim3(np::grid_for(n, per_cu)), dim3(256), 0,
-                       (hipStream_t)stream, M, r0, r_capture, ps_band, max_steps, d_b, n, d_cls,
-                       d_dphi, d_steps, slot, d_total_steps);
+    if (diag)
+        hipLaunchKernelGGL(np::schw_sweep_kernel<true>, dim3(np::grid_for(n, per_cu)), dim3(256), 0,
+                           (hipStream_t)stream, M, r0, r_capture, ps_band, max_steps, d_b, n, d_cls,
+                           d_dphi, d_steps, d_rmin, d_hmax, slot, d_total_steps);
+    else
+        hipLaunchKernelGGL(np::schw_sweep_kernel<false>, dim3(np::grid_for(n, per_cu)), dim3(256), 0,
+                           (hipStream_t)stream, M, r0, r_capture, ps_band, max_steps, d_b, n, d_cls,
+                           d_dphi, d_steps, d_rmin, d_hmax, slot, d_total_steps);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
