@@ -13,7 +13,7 @@
 // (np_pass_graph_*), and only checkpoints copy the image back.
 //
 // Additive flags (defaults reproduce the reference):
-//   --precision fp64|fp32c    --rtol X --atol X    --max-steps N    --seed N
+//   --precision fp64|fp32c|fp64-carter    --rtol X --atol X    --max-steps N    --seed N
 //   --disk-rin isco|X         --disk-rout X        (X in units of M)
 //   --emissivity-index q      (emissivity (r/M)^-q; default r^-2 absolute)
 //   --raw-out FILE            (linear float32 RGB at the end, accum / spp, row-major)
@@ -195,9 +195,9 @@ const std::vector<Rule>& rules() {
          "Invalid FOV; expected (0, 179) degrees."},
         {[](const Settings& s) {
              const std::string p = folded(s.ext.precision);
-             return p == "fp64" || p == "fp32c";
+             return p == "fp64" || p == "fp32c" || p == "fp64-carter";
          },
-         "Invalid --precision; expected fp64 or fp32c."},
+         "Invalid --precision; expected fp64, fp32c or fp64-carter."},
         {[](const Settings& s) {
              const std::string c = folded(s.ext.crossing);
              return c == "linear" || c == "hermite";
@@ -455,7 +455,8 @@ np_render_params to_params(Settings& s) {
     p.check_inv = s.look.invariants ? 1 : 0;
     // extensions
     p.max_steps = s.ext.max_steps;
-    p.precision = folded(s.ext.precision) == "fp32c" ? NP_PREC_FP32C : NP_PREC_FP64;
+    const std::string prec = folded(s.ext.precision);
+    p.precision = prec == "fp32c" ? NP_PREC_FP32C : prec == "fp64-carter" ? NP_PREC_FP64_CARTER : NP_PREC_FP64;
     if (s.ext.rtol > 0.0) p.rtol = s.ext.rtol;
     if (s.ext.atol >= 0.0) p.atol = s.ext.atol;
     p.seed = s.ext.seed;

@@ -28,7 +28,7 @@ def run(*args, cwd=None):
     (["--spin", "-1.5"], "Spin must satisfy |a/M| < 1."),
     (["--fov", "179"], "Invalid FOV; expected (0, 179) degrees."),
     (["--fov", "0"], "Invalid FOV; expected (0, 179) degrees."),
-    (["--precision", "fp16"], "Invalid --precision; expected fp64 or fp32c."),
+    (["--precision", "fp16"], "Invalid --precision; expected fp64, fp32c or fp64-carter."),
     (["--crossing", "cubic"], "Invalid --crossing; expected linear or hermite."),
     (["--resume", "s.f32", "--frames", "2"], "--resume applies to single-frame renders."),
     # validation order: the first failing rule reports
