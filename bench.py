@@ -354,12 +354,12 @@ def bench_adjoint(args) -> dict:
     # the last counted crossing), from the workspace's per-ray records
     nrays = n * n
     nck = (base.n_steps + base.ckpt_every - 1) // base.ckpt_every
-    meta = ws.buf[nck * 8 * nrays * 4:(nck * 8 + 6) * nrays * 4].view(torch.int32).view(6, nrays).cpu().numpy()
-    term, ncross = meta[0].astype(np.int64), meta[1]
-    last = np.where(ncross > 0, meta[1 + np.clip(ncross, 1, 3), np.arange(nrays)], -1)
-    w = (gimg.view(nrays, 3).double() * torch.tensor([1.0, 0.75, 0.45], dtype=torch.float64,
-                                                        device=gimg.device)).sum(dim=1).cpu().numpy()
-    bsteps = np.where((ncross > 0) & (w != 0.0), last + 1, 0).sum()
+    # workspace rows: last step, crossings, crossing steps x3, work key, work
+    # list, crossings up to the last contributing one (written by the backward)
+    meta = ws.buf[nck * 8 * nrays * 4:(nck * 8 + 8) * nrays * 4].view(torch.int32).view(8, nrays).cpu().numpy()
+    term, live = meta[0].astype(np.int64), meta[7]
+    last = np.where(live > 0, meta[1 + np.clip(live, 1, 3), np.arange(nrays)], -1)
+    bsteps = np.where((live > 0) & (meta[5] >= 0), last + 1, 0).sum()
     fl_f, fl_b = float(term.sum()) * ADJ_FWD_STEP, float(bsteps) * ADJ_BWD_STEP
     res = {
         "workload": f"{n}x{n}, 4096 fixed RK4 steps/ray FP32-comp, checkpoint every 64",

@@ -328,8 +328,9 @@ __device__ __forceinline__ bool crosses(const AdjConst& K, double th0, double th
 // Workspace layout (SoA, one column per ray or per work-list slot; every
 // section starts on a 256-byte boundary):
 //   ck   [nck][8][nrays] float     checkpoints (hi, lo of r, theta, p_r, p_theta)
-//   meta [7][nrays] int32          term step, crossings, crossing steps x3,
-//                                  work key (-1: no adjoint work), work list
+//   meta [8][nrays] int32          term step, crossings, crossing steps x3,
+//                                  work key (-1: no adjoint work), work list,
+//                                  crossings up to the last one that contributes
 //   xs   [3][4][nrays] double      per crossing: r and theta - pi/2 of the two
 //                                  states around it (r0, r1, d0, d1)
 //   co   [8][nrays] double         initial-state partials d(p_r, p_theta, E, L)/d(a, theta_obs)
@@ -370,7 +371,7 @@ __host__ __device__ inline WSOffsets ws_offsets(int64_t nrays, int nck, int ck) 
     WSOffsets o;
     o.ck = 0;
     o.meta = ws_align(o.ck + (size_t)nck * 8 * n * sizeof(float));
-    o.xs = ws_align(o.meta + 7 * n * sizeof(int32_t));
+    o.xs = ws_align(o.meta + 8 * n * sizeof(int32_t));
     o.co = ws_align(o.xs + 12 * n * sizeof(double));
     o.el = ws_align(o.co + 8 * n * sizeof(double));
     o.inj = ws_align(o.el + 2 * n * sizeof(float));
@@ -539,7 +540,7 @@ adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws
          ray += (int64_t)gridDim.x * blockDim.x) {
         const int ncross = ws.meta[1 * nrays + ray];
         const double wpix = 1.0 * gimg[3 * ray + 0] + 0.75 * gimg[3 * ray + 1] + 0.45 * gimg[3 * ray + 2];
-        int key = -1;
+        int key = -1, nlive = 0;
         if (ncross > 0 && wpix != 0.0) {
             const int x = (int)(ray % K.W), y = (int)(ray / K.W);
             float n_r, n_th, n_ph;
@@ -565,6 +566,10 @@ adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws
                 g_a += wpix * C.a;
                 g_q += wpix * (-lr) * C.v;
                 g_gain += wpix * C.v / K.gain;
+                // outside the disk window the term and all its partials are
+                // exactly zero: the adjoint walk starts at the last crossing
+                // that contributes
+                if (C.v != 0.0 || C.r != 0.0 || C.E != 0.0 || C.L != 0.0 || C.a != 0.0) nlive = c + 1;
             }
             g_a += gE_x * in.dE[0] + gL_x * in.dL[0];
             g_th += gE_x * in.dE[1] + gL_x * in.dL[1];
@@ -575,11 +580,14 @@ adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws
             co[6 * nrays] = in.dL[0]; co[7 * nrays] = in.dL[1];
             ws.el[0 * nrays + ray] = (float)in.E;
             ws.el[1 * nrays + ray] = (float)in.L;
-            const int n_last = ws.meta[(1 + ncross) * nrays + ray];
-            key = work_key(n_last / K.ck + 1, K.nck);
-            atomicAdd(&hist[key], 1u);
+            if (nlive > 0) {
+                const int n_last = ws.meta[(1 + nlive) * nrays + ray];
+                key = work_key(n_last / K.ck + 1, K.nck);
+                atomicAdd(&hist[key], 1u);
+            }
         }
         ws.meta[5 * nrays + ray] = key;
+        ws.meta[7 * nrays + ray] = nlive;
     }
     for (int off = 32; off > 0; off >>= 1) {
         g_a += __shfl_xor(g_a, off);
@@ -664,7 +672,7 @@ adj_backward_f32(const AdjConst K, void* __restrict__ ws_base, double* __restric
         const int64_t i = (int64_t)base + lane_id();
         if (i >= total) continue;
         const int64_t ray = ws.meta[6 * nrays + i];
-        const int ncross = ws.meta[1 * nrays + ray];
+        const int ncross = ws.meta[7 * nrays + ray];
         int cs[3];
         cs[0] = ws.meta[2 * nrays + ray];
         cs[1] = ws.meta[3 * nrays + ray];
