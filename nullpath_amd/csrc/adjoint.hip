@@ -5,7 +5,8 @@
 // phi = 0) with the renderer's jittered pixel directions (sample 0); RK4 of
 // the Carter-form flow in the parameter s, dlambda = r ds (np_carter.hpp),
 // with constant step h for n_steps steps (a ray freezes when it reaches the
-// horizon + 1e-3, a pole, or r > r_cam + 20 after step 10); every
+// horizon + 1e-3, a pole, or r > r_cam + 20 after step 10; the FP32C
+// forward also once it moves outward beyond max(disk_rout, 5)); every
 // equatorial crossing on a step that does not end at the horizon (at most
 // max_orders, r interpolated linearly in theta between steps as
 // render.cu:211-226 does) adds
@@ -45,7 +46,7 @@ namespace {
 constexpr int kTetrad = 10;   // g_tt g_tphi g_phiphi g_rr g_thth ct0 cph0 cph_ph cr cth
 
 struct AdjConst {
-    double a, th, rp, rm, r_cam, r_far, h, rout, ew, q, gain;
+    double a, th, rp, rm, r_cam, r_far, h, rout, ew, q, gain, r_free;
     double T[kTetrad], dT_da[kTetrad], dT_dth[kTetrad];
     float af, rpf, rmf, hf, aspect, tan_half;
     uint32_t salt;
@@ -100,6 +101,7 @@ int make_adj(const np_adjoint_params* p, AdjConst* K) {
     K->r_far = p->r_cam + 20.0;
     K->h = p->h;
     K->rout = p->disk_rout;
+    K->r_free = std::fmax(p->disk_rout, 5.0);
     K->ew = p->edge_width;
     K->q = p->emiss_q;
     K->gain = p->gain;
@@ -428,7 +430,13 @@ adj_forward_f32(const AdjConst K, float* __restrict__ image, void* __restrict__ 
                 cs[ncross++] = n;
             }
             s = t;
-            if (frozen(K, r1, ffp_d(t.th), n)) {
+            // Also stop once the ray moves outward beyond the disk's outer
+            // edge (r_free = max(disk_rout, 5 M)): outside the photon
+            // orbits (r <= 4 M) an outgoing ray has
+            // no radial turning point ahead (dr/ds has the sign of p_r), so
+            // every later crossing lies where the disk window and all its
+            // partials are zero and adds nothing to the image or the adjoint.
+            if (frozen(K, r1, ffp_d(t.th), n) || (r1 > K.r_free && t.pr.hi > 0.0f)) {
                 term = n + 1;
                 break;
             }
