@@ -447,9 +447,9 @@ static inline void np_adjoint_params_default(np_adjoint_params* p) {
 }
 
 /* Device workspace bytes np_adjoint_forward (FP32C) and np_adjoint_backward
- * need (checkpoints, crossings, the backward's work list and segment
- * recompute), or -1 for invalid parameters.  About 3.3 KB per pixel at the
- * defaults. */
+ * need (checkpoints, crossings, the backward's work list, the chunk maps
+ * of split rays and segment recompute), or -1 for invalid parameters.
+ * About 4.1 KB per pixel at the defaults. */
 int64_t np_adjoint_workspace_bytes(const np_adjoint_params* p);
 
 /* Forward render into d_image (float RGB, width*height*3).  precision

@@ -327,11 +327,11 @@ __device__ __forceinline__ bool crosses(const AdjConst& K, double th0, double th
     return ((d0 > 0 && d1 < 0) || (d0 < 0 && d1 > 0)) && r1 > K.rp + 1e-3;
 }
 
-// Workspace layout (SoA, one column per ray or per work-list slot; every
-// section starts on a 256-byte boundary):
+// Workspace layout (SoA, one column per ray, work-list slot or backward
+// lane; every section starts on a 256-byte boundary):
 //   ck   [nck][8][nrays] float     checkpoints (hi, lo of r, theta, p_r, p_theta)
 //   meta [8][nrays] int32          term step, crossings, crossing steps x3,
-//                                  work key (-1: no adjoint work), work list,
+//                                  work key (-1: no adjoint work), chunk count,
 //                                  crossings up to the last one that contributes
 //   xs   [3][4][nrays] double      per crossing: r and theta - pi/2 of the two
 //                                  states around it (r0, r1, d0, d1)
@@ -339,10 +339,30 @@ __device__ __forceinline__ bool crosses(const AdjConst& K, double th0, double th
 //   el   [2][nrays] float          E, L as the float stepping uses them
 //   inj  [3][4][nrays] float       per crossing: adjoint injected into the later
 //                                  state (r, theta) and into the earlier one
-//   scr  [ck+1][4][nrays] float    backward recompute (float stage inputs),
-//                                  one column per work-list slot
+//   list [2][cap] int32            work list: ray, task (chunk << 3 | mode)
+//   xo   [nrays][kmax][5][8] float per chunk of a split ray: the adjoint at its
+//                                  lower end and the E, L, a sensitivities
+//   scr  [ck+1][4][lanes] float    backward recompute (float stage inputs),
+//                                  one column per backward lane
 //   ctl  uint32 bucket[kBuckets], total, pad; uint64 fetch
 constexpr int kBuckets = 512;
+// A ray that walks back more than kChunkSegs checkpoint segments is split:
+// its top kChunkSegs segments run as one task, the segments below in chunks
+// of at most kChunkSegs, each as 5 tasks (the chunk's response to the four
+// unit adjoints at its upper end and to its own crossings), chained
+// afterwards.  The split bounds the longest serial walk.
+constexpr int kChunkSegs = 24;
+
+__host__ __device__ inline int kmax_chunks(int nck) {
+    return nck <= kChunkSegs ? 1 : 1 + (nck - kChunkSegs + kChunkSegs - 1) / kChunkSegs;
+}
+__host__ __device__ inline int64_t list_cap(int64_t nrays, int nck) {
+    return nrays * (1 + 5 * (int64_t)(kmax_chunks(nck) - 1));
+}
+__host__ __device__ inline int64_t scr_lanes(int64_t nrays) {
+    const int64_t n = nrays < 256 ? 256 : nrays;
+    return (n + 255) / 256 * 256;
+}
 
 struct Ctl {
     uint32_t bucket[kBuckets];
@@ -357,13 +377,16 @@ struct WS {
     double* co;
     float* el;
     float* inj;
+    int32_t* list;
+    float* xo;
     float* scr;
     Ctl* ctl;
+    int64_t cap, lanes;
 };
 
 // byte offsets of the sections and the total size
 struct WSOffsets {
-    size_t ck, meta, xs, co, el, inj, scr, ctl, end;
+    size_t ck, meta, xs, co, el, inj, list, xo, scr, ctl, end;
 };
 
 __host__ __device__ inline size_t ws_align(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -377,8 +400,11 @@ __host__ __device__ inline WSOffsets ws_offsets(int64_t nrays, int nck, int ck) 
     o.co = ws_align(o.xs + 12 * n * sizeof(double));
     o.el = ws_align(o.co + 8 * n * sizeof(double));
     o.inj = ws_align(o.el + 2 * n * sizeof(float));
-    o.scr = ws_align(o.inj + 12 * n * sizeof(float));
-    o.ctl = ws_align(o.scr + (size_t)(ck + 1) * 4 * n * sizeof(float));
+    o.list = ws_align(o.inj + 12 * n * sizeof(float));
+    o.xo = ws_align(o.list + 2 * (size_t)list_cap(nrays, nck) * sizeof(int32_t));
+    const size_t kx = kmax_chunks(nck) > 1 ? (size_t)kmax_chunks(nck) : 0;
+    o.scr = ws_align(o.xo + n * kx * 5 * 8 * sizeof(float));
+    o.ctl = ws_align(o.scr + (size_t)(ck + 1) * 4 * (size_t)scr_lanes(nrays) * sizeof(float));
     o.end = o.ctl + sizeof(Ctl);
     return o;
 }
@@ -387,7 +413,8 @@ __host__ __device__ inline WS ws_layout(void* base, int64_t nrays, int nck, int 
     const WSOffsets o = ws_offsets(nrays, nck, ck);
     char* p = (char*)base;
     return WS{(float*)(p + o.ck), (int32_t*)(p + o.meta), (double*)(p + o.xs), (double*)(p + o.co),
-              (float*)(p + o.el), (float*)(p + o.inj), (float*)(p + o.scr), (Ctl*)(p + o.ctl)};
+              (float*)(p + o.el), (float*)(p + o.inj), (int32_t*)(p + o.list), (float*)(p + o.xo),
+              (float*)(p + o.scr), (Ctl*)(p + o.ctl), list_cap(nrays, nck), scr_lanes(nrays)};
 }
 
 int64_t ws_bytes(int64_t nrays, int nck, int ck) { return (int64_t)ws_offsets(nrays, nck, ck).end; }
@@ -518,10 +545,36 @@ __device__ __forceinline__ CState<float> step_vjp(const AdjConst& K, const CStat
     return out;
 }
 
-// Work key of a ray that walks back `segs` checkpoint segments: the
+// Work key of a task that walks back `segs` checkpoint segments: the
 // segment count itself when it fits the buckets, else scaled onto them.
 __device__ __forceinline__ int work_key(int segs, int nck) {
     return nck <= kBuckets ? segs - 1 : (int)((int64_t)(segs - 1) * kBuckets / nck);
+}
+
+// Task modes on the work list (low 3 bits of the task word; the chunk index
+// above them): 0-3 a lower chunk's response to the unit adjoint e_m at its
+// upper end, 4 its response to its own crossings, 5 the top chunk of a split
+// ray, 6 a whole ray.
+constexpr int kModeAffine = 4, kModeTop = 5, kModeWhole = 6;
+
+// Chunks of a ray that walks back S = n_last / ck + 1 segments: chunk 0 is
+// the top min(S, kChunkSegs) segments (from step n_last down), chunks
+// 1.. the segments below in runs of at most kChunkSegs, upper ones first.
+__device__ __forceinline__ int n_chunks(int S) {
+    return S <= kChunkSegs ? 1 : 1 + (S - kChunkSegs + kChunkSegs - 1) / kChunkSegs;
+}
+struct Chunk {
+    int n_hi, seg_lo;   // highest step walked, lowest segment
+};
+__device__ __forceinline__ Chunk chunk_of(int n_last, int ck, int j) {
+    const int S = n_last / ck + 1;
+    if (j == 0) return Chunk{n_last, S > kChunkSegs ? S - kChunkSegs : 0};
+    const int hi = S - kChunkSegs - (j - 1) * kChunkSegs;
+    return Chunk{hi * ck - 1, hi - kChunkSegs > 0 ? hi - kChunkSegs : 0};
+}
+__device__ __forceinline__ int chunk_segs(int n_last, int ck, int j) {
+    const Chunk c = chunk_of(n_last, ck, j);
+    return c.n_hi / ck + 1 - c.seg_lo;
 }
 
 __global__ void __launch_bounds__(kBuckets) adj_clear(double* __restrict__ grad4, Ctl* __restrict__ ctl) {
@@ -533,8 +586,8 @@ __global__ void __launch_bounds__(kBuckets) adj_clear(double* __restrict__ grad4
 // adjoint state (their partials in r, theta of the two bracketing states,
 // scaled by the pixel weight, for the backward to inject; their E, L, a,
 // q, gain partials folded into the gradient here), the initial-state
-// partials the backward contracts with at the end, and the ray's work key
-// counted in the bucket histogram.
+// partials the backward contracts with at the end, and the ray's tasks
+// counted in the bucket histogram by length.
 __global__ void __launch_bounds__(256)
 adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws_base,
          double* __restrict__ grad4) {
@@ -548,7 +601,7 @@ adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws
          ray += (int64_t)gridDim.x * blockDim.x) {
         const int ncross = ws.meta[1 * nrays + ray];
         const double wpix = 1.0 * gimg[3 * ray + 0] + 0.75 * gimg[3 * ray + 1] + 0.45 * gimg[3 * ray + 2];
-        int key = -1, nlive = 0;
+        int key = -1, nlive = 0, nch = 0;
         if (ncross > 0 && wpix != 0.0) {
             const int x = (int)(ray % K.W), y = (int)(ray / K.W);
             float n_r, n_th, n_ph;
@@ -590,11 +643,14 @@ adj_prep(const AdjConst K, const float* __restrict__ gimg, void* __restrict__ ws
             ws.el[1 * nrays + ray] = (float)in.L;
             if (nlive > 0) {
                 const int n_last = ws.meta[(1 + nlive) * nrays + ray];
-                key = work_key(n_last / K.ck + 1, K.nck);
+                nch = n_chunks(n_last / K.ck + 1);
+                key = work_key(chunk_segs(n_last, K.ck, 0), K.nck);
                 atomicAdd(&hist[key], 1u);
+                for (int j = 1; j < nch; ++j) atomicAdd(&hist[work_key(chunk_segs(n_last, K.ck, j), K.nck)], 5u);
             }
         }
         ws.meta[5 * nrays + ray] = key;
+        ws.meta[6 * nrays + ray] = nch;
         ws.meta[7 * nrays + ray] = nlive;
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -635,9 +691,10 @@ __global__ void __launch_bounds__(kBuckets) adj_scan(Ctl* __restrict__ ctl) {
     }
 }
 
-// Each ray with work takes the next slot of its bucket.  Neighbouring rays
-// mostly share a key, so a wave claims the slots of all its lanes with one
-// key in one atomic (one per distinct key in the wave).
+// Each task takes the next slot of its bucket.  A ray has 1 + 5 (chunks - 1)
+// tasks; round e places every lane's e-th task, and neighbouring rays mostly
+// share a key, so a wave claims the slots of all its lanes with one key in
+// one atomic (one per distinct key in the wave and round).
 __global__ void __launch_bounds__(256) adj_scatter(const AdjConst K, void* __restrict__ ws_base) {
     const int64_t nrays = (int64_t)K.W * K.H;
     const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
@@ -646,31 +703,52 @@ __global__ void __launch_bounds__(256) adj_scatter(const AdjConst K, void* __res
     // takes part in the ballots
     for (int64_t b0 = blockIdx.x * (int64_t)blockDim.x; b0 < nrays; b0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t ray = b0 + threadIdx.x;
-        const int key = ray < nrays ? ws.meta[5 * nrays + ray] : -1;
-        uint64_t pending = __ballot(key >= 0);
-        while (pending) {
-            const int leader = __ffsll((long long)pending) - 1;
-            const int lk = __shfl(key, leader);
-            const uint64_t peers = __ballot(key == lk) & pending;
-            uint32_t base = 0;
-            if (lane_id() == leader) base = atomicAdd(&ws.ctl->bucket[lk], (uint32_t)__popcll(peers));
-            base = __shfl(base, leader);
-            if (key == lk) ws.meta[6 * nrays + base + (uint32_t)__popcll(peers & below)] = (int32_t)ray;
-            pending &= ~peers;
+        int nch = 0, n_last = 0;
+        if (ray < nrays && ws.meta[5 * nrays + ray] >= 0) {
+            nch = ws.meta[6 * nrays + ray];
+            n_last = ws.meta[(1 + ws.meta[7 * nrays + ray]) * nrays + ray];
+        }
+        const int ntask = nch > 0 ? 1 + 5 * (nch - 1) : 0;
+        for (int e = 0; __ballot(e < ntask) != 0; ++e) {
+            const bool mine = e < ntask;
+            const int j = e == 0 ? 0 : 1 + (e - 1) / 5;
+            const int mode = e == 0 ? (nch == 1 ? kModeWhole : kModeTop) : (e - 1) % 5;
+            const int key = mine ? work_key(chunk_segs(n_last, K.ck, j), K.nck) : -1;
+            uint64_t pending = __ballot(mine);
+            while (pending) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const int lk = __shfl(key, leader);
+                const uint64_t peers = __ballot(key == lk) & pending;
+                uint32_t base = 0;
+                if (lane_id() == leader) base = atomicAdd(&ws.ctl->bucket[lk], (uint32_t)__popcll(peers));
+                base = __shfl(base, leader);
+                if (mine && key == lk) {
+                    const int64_t slot = (int64_t)base + __popcll(peers & below);
+                    ws.list[slot] = (int32_t)ray;
+                    ws.list[ws.cap + slot] = (j << 3) | mode;
+                }
+                pending &= ~peers;
+            }
         }
     }
 }
 
-// The adjoint walk of the rays on the work list: persistent waves take 64
-// consecutive slots at a time (longest work first, so a wave's lanes walk
-// back nearly the same number of segments and the short rays fill the
-// tail).  The recompute buffer is indexed by list slot, so a wave's
-// stores and loads there are coalesced.
+// The adjoint walks on the work list: persistent waves take 64 consecutive
+// tasks at a time (longest first, so a wave's lanes walk back nearly the
+// same number of segments and the short tasks fill the tail).  A task walks
+// its chunk from its highest step down: each checkpoint segment is
+// recomputed forward into the lane's column of the recompute buffer (so a
+// wave's stores and loads there are coalesced), then the discrete RK4
+// adjoint runs back over it.  A whole ray finishes into the gradient; a
+// chunk of a split ray leaves its adjoint and E, L, a sensitivities for
+// adj_combine.
 __global__ void __launch_bounds__(256)
 adj_backward_f32(const AdjConst K, void* __restrict__ ws_base, double* __restrict__ grad4) {
     const int64_t nrays = (int64_t)K.W * K.H;
     const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
     const uint32_t total = ws.ctl->total;
+    const int kmax = kmax_chunks(K.nck);
+    const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;   // recompute column
     double g_a = 0.0, g_th = 0.0;
     for (;;) {
         unsigned long long base = 0;
@@ -679,37 +757,44 @@ adj_backward_f32(const AdjConst K, void* __restrict__ ws_base, double* __restric
         if (base >= total) break;
         const int64_t i = (int64_t)base + lane_id();
         if (i >= total) continue;
-        const int64_t ray = ws.meta[6 * nrays + i];
+        const int64_t ray = ws.list[i];
+        const int task = ws.list[ws.cap + i];
+        const int mode = task & 7, jc = task >> 3;
         const int ncross = ws.meta[7 * nrays + ray];
         int cs[3];
         cs[0] = ws.meta[2 * nrays + ray];
         cs[1] = ws.meta[3 * nrays + ray];
         cs[2] = ws.meta[4 * nrays + ray];
         const int n_last = cs[ncross - 1];
+        const Chunk ch = chunk_of(n_last, K.ck, jc);
         const float E = ws.el[ray], L = ws.el[nrays + ray];
-        CState<float> lam{0.0f, 0.0f, 0.0f, 0.0f};
+        CState<float> lam{mode == 0 ? 1.0f : 0.0f, mode == 1 ? 1.0f : 0.0f, mode == 2 ? 1.0f : 0.0f,
+                          mode == 3 ? 1.0f : 0.0f};
         float gE = 0.0f, gL = 0.0f, ga = 0.0f;
-        int ci = ncross - 1;
-        for (int seg = n_last / K.ck; seg >= 0; --seg) {
+        // crossings injected: all of a whole ray or a top chunk, those inside
+        // the chunk for its crossing task, none for the unit-adjoint tasks
+        int ci = mode >= kModeAffine ? ncross - 1 : -1;
+        while (ci >= 0 && cs[ci] > ch.n_hi) --ci;
+        for (int seg = ch.n_hi / K.ck; seg >= ch.seg_lo; --seg) {
             const int n0 = seg * K.ck;
             // recompute the segment: states n0 .. n0 + ck (float stage inputs)
             const float* c = ws.ck + (size_t)seg * 8 * nrays + ray;
             FFState s;
             s.r.hi = c[0 * nrays]; s.r.lo = c[1 * nrays]; s.th.hi = c[2 * nrays]; s.th.lo = c[3 * nrays];
             s.pr.hi = c[4 * nrays]; s.pr.lo = c[5 * nrays]; s.pth.hi = c[6 * nrays]; s.pth.lo = c[7 * nrays];
-            const int jmax = min(K.ck - 1, n_last - n0);
+            const int jmax = min(K.ck - 1, ch.n_hi - n0);
             for (int j = 0; j <= jmax; ++j) {
-                float* o = ws.scr + (size_t)j * 4 * nrays + i;
-                o[0 * nrays] = ffp_f(s.r);
-                o[1 * nrays] = ffp_f(s.th);
-                o[2 * nrays] = ffp_f(s.pr);
-                o[3 * nrays] = ffp_f(s.pth);
+                float* o = ws.scr + (size_t)j * 4 * ws.lanes + col;
+                o[0 * ws.lanes] = ffp_f(s.r);
+                o[1 * ws.lanes] = ffp_f(s.th);
+                o[2 * ws.lanes] = ffp_f(s.pr);
+                o[3 * ws.lanes] = ffp_f(s.pth);
                 if (j < jmax) s = step_f32(K, s, E, L);
             }
             for (int j = jmax; j >= 0; --j) {
                 const int n = n0 + j;
-                const float* yj = ws.scr + (size_t)j * 4 * nrays + i;
-                const CState<float> y0{yj[0], yj[nrays], yj[2 * nrays], yj[3 * nrays]};
+                const float* yj = ws.scr + (size_t)j * 4 * ws.lanes + col;
+                const CState<float> y0{yj[0], yj[ws.lanes], yj[2 * ws.lanes], yj[3 * ws.lanes]};
                 float xr = 0.0f, xth = 0.0f;
                 if (ci >= 0 && cs[ci] == n) {
                     const float* q = ws.inj + (size_t)ci * 4 * nrays + ray;
@@ -724,12 +809,60 @@ adj_backward_f32(const AdjConst K, void* __restrict__ ws_base, double* __restric
                 lam.th += xth;
             }
         }
-        // initial state: y0 = (r_cam, theta_obs, p_r, p_th), E, L from the tetrad
+        if (mode == kModeWhole) {
+            // initial state: y0 = (r_cam, theta_obs, p_r, p_th), E, L from the tetrad
+            const double* co = ws.co + ray;
+            g_a += (double)lam.pr * co[0] + (double)lam.pth * co[2 * nrays] + (double)gE * co[4 * nrays] +
+                   (double)gL * co[6 * nrays] + (double)ga;
+            g_th += (double)lam.th + (double)lam.pr * co[nrays] + (double)lam.pth * co[3 * nrays] +
+                    (double)gE * co[5 * nrays] + (double)gL * co[7 * nrays];
+        } else {
+            float* o = ws.xo + ((ray * kmax + jc) * 5 + (mode == kModeTop ? 0 : mode)) * 8;
+            o[0] = lam.r; o[1] = lam.th; o[2] = lam.pr; o[3] = lam.pth;
+            o[4] = gE; o[5] = gL; o[6] = ga;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        g_a += __shfl_xor(g_a, off);
+        g_th += __shfl_xor(g_th, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&grad4[0], g_a);
+        atomicAdd(&grad4[1], g_th);
+    }
+}
+
+// Split rays: the top chunk's adjoint at its lower end, carried down through
+// each lower chunk's affine map (lambda_lo = sum_m lambda_m T_m + v, the
+// sensitivities likewise), then contracted with the initial-state partials.
+__global__ void __launch_bounds__(256)
+adj_combine(const AdjConst K, void* __restrict__ ws_base, double* __restrict__ grad4) {
+    const int64_t nrays = (int64_t)K.W * K.H;
+    const WS ws = ws_layout(ws_base, nrays, K.nck, K.ck);
+    const int kmax = kmax_chunks(K.nck);
+    double g_a = 0.0, g_th = 0.0;
+    for (int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ray < nrays;
+         ray += (int64_t)gridDim.x * blockDim.x) {
+        const int nch = ws.meta[5 * nrays + ray] >= 0 ? ws.meta[6 * nrays + ray] : 0;
+        if (nch < 2) continue;
+        const float* t = ws.xo + (ray * kmax) * 5 * 8;
+        double lam[4] = {t[0], t[1], t[2], t[3]};
+        double gE = t[4], gL = t[5], ga = t[6];
+        for (int j = 1; j < nch; ++j) {
+            const float* o = ws.xo + ((ray * kmax + j) * 5) * 8;
+            double nl[4], ng[3];
+            for (int k = 0; k < 4; ++k) nl[k] = o[4 * 8 + k];
+            for (int k = 0; k < 3; ++k) ng[k] = o[4 * 8 + 4 + k];
+            for (int m = 0; m < 4; ++m) {
+                for (int k = 0; k < 4; ++k) nl[k] += lam[m] * o[m * 8 + k];
+                for (int k = 0; k < 3; ++k) ng[k] += lam[m] * o[m * 8 + 4 + k];
+            }
+            for (int k = 0; k < 4; ++k) lam[k] = nl[k];
+            gE += ng[0]; gL += ng[1]; ga += ng[2];
+        }
         const double* co = ws.co + ray;
-        g_a += (double)lam.pr * co[0] + (double)lam.pth * co[2 * nrays] + (double)gE * co[4 * nrays] +
-               (double)gL * co[6 * nrays] + (double)ga;
-        g_th += (double)lam.th + (double)lam.pr * co[nrays] + (double)lam.pth * co[3 * nrays] +
-                (double)gE * co[5 * nrays] + (double)gL * co[7 * nrays];
+        g_a += lam[2] * co[0] + lam[3] * co[2 * nrays] + gE * co[4 * nrays] + gL * co[6 * nrays] + ga;
+        g_th += lam[1] + lam[2] * co[nrays] + lam[3] * co[3 * nrays] + gE * co[5 * nrays] + gL * co[7 * nrays];
     }
     for (int off = 32; off > 0; off >>= 1) {
         g_a += __shfl_xor(g_a, off);
@@ -750,14 +883,16 @@ int grid_for(int64_t n) {
     return (int)(want < cap ? (want < 1 ? 1 : want) : cap);
 }
 
-// persistent grid of the backward: every CU filled at the kernel's occupancy
-int backward_grid() {
-    int dev = 0, cus = 256, per_cu = 1;
+// Persistent grid of the backward: one block of 4 waves per CU, i.e. one
+// wave per SIMD (a lone wave of this kernel keeps its SIMD busy; with one
+// per SIMD the longest-first list puts the longest tasks on separate SIMDs),
+// and no more lanes than the recompute buffer has columns.
+int backward_grid(int64_t lanes) {
+    int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adj_backward_f32, 256, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    return cus * per_cu;
+    const int64_t cap = lanes / 256;
+    return (int)(cus < cap ? cus : cap);
 }
 
 }  // namespace
@@ -806,8 +941,10 @@ int np_adjoint_backward(const np_adjoint_params* p, const float* d_grad_image, v
     hipLaunchKernelGGL(np::adj_prep, grid, dim3(256), 0, st, K, d_grad_image, d_workspace, d_grad4);
     hipLaunchKernelGGL(np::adj_scan, dim3(1), dim3(np::kBuckets), 0, st, ctl);
     hipLaunchKernelGGL(np::adj_scatter, grid, dim3(256), 0, st, K, d_workspace);
-    hipLaunchKernelGGL(np::adj_backward_f32, dim3(np::backward_grid()), dim3(256), 0, st, K, d_workspace,
-                       d_grad4);
+    hipLaunchKernelGGL(np::adj_backward_f32, dim3(np::backward_grid(np::scr_lanes(nrays))), dim3(256), 0, st, K,
+                       d_workspace, d_grad4);
+    if (np::kmax_chunks(K.nck) > 1)
+        hipLaunchKernelGGL(np::adj_combine, grid, dim3(256), 0, st, K, d_workspace, d_grad4);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
