@@ -129,7 +129,7 @@ def parse():
                          "takes ~18 s, so it does not follow --steps/--warmup)")
     ap.add_argument("--crossing", default="hermite", choices=["hermite", "linear"])
     ap.add_argument("--waves-per-simd", type=int, default=0, choices=[0, 2, 3, 4],
-                    help="FP32C kernel variant (0 = the library default, 3 waves per SIMD)")
+                    help="FP32C kernel variant (0 = the library default, 4 waves per SIMD)")
     ap.add_argument("--linear-check", default="960x540x1",
                     help="WxHxSPP of the linear-crossing comparison; 'none' skips it")
     ap.add_argument("--adjoint", type=int, default=1,
@@ -840,7 +840,7 @@ def main() -> int:
     head = "fp32c" if "fp32c" in res else precs[0]
     R = res[head]
     value = R["rays"] / R["wall"]
-    waves = args.waves_per_simd or 3
+    waves = args.waves_per_simd or 4
     if head != "fp32c":
         kernel = "kerr_render<StepF64, adaptive>"
     elif waves == 2:
@@ -849,7 +849,7 @@ def main() -> int:
         kernel = f"kerr_render_f32c_capped<adaptive, {waves} waves>"
     achieved = (R["flops32"] + 2.0 * R["flops64"]) / R["mean_s"] / 1e12
     # the traffic constant was measured on the default kernel variant
-    head_1080p4 = head == "fp32c" and waves == 3 and (args.width, args.height, args.spp) == (1920, 1080, 4)
+    head_1080p4 = head == "fp32c" and waves == 4 and (args.width, args.height, args.spp) == (1920, 1080, 4)
     line = {
         "metric": METRIC,
         "value": value,

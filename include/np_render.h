@@ -78,11 +78,11 @@ typedef struct np_render_params {
     uint32_t seed;            /* jitter seed; 0 gives the reference's hash keys */
     double  fp64_radius;      /* NP_PREC_FP32C only: steps that start inside this radius (absolute
                                  units) run in FP64; 0 = auto, max(30 M, 1.5 disk_rout) */
-    int32_t waves_per_simd;   /* NP_PREC_FP32C kernel variant: 0 or 3 = capped at 168 VGPRs
-                                 (3 waves per SIMD; the rare FP64 branch spills), 4 = capped at
-                                 128 VGPRs (spills in the step loop too: a little faster, far
-                                 more scratch traffic), 2 = the compiler's own register budget
-                                 (2 waves per SIMD); other values are rejected */
+    int32_t waves_per_simd;   /* NP_PREC_FP32C kernel variant: 0 or 4 = capped at 128 VGPRs
+                                 (4 waves per SIMD; the float attempt runs without spills, the
+                                 rare FP64 branch spills), 3 = capped at 168 VGPRs (3 waves),
+                                 2 = the compiler's own register budget (2 waves per SIMD);
+                                 other values are rejected */
     int32_t crossing;         /* np_crossing: disk-crossing radius between accepted steps
                                  (reference: linear in theta) */
     int32_t estimate_every_step; /* NP_PREC_FP32C, adaptive: 0 = a ray's steps at the maximum

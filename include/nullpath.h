@@ -144,7 +144,7 @@ typedef struct np_tuning {
  * (frame i starts at 3 * sum_{j<i} width_j * height_j floats; colour sums
  * over spp samples); d_records (optional) likewise with spp records per
  * pixel; d_counters (optional) sums over all frames.  tuning NULL = block
- * 256, the frames' own FP32C variant (3 waves by default), chunk 1, frame radii.
+ * 256, the frames' own FP32C variant (4 waves by default), chunk 1, frame radii.
  */
 int np_render_batch(const np_render_params* frames, int nframes, int spp, int sample_offset,
                     float* d_rgb, np_ray_record* d_records, np_trace_counters* d_counters,

@@ -332,31 +332,114 @@ __device__ void invariants(const FrameConst& F, double r, double th, double pt, 
 }
 
 // ---------------------------------------------------------------------------
-// Per-lane ray bookkeeping shared by both steppers (render.cu:171-183).
+// Lane rows in LDS.
+//
+// What a lane touches only when a ray starts, crosses the disk plane, runs
+// an FP64 attempt or ends (the pixel and sample it works on, the colour
+// sums, the crossing count, p_t and p_phi in double, the per-lane totals)
+// lives in a row of LDS words instead of VGPRs, so the integrator attempt
+// has the register file to itself.  Rows are kLaneSlots words long, an odd
+// stride, so the 64 lanes of a wave reading the same slot hit distinct
+// banks; every slot is addressed from one base register with an immediate
+// offset.  Doubles and u64 take two words (rows are only 4-byte aligned).
+// The invariant monitor's values sit in a second region after the rows of
+// the whole block.  Launches size the dynamic LDS with lane_lds_bytes.
+// ---------------------------------------------------------------------------
+enum LaneSlot : int {
+    LS_PIX = 0,        // u64: pixel of the window the lane works on
+    LS_CEND = 2,       // u64: end of the lane's refill chunk
+    LS_SAMPLE = 4,     // sample index within the pixel
+    LS_ACC = 5,        // 3 floats: colour sum of the pixel's samples
+    LS_N_RAYS = 8,     // per-lane totals (u32), reduced at the end
+    LS_N_STEPS = 9,
+    LS_N_CAP = 10,
+    LS_N_ESC = 11,
+    LS_N_STEPCAP = 12,
+    LS_N_DISK = 13,
+    LS_N_ATT = 14,
+    LS_ATT_RAY = 15,   // attempts of the current ray
+    LS_DISK = 16,      // 3 floats: disk radiance of the current ray
+    LS_CAMY = 19,      // camera-frame y of the ray direction (sky shading)
+    LS_ORDERS = 20,    // disk crossings counted so far
+    LS_PT = 21,        // double: covariant p_t (exactly conserved)
+    LS_PPH = 23,       // double: covariant p_phi
+    LS_FULL = 25,      // 5 doubles: the FP64 re-run's full step (FP32C)
+};
+constexpr int kLaneSlots = 35;
+// invariant monitor region (INV kernels): p_t, p_phi at launch, the ray's
+// H / dE / dL maxima and the lane's maxima over its rays
+enum InvSlot : int { IS_PT0 = 0, IS_PPH0 = 2, IS_INV = 4, IS_INVL = 7 };
+constexpr int kInvSlots = 11;
+
+extern __shared__ uint32_t np_lane_lds[];
+
+__device__ __forceinline__ uint32_t* lane_row() { return np_lane_lds + threadIdx.x * kLaneSlots; }
+__device__ __forceinline__ uint32_t* inv_row() {
+    return np_lane_lds + blockDim.x * kLaneSlots + threadIdx.x * kInvSlots;
+}
+__device__ __forceinline__ float ld_f(const uint32_t* row, int k) { return __uint_as_float(row[k]); }
+__device__ __forceinline__ void st_f(uint32_t* row, int k, float v) { row[k] = __float_as_uint(v); }
+__device__ __forceinline__ double ld_d(const uint32_t* row, int k) {
+    return __hiloint2double((int)row[k + 1], (int)row[k]);
+}
+__device__ __forceinline__ void st_d(uint32_t* row, int k, double v) {
+    row[k] = (uint32_t)__double2loint(v);
+    row[k + 1] = (uint32_t)__double2hiint(v);
+}
+__device__ __forceinline__ uint64_t ld_u64(const uint32_t* row, int k) {
+    return (uint64_t)row[k] | ((uint64_t)row[k + 1] << 32);
+}
+__device__ __forceinline__ void st_u64(uint32_t* row, int k, uint64_t v) {
+    row[k] = (uint32_t)v;
+    row[k + 1] = (uint32_t)(v >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// Per-lane ray bookkeeping shared by the steppers (render.cu:171-183): the
+// values every accepted step reads stay in registers, the rest in the lane
+// row above.
 // ---------------------------------------------------------------------------
 struct RayBase {
-    double pt, pph, pt0, pph0;       // covariant p_t, p_phi (exactly conserved)
     double prev_r, prev_th, min_r;   // previous accepted state, closest approach
-    float disk[3];
-    float inv[3];
-    float cam_x, cam_y;
-    int step, attempt, orders, attempts_total;
+    int step, attempt;
 };
 
-__device__ __forceinline__ void base_start(const FrameConst& F, RayBase& R, float cam_x, float cam_y) {
-    R.pt0 = R.pt;
-    R.pph0 = R.pph;
+template <bool INV>
+__device__ __forceinline__ void base_start(const FrameConst& F, RayBase& R, float cam_y, double pt,
+                                           double pph) {
     R.prev_r = F.r_cam;
     R.prev_th = F.th_cam;
     R.min_r = F.r_cam;
-    R.disk[0] = R.disk[1] = R.disk[2] = 0.0f;
-    R.inv[0] = R.inv[1] = R.inv[2] = 0.0f;
-    R.cam_x = cam_x;
-    R.cam_y = cam_y;
     R.step = 0;
     R.attempt = 0;
-    R.orders = 0;
-    R.attempts_total = 0;
+    uint32_t* row = lane_row();
+    st_f(row, LS_DISK + 0, 0.0f);
+    st_f(row, LS_DISK + 1, 0.0f);
+    st_f(row, LS_DISK + 2, 0.0f);
+    st_f(row, LS_CAMY, cam_y);
+    row[LS_ORDERS] = 0;
+    row[LS_ATT_RAY] = 0;
+    st_d(row, LS_PT, pt);
+    st_d(row, LS_PPH, pph);
+    if (INV) {
+        uint32_t* iv = inv_row();
+        st_d(iv, IS_PT0, pt);
+        st_d(iv, IS_PPH0, pph);
+        st_f(iv, IS_INV + 0, 0.0f);
+        st_f(iv, IS_INV + 1, 0.0f);
+        st_f(iv, IS_INV + 2, 0.0f);
+    }
+}
+
+// invariants() on the lane's own monitor values
+__device__ __forceinline__ void invariants_lane(const FrameConst& F, double r, double th, double pt,
+                                                double pr, double pth, double pph) {
+    uint32_t* iv = inv_row();
+    float inv[3] = {ld_f(iv, IS_INV + 0), ld_f(iv, IS_INV + 1), ld_f(iv, IS_INV + 2)};
+    invariants(F, r, th, pt, pr, pth, pph, ld_d(iv, IS_PT0), ld_d(iv, IS_PPH0), inv);
+    st_f(iv, IS_INV + 0, inv[0]);
+    st_f(iv, IS_INV + 1, inv[1]);
+    st_f(iv, IS_INV + 2, inv[2]);
 }
 
 // Crossing radius from the cubic Hermite interpolants of theta(lambda) and
@@ -399,7 +482,8 @@ __device__ __forceinline__ int after_accept(const FrameConst& F, typename S::Ray
     const bool sign_change = (d_prev > 0 && d_curr < 0) || (d_prev < 0 && d_curr > 0);
     const bool near_zero = (fabs(d_prev) < eps && fabs(d_curr) >= eps) ||
                            (fabs(d_curr) < eps && fabs(d_prev) >= eps);
-    if ((sign_change || near_zero) && R.orders < F.disk_max_orders) {
+    uint32_t* row = lane_row();
+    if ((sign_change || near_zero) && (int)row[LS_ORDERS] < F.disk_max_orders) {
         const double denom = fabs(d_prev) + fabs(d_curr) + 1e-30;
         const double alpha = fabs(d_prev) / denom;
         double r_cross = R.prev_r + alpha * (r - R.prev_r);
@@ -410,16 +494,16 @@ __device__ __forceinline__ int after_accept(const FrameConst& F, typename S::Ray
         }
         if (r_cross >= F.rin && r_cross <= F.rout) {
             // p_t and p_phi are constant, so their interpolants equal them
-            const float g = (float)doppler_g(F, r_cross, R.pt, R.pph);
+            const double pt = S::pt(R), pph = S::pph(R);
+            const float g = (float)doppler_g(F, r_cross, pt, pph);
             const float rr = F.emiss_m_units ? (float)(r_cross / F.M) : (float)r_cross;
             const float j = powf(fmaxf(rr, 1.0f), -F.emiss_q);
             const float I = j * g * g * g;
-            R.disk[0] += (1.0f * I) * F.disk_gain;
-            R.disk[1] += (0.75f * I) * F.disk_gain;
-            R.disk[2] += (0.45f * I) * F.disk_gain;
-            R.orders += 1;
-            if (INV)
-                invariants(F, r_cross, NP_PI_D * 0.5, R.pt, pr, pth, R.pph, R.pt0, R.pph0, R.inv);
+            st_f(row, LS_DISK + 0, ld_f(row, LS_DISK + 0) + (1.0f * I) * F.disk_gain);
+            st_f(row, LS_DISK + 1, ld_f(row, LS_DISK + 1) + (0.75f * I) * F.disk_gain);
+            st_f(row, LS_DISK + 2, ld_f(row, LS_DISK + 2) + (0.45f * I) * F.disk_gain);
+            row[LS_ORDERS] += 1;
+            if (INV) invariants_lane(F, r_cross, NP_PI_D * 0.5, pt, pr, pth, pph);
         }
     }
     R.prev_r = r;
@@ -434,6 +518,8 @@ __device__ __forceinline__ int after_accept(const FrameConst& F, typename S::Ray
 // Colour of a finished sample (render.cu:312-359).
 __device__ void shade_final(const FrameConst& F, const RayBase& R, int cls, float col[3]) {
     const float Rs = F.Rs;
+    const uint32_t* row = lane_row();
+    const int orders = (int)row[LS_ORDERS];
     if (cls == NP_RAY_CAPTURED) {
         if (F.bg_mode == NP_BG_POINT) {
             const float r_ph = 1.5f * Rs;
@@ -445,10 +531,10 @@ __device__ void shade_final(const FrameConst& F, const RayBase& R, int cls, floa
         } else {
             col[0] = col[1] = col[2] = 0.0f;
         }
-    } else if (R.orders > 0 && F.disk_on) {
-        col[0] = R.disk[0];
-        col[1] = R.disk[1];
-        col[2] = R.disk[2];
+    } else if (orders > 0 && F.disk_on) {
+        col[0] = ld_f(row, LS_DISK + 0);
+        col[1] = ld_f(row, LS_DISK + 1);
+        col[2] = ld_f(row, LS_DISK + 2);
     } else if (F.bg_mode == NP_BG_POINT) {
         const float r_ph = 1.5f * Rs;
         const float width = fmaxf(0.02f * Rs, 0.005f * Rs + 0.02f * F.bg_sigma_rad * Rs);
@@ -458,7 +544,7 @@ __device__ void shade_final(const FrameConst& F, const RayBase& R, int cls, floa
         col[0] = col[1] = col[2] = I;
     } else {
         // shade_background (render.cu:74-86) with v from the camera direction
-        const float v = R.cam_y * 0.5f + 0.5f;
+        const float v = ld_f(row, LS_CAMY) * 0.5f + 0.5f;
         const float t = fminf(fmaxf(v, 0.f), 1.f);
         float c0 = 0.02f * (1 - t) + 0.08f * t;
         float c1 = 0.03f * (1 - t) + 0.09f * t;
@@ -480,12 +566,14 @@ struct StepF64 {
     struct Ray : RayBase {
         Y5 y;
         Deriv k1;
+        double pt, pph;   // read by every right-hand side
         double h_adapt, h;
         bool k1_valid;
     };
 
+    template <bool INV>
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
-                                 float cam_x, float cam_y) {
+                                 float cam_y) {
         init_momentum(F, n_r, n_th, n_ph, R.pt, R.y.pr, R.y.pth, R.pph);
         R.y.r = F.r_cam;
         R.y.th = F.th_cam;
@@ -493,8 +581,11 @@ struct StepF64 {
         R.h_adapt = 0.0;
         R.h = 0.0;
         R.k1_valid = false;
-        base_start(F, R, cam_x, cam_y);
+        base_start<INV>(F, R, cam_y, R.pt, R.pph);
     }
+
+    __device__ static double pt(const Ray& R) { return R.pt; }
+    __device__ static double pph(const Ray& R) { return R.pph; }
 
     __device__ static double r(const Ray& R) { return R.y.r; }
     __device__ static double th(const Ray& R) { return R.y.th; }
@@ -697,7 +788,7 @@ __device__ __forceinline__ DerivF kerr_rhs_f32_sc(const FrameConst& F, float r, 
 __device__ __forceinline__ DerivF kerr_rhs_f32(const FrameConst& F, float r, float th, float thd,
                                                float pr, float pth, float E, float L) {
     float s0, c0;
-    sincosf(th, &s0, &c0);
+    sincos_f(th, s0, c0);
     return kerr_rhs_f32_sc(F, r, s0, c0, thd, pr, pth, E, L);
 }
 
@@ -746,19 +837,20 @@ __device__ __forceinline__ bool in_domain_d(const FrameConst& F, const Y5& y) {
 // plain FP64 stage sums, as three RK4 runs of four stages in one
 // non-unrolled loop.  This path is rare in a wave, so its live set is kept
 // small: k1 = f(y) is evaluated twice (runs 0 and 1) instead of being held,
-// the full step waits in LDS (full[c * blockDim.x], c = r, th, ph, pr, pth)
-// and the intermediate half step is only checked against the domain.
+// the full step waits in the lane row (slots LS_FULL + 2c, c = r, th, ph,
+// pr, pth) and the intermediate half step is only checked against the
+// domain.
 // Returns true when all three results are inside the domain; k1r, k1th are
 // the r and theta components of f(y).
 __device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5& y, const double E,
-                                                  const double L, const double h, double* full,
-                                                  Y5& half2, double& k1r, double& k1th) {
+                                                  const double L, const double h, Y5& half2,
+                                                  double& k1r, double& k1th) {
     Deriv S, kp;
     Y5 base = ff5_to_y5(y);
     S.r = S.th = S.ph = S.pr = S.pth = 0.0;
     kp = S;
     bool ok = true;
-    const int stride = blockDim.x;
+    uint32_t* row = lane_row();
 #pragma clang loop unroll(disable)
     for (int j = 0; j < 12; ++j) {
         const int st = j & 3, run = j >> 2;
@@ -787,11 +879,11 @@ __device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5
             o.pth = base.pth + h6 * S.pth;
             ok = ok && in_domain_d(F, o);
             if (run == 0) {
-                full[0 * stride] = o.r;
-                full[1 * stride] = o.th;
-                full[2 * stride] = o.ph;
-                full[3 * stride] = o.pr;
-                full[4 * stride] = o.pth;
+                st_d(row, LS_FULL + 0, o.r);
+                st_d(row, LS_FULL + 2, o.th);
+                st_d(row, LS_FULL + 4, o.ph);
+                st_d(row, LS_FULL + 6, o.pr);
+                st_d(row, LS_FULL + 8, o.pth);
                 base = ff5_to_y5(y);
             } else if (run == 1) {
                 base = o;
@@ -830,7 +922,7 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
     const float hh = 0.5f * h;
     const float yr = ff_f(y.r), ypr = ff_f(y.pr), ypth = ff_f(y.pth);
     float s0, c0;
-    sincosf(y.th.hi, &s0, &c0);
+    sincos_f(y.th.hi, s0, c0);
     const DerivF k2 = kerr_rhs_f32_sc(F, yr + hh * k1.r, s0, c0, y.th.lo + hh * k1.th, ypr + hh * k1.pr,
                                       ypth + hh * k1.pth, E, L);
     const DerivF k3 = kerr_rhs_f32_sc(F, yr + hh * k2.r, s0, c0, y.th.lo + hh * k2.th, ypr + hh * k2.pr,
@@ -901,7 +993,7 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
             if (j == 5) {
                 half1 = o;
                 br = ff_f(o.r); bthl = o.th.lo; bpr = ff_f(o.pr); bpth = ff_f(o.pth);
-                sincosf(o.th.hi, &bs, &bc);
+                sincos_f(o.th.hi, bs, bc);
             } else {
                 half2 = o;
             }
@@ -958,30 +1050,34 @@ struct StepF32C {
         uint8_t skip;
     };
 
+    // p_t and p_phi in double live in the lane row (FP64 attempts and
+    // crossings read them); float steps use E and L.
+    __device__ static double pt(const Ray&) { return ld_d(lane_row(), LS_PT); }
+    __device__ static double pph(const Ray&) { return ld_d(lane_row(), LS_PPH); }
+
+    template <bool INV>
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
-                                 float cam_x, float cam_y) {
-        double pr, pth;
-        init_momentum(F, n_r, n_th, n_ph, R.pt, pr, pth, R.pph);
+                                 float cam_y) {
+        double pt, pph, pr, pth;
+        init_momentum(F, n_r, n_th, n_ph, pt, pr, pth, pph);
         R.y.r = ff_from_d(F.r_cam);
         R.y.th = ff_from_d(F.th_cam);
         R.y.ph = ff_from_d(F.ph_cam);
         R.y.pr = ff_from_d(pr);
         R.y.pth = ff_from_d(pth);
-        R.E = (float)(-R.pt);
-        R.L = (float)R.pph;
+        R.E = (float)(-pt);
+        R.L = (float)pph;
         R.h_adapt = 0.0f;
         R.h = 0.0f;
         R.k1_valid = false;
         R.mode = 0;
         R.skip = 0;
         {
-            double st, ct;
-            sincos_d(F.th_cam, st, ct);
-            const double E = -R.pt, L = R.pph, a2E2 = F.a * F.a * E * E;
-            const double Q = pth * pth + ct * ct * (L * L / fmax(st * st, 1e-300) - a2E2);
+            const double E = -pt, L = pph, a2E2 = F.a * F.a * E * E;
+            const double Q = pth * pth + F.c2_cam * (L * L / fmax(F.s2_cam, 1e-300) - a2E2);
             R.qb = (float)(sqrt(fmax(Q + a2E2, 0.0)) * (1.0 + 1e-6));
         }
-        base_start(F, R, cam_x, cam_y);
+        base_start<INV>(F, R, cam_y, pt, pph);
     }
 
     __device__ static double r(const Ray& R) { return ff_to_d(R.y.r); }
@@ -1003,7 +1099,7 @@ struct StepF32C {
         dr0 = (double)R.k1.r;
         dth0 = (double)R.k1.th;
         const Deriv k = kerr_rhs_c64(F, ff_to_d(R.y.r), ff_to_d(R.y.th), ff_to_d(R.y.pr),
-                                     ff_to_d(R.y.pth), -R.pt, R.pph);
+                                     ff_to_d(R.y.pth), -pt(R), pph(R));
         dr1 = k.r;
         dth1 = k.th;
         R.k1 = DerivF{(float)k.r, (float)k.th, (float)k.ph, (float)k.pr, (float)k.pth};
@@ -1016,12 +1112,10 @@ struct StepF32C {
     // Step sizes are held in float (a 6e-8 relative change of h); so is the
     // step factor.
     __device__ static int attempt64(const FrameConst& F, Ray& R, int& rhs64) {
-        __shared__ double s_full[5 * 256];
-        double* full = s_full + threadIdx.x;
         const double h = (double)R.h;
         Y5 half2;
         double k1r, k1th;
-        const bool ok = step_doubling_c64(F, R.y, -R.pt, R.pph, h, full, half2, k1r, k1th);
+        const bool ok = step_doubling_c64(F, R.y, -pt(R), pph(R), h, half2, k1r, k1th);
         rhs64 += 12;
         float h_try = R.h;
         float fac = 0.0f;
@@ -1029,12 +1123,12 @@ struct StepF32C {
             h_try *= 0.5f;
         } else {
             const double rt = F.rtol, at = F.atol;
-            const int sd = blockDim.x;
+            const uint32_t* row = lane_row();
             double e = 0.0;
             const double* hv = &half2.r;
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
-                const double fv = full[c * sd];
+                const double fv = ld_d(row, LS_FULL + 2 * c);
                 e = fmax(e, err1(fv, hv[c], at + rt * fmax(fabs(fv), fabs(hv[c]))));
             }
             const float le = __log2f(fmaxf(1e-12f, (float)e));
@@ -1072,7 +1166,7 @@ struct StepF32C {
             R.h = h;
             if (ff_to_d(R.y.r) < F.r_isl) {
                 double k1r, k1th;
-                R.y = to_ff5(rk4_c64(F, ff5_to_y5(R.y), -R.pt, R.pph, (double)h, k1r, k1th));
+                R.y = to_ff5(rk4_c64(F, ff5_to_y5(R.y), -pt(R), pph(R), (double)h, k1r, k1th));
                 R.k1.r = (float)k1r;
                 R.k1.th = (float)k1th;
                 rhs64 += 4;
@@ -1101,7 +1195,7 @@ struct StepF32C {
         }
         const float h = R.h;
         float s0, c0;
-        sincosf(R.y.th.hi, &s0, &c0);
+        sincos_f(R.y.th.hi, s0, c0);
         if (!R.k1_valid) {
             R.k1 = kerr_rhs_f32_sc(F, ff_f(R.y.r), s0, c0, R.y.th.lo, ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
             rhs_count += 1;
@@ -1226,12 +1320,17 @@ struct StepC64 {
     struct Ray : RayBase {
         Y5 y;
         Deriv k1;
+        double pt, pph;   // read by every right-hand side
         double h_adapt, h, qb;
         bool k1_valid, skip;
     };
 
+    __device__ static double pt(const Ray& R) { return R.pt; }
+    __device__ static double pph(const Ray& R) { return R.pph; }
+
+    template <bool INV>
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
-                                 float cam_x, float cam_y) {
+                                 float cam_y) {
         init_momentum(F, n_r, n_th, n_ph, R.pt, R.y.pr, R.y.pth, R.pph);
         R.y.r = F.r_cam;
         R.y.th = F.th_cam;
@@ -1240,12 +1339,10 @@ struct StepC64 {
         R.h = 0.0;
         R.k1_valid = false;
         R.skip = false;
-        double st, ct;
-        sincos_d(F.th_cam, st, ct);
         const double E = -R.pt, L = R.pph, a2E2 = F.a * F.a * E * E;
-        const double Q = R.y.pth * R.y.pth + ct * ct * (L * L / fmax(st * st, 1e-300) - a2E2);
+        const double Q = R.y.pth * R.y.pth + F.c2_cam * (L * L / fmax(F.s2_cam, 1e-300) - a2E2);
         R.qb = sqrt(fmax(Q + a2E2, 0.0)) * (1.0 + 1e-6);
-        base_start(F, R, cam_x, cam_y);
+        base_start<INV>(F, R, cam_y, R.pt, R.pph);
     }
 
     __device__ static double r(const Ray& R) { return R.y.r; }
@@ -1376,13 +1473,23 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
     int fid = __builtin_amdgcn_readfirstlane(wave_global % nframes);
     typename S::Ray R;
     bool has = false, done = false;
-    uint64_t pix = 0, chunk_end = 0;
-    int sample = 0;
-    float acc[3] = {0.0f, 0.0f, 0.0f};
-    float inv_lane[3] = {0.0f, 0.0f, 0.0f};
-    // per-lane counters, flushed to the u64 totals before they can wrap
-    uint32_t c_rhs = 0, c_att = 0, c_steps = 0, c_rays = 0, c_cap = 0, c_esc = 0, c_cap_steps = 0,
-             c_disk = 0, c_rhs64 = 0, c_att64 = 0, c_lazy = 0;
+    uint32_t* row = lane_row();
+    st_u64(row, LS_PIX, 0);
+    st_u64(row, LS_CEND, 0);
+    row[LS_SAMPLE] = 0;
+#pragma unroll
+    for (int k = LS_N_RAYS; k <= LS_N_ATT; ++k) row[k] = 0;
+    if (INV) {
+        uint32_t* iv = inv_row();
+        st_f(iv, IS_INVL + 0, 0.0f);
+        st_f(iv, IS_INVL + 1, 0.0f);
+        st_f(iv, IS_INVL + 2, 0.0f);
+    }
+    // Right-hand sides per lane (a register: it moves on every attempt),
+    // flushed before it can wrap; FP64 and lazy attempts as wave totals
+    // (scalar registers, from ballots).
+    uint32_t c_rhs = 0;
+    uint64_t w_att64 = 0, w_lazy = 0;
     if (spp <= 0) done = true;
 
     // Outer loop: one frame at a time; inner loop: one integrator attempt per
@@ -1397,6 +1504,7 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
         while (true) {
             const bool need = !has && !done;
             if (__ballot(need)) {
+                const uint64_t pix = ld_u64(row, LS_PIX), chunk_end = ld_u64(row, LS_CEND);
                 const bool in_chunk = need && (pix + 1 < chunk_end);
                 const bool fetch = need && !in_chunk;
                 const uint64_t unit = wave_fetch(fetch, work + fid);
@@ -1406,19 +1514,21 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
                         p = pix + 1;
                     } else {
                         p = unit * (uint64_t)chunk;
-                        chunk_end = min(p + (uint64_t)chunk, npix);
+                        st_u64(row, LS_CEND, min(p + (uint64_t)chunk, npix));
                     }
                     if (p < npix) {
-                        pix = p;
-                        sample = 0;
-                        acc[0] = acc[1] = acc[2] = 0.0f;
+                        st_u64(row, LS_PIX, p);
+                        row[LS_SAMPLE] = 0;
+                        st_f(row, LS_ACC + 0, 0.0f);
+                        st_f(row, LS_ACC + 1, 0.0f);
+                        st_f(row, LS_ACC + 2, 0.0f);
                         const FrameConst& Fc = cold(B);
-                        const uint64_t ap = B->pix0 + pix;
+                        const uint64_t ap = B->pix0 + p;
                         const int x = (int)(ap % (uint64_t)W);
                         const int yy = (int)(ap / (uint64_t)W);
                         float n_r, n_th, n_ph, cx, cy;
-                        camera_dir(Fc, x, yy, B->sample0 + (uint32_t)sample, n_r, n_th, n_ph, cx, cy);
-                        S::start(Fc, R, n_r, n_th, n_ph, cx, cy);
+                        camera_dir(Fc, x, yy, B->sample0, n_r, n_th, n_ph, cx, cy);
+                        S::template start<INV>(Fc, R, n_r, n_th, n_ph, cy);
                         has = true;
                     } else {
                         done = true;
@@ -1426,78 +1536,90 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
                 }
             }
             if (__ballot(has) == 0) break;
-            if (counters && __ballot(c_rhs > 0x40000000u || c_steps > 0x40000000u)) {
-                wave_add_u64(&counters->steps, c_steps);
-                wave_add_u64(&counters->attempts, c_att);
+            if (counters && __ballot(c_rhs > 0x40000000u)) {
                 wave_add_u64(&counters->rhs_evals, c_rhs);
-                wave_add_u64(&counters->attempts_f64, c_att64);
-                wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
-                wave_add_u64(&counters->attempts_lazy, c_lazy);
-                c_steps = c_att = c_rhs = c_att64 = c_rhs64 = c_lazy = 0;
+                c_rhs = 0;
             }
+            int rhs64 = 0, lazy_att = 0;
+            bool ended = false;
             if (has) {
-                int rhs_count = 0, rhs64 = 0, lazy_att = 0;
-                R.attempts_total += 1;
-                c_att += 1;
+                int rhs_count = 0;
+                row[LS_ATT_RAY] += 1;
                 const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count, rhs64, lazy_att);
-                c_lazy += (uint32_t)lazy_att;
                 c_rhs += (uint32_t)(rhs_count + rhs64);
-                c_rhs64 += (uint32_t)rhs64;
-                c_att64 += (rhs64 > 0);
                 int cls = -1;
                 if (st == 2) cls = NP_RAY_CAPTURED;
                 else if (st == 1) cls = after_accept<S, INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
                 if (cls >= 0) {
+                    ended = true;
                     const FrameConst& Fc = cold(B);
                     float col[3];
                     shade_final(Fc, R, cls, col);
-                    if (INV && cls != NP_RAY_CAPTURED && !(R.orders > 0 && Fc.disk_on))
-                        invariants(Fc, S::r(R), S::th(R), R.pt, S::pr(R), S::pth(R), R.pph, R.pt0, R.pph0,
-                                   R.inv);
-                    acc[0] += col[0];
-                    acc[1] += col[1];
-                    acc[2] += col[2];
-                    inv_lane[0] = fmaxf(inv_lane[0], R.inv[0]);
-                    inv_lane[1] = fmaxf(inv_lane[1], R.inv[1]);
-                    inv_lane[2] = fmaxf(inv_lane[2], R.inv[2]);
-                    c_rays += 1;
-                    c_steps += (uint32_t)R.step;
-                    c_cap += (cls == NP_RAY_CAPTURED);
-                    c_esc += (cls == NP_RAY_ESCAPED);
-                    c_cap_steps += (cls == NP_RAY_STEPCAP);
-                    c_disk += (R.orders > 0);
+                    const uint32_t orders = row[LS_ORDERS], att = row[LS_ATT_RAY];
+                    if (INV) {
+                        if (cls != NP_RAY_CAPTURED && !(orders > 0 && Fc.disk_on))
+                            invariants_lane(Fc, S::r(R), S::th(R), S::pt(R), S::pr(R), S::pth(R), S::pph(R));
+                        uint32_t* iv = inv_row();
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                            st_f(iv, IS_INVL + k, fmaxf(ld_f(iv, IS_INVL + k), ld_f(iv, IS_INV + k)));
+                    }
+                    st_f(row, LS_ACC + 0, ld_f(row, LS_ACC + 0) + col[0]);
+                    st_f(row, LS_ACC + 1, ld_f(row, LS_ACC + 1) + col[1]);
+                    st_f(row, LS_ACC + 2, ld_f(row, LS_ACC + 2) + col[2]);
+                    row[LS_N_RAYS] += 1;
+                    row[LS_N_STEPS] += (uint32_t)R.step;
+                    row[LS_N_CAP] += (cls == NP_RAY_CAPTURED);
+                    row[LS_N_ESC] += (cls == NP_RAY_ESCAPED);
+                    row[LS_N_STEPCAP] += (cls == NP_RAY_STEPCAP);
+                    row[LS_N_DISK] += (orders > 0);
+                    row[LS_N_ATT] += att;
+                    const uint64_t pix = ld_u64(row, LS_PIX);
+                    const uint32_t sample = row[LS_SAMPLE];
                     if (records_base) {
                         np_ray_record rec;
                         rec.cls = cls;
-                        rec.orders = R.orders;
+                        rec.orders = (int)orders;
                         rec.steps = R.step;
-                        rec.attempts = R.attempts_total;
+                        rec.attempts = (int)att;
                         rec.min_r = R.min_r;
                         rec.r_end = S::r(R);
                         records_base[B->rec_off + pix * (uint64_t)spp + (uint64_t)sample] = rec;
                     }
-                    sample += 1;
-                    if (sample < spp) {
+                    if ((int)sample + 1 < spp) {
+                        row[LS_SAMPLE] = sample + 1;
                         const uint64_t ap = B->pix0 + pix;
                         const int x = (int)(ap % (uint64_t)W);
                         const int yy = (int)(ap / (uint64_t)W);
                         float n_r, n_th, n_ph, cx, cy;
-                        camera_dir(Fc, x, yy, B->sample0 + (uint32_t)sample, n_r, n_th, n_ph, cx, cy);
-                        S::start(Fc, R, n_r, n_th, n_ph, cx, cy);
+                        camera_dir(Fc, x, yy, B->sample0 + sample + 1u, n_r, n_th, n_ph, cx, cy);
+                        S::template start<INV>(Fc, R, n_r, n_th, n_ph, cy);
                     } else {
                         float* o = out_base + B->out_off + 3 * pix;
+                        const float a0 = ld_f(row, LS_ACC + 0), a1 = ld_f(row, LS_ACC + 1),
+                                    a2 = ld_f(row, LS_ACC + 2);
                         if (accumulate) {
-                            o[0] += acc[0];
-                            o[1] += acc[1];
-                            o[2] += acc[2];
+                            o[0] += a0;
+                            o[1] += a1;
+                            o[2] += a2;
                         } else {
-                            o[0] = acc[0];
-                            o[1] = acc[1];
-                            o[2] = acc[2];
+                            o[0] = a0;
+                            o[1] = a1;
+                            o[2] = a2;
                         }
                         has = false;
                     }
                 }
+            }
+            w_att64 += (uint64_t)__popcll(__ballot(rhs64 > 0));
+            w_lazy += (uint64_t)__popcll(__ballot(lazy_att != 0));
+            // the lane-row totals grow only when a ray ends
+            if (counters &&
+                __ballot(ended && (row[LS_N_STEPS] > 0x40000000u || row[LS_N_ATT] > 0x40000000u))) {
+                wave_add_u64(&counters->steps, row[LS_N_STEPS]);
+                wave_add_u64(&counters->attempts, row[LS_N_ATT]);
+                row[LS_N_STEPS] = 0;
+                row[LS_N_ATT] = 0;
             }
         }
         // every lane is idle: move to the next frame that still has work
@@ -1514,26 +1636,29 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
         if (next < 0) break;
         fid = __builtin_amdgcn_readfirstlane(next);
         done = (spp <= 0);
-        pix = 0;
-        chunk_end = 0;
+        st_u64(row, LS_PIX, 0);
+        st_u64(row, LS_CEND, 0);
     }
     if (counters) {
-        wave_add_u64(&counters->rays, c_rays);
-        wave_add_u64(&counters->steps, c_steps);
-        wave_add_u64(&counters->attempts, c_att);
+        // FP64 attempts run 12 right-hand sides (4 with fixed steps)
+        const uint64_t rhs_per64 = ADAPTIVE ? 12u : 4u;
+        wave_add_u64(&counters->rays, row[LS_N_RAYS]);
+        wave_add_u64(&counters->steps, row[LS_N_STEPS]);
+        wave_add_u64(&counters->attempts, row[LS_N_ATT]);
         wave_add_u64(&counters->rhs_evals, c_rhs);
-        wave_add_u64(&counters->captured, c_cap);
-        wave_add_u64(&counters->escaped, c_esc);
-        wave_add_u64(&counters->stepcap, c_cap_steps);
-        wave_add_u64(&counters->disk_hits, c_disk);
-        wave_add_u64(&counters->attempts_f64, c_att64);
-        wave_add_u64(&counters->rhs_evals_f64, c_rhs64);
-        wave_add_u64(&counters->attempts_lazy, c_lazy);
+        wave_add_u64(&counters->captured, row[LS_N_CAP]);
+        wave_add_u64(&counters->escaped, row[LS_N_ESC]);
+        wave_add_u64(&counters->stepcap, row[LS_N_STEPCAP]);
+        wave_add_u64(&counters->disk_hits, row[LS_N_DISK]);
+        wave_add_uniform_u64(&counters->attempts_f64, w_att64);
+        wave_add_uniform_u64(&counters->rhs_evals_f64, w_att64 * rhs_per64);
+        wave_add_uniform_u64(&counters->attempts_lazy, w_lazy);
     }
     if (INV && inv_max3) {
-        atomic_max_nonneg(&inv_max3[0], inv_lane[0]);
-        atomic_max_nonneg(&inv_max3[1], inv_lane[1]);
-        atomic_max_nonneg(&inv_max3[2], inv_lane[2]);
+        const uint32_t* iv = inv_row();
+        atomic_max_nonneg(&inv_max3[0], ld_f(iv, IS_INVL + 0));
+        atomic_max_nonneg(&inv_max3[1], ld_f(iv, IS_INVL + 1));
+        atomic_max_nonneg(&inv_max3[2], ld_f(iv, IS_INVL + 2));
     }
 }
 
@@ -1618,20 +1743,28 @@ __global__ void set_u32(uint32_t* __restrict__ p, const uint32_t v) {
 
 }  // namespace
 
+// Dynamic LDS of a launch: the lane rows, and the monitor region when the
+// invariant monitor is on.
+size_t lane_lds_bytes(bool inv, int block) {
+    return (size_t)block * sizeof(uint32_t) * (size_t)(kLaneSlots + (inv ? kInvSlots : 0));
+}
+
 int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
                 int variant, int spp, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
+    if (block <= 0 || block > 256 || block % 64 != 0) return NP_EINVAL;
     const void* fn = select_kernel(precision, adaptive, inv, variant);
     void* args[] = {&d_frames, &nframes, &spp, &d_out, &accumulate, &d_inv, &d_counters, &d_records,
                     &d_work, &chunk};
-    const hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+    const hipError_t e =
+        hipLaunchKernel(fn, dim3(grid), dim3(block), args, lane_lds_bytes(inv, block), stream);
     return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
 }
 
 int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu) {
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, select_kernel(precision, adaptive, inv, variant), block, 0);
+        blocks_per_cu, select_kernel(precision, adaptive, inv, variant), block, lane_lds_bytes(inv, block));
     return e == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

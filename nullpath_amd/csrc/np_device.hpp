@@ -54,6 +54,11 @@ __device__ __forceinline__ void wave_add_u64(uint64_t* dst, uint64_t v) {
     if (lane_id() == 0 && w) atomicAdd(reinterpret_cast<unsigned long long*>(dst), w);
 }
 
+// Add a wave-uniform total to *dst once (lane 0).
+__device__ __forceinline__ void wave_add_uniform_u64(uint64_t* dst, uint64_t v) {
+    if (lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
+}
+
 // Atomic maximum of a non-negative float through its bit pattern (the
 // ordering of non-negative IEEE floats equals that of their bits).
 __device__ __forceinline__ void atomic_max_nonneg(float* addr, float v) {

@@ -103,16 +103,18 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->rtol = p->rtol;
     f->atol = p->atol;
     f->crossing = p->crossing == NP_CROSSING_HERMITE ? 1 : 0;
-    // FP32C kernel variant = waves-per-SIMD cap: 3 by default, 4, or 0 (the
+    // FP32C kernel variant = waves-per-SIMD cap: 4 by default, 3, or 0 (the
     // compiler's register budget, asked for as 2)
     f->variant = (p->precision != NP_PREC_FP32C || p->waves_per_simd == 2) ? 0
-                 : (p->waves_per_simd == 4) ? 4 : 3;
+                 : (p->waves_per_simd == 3) ? 3 : 4;
     f->r_isl = p->fp64_radius > 0.0 ? p->fp64_radius : std::fmax(30.0 * M, 1.5 * p->disk_rout);
     if (p->estimate_every_step != 0 && p->estimate_every_step != 1) return NP_EINVAL;
     f->lazy = p->estimate_every_step == 0 ? 1 : 0;
     f->r_lazyf = (float)(100.0 * M);
     f->s_eq = std::sin(3.141592653589793238462643383279502884 * 0.5);
     f->c_eq = std::cos(3.141592653589793238462643383279502884 * 0.5);
+    f->s2_cam = std::sin(f->th_cam) * std::sin(f->th_cam);
+    f->c2_cam = std::cos(f->th_cam) * std::cos(f->th_cam);
     f->Mf = (float)M;
     f->af = (float)a;
     f->a2f = f->af * f->af;

@@ -18,6 +18,7 @@ struct FrameConst {
     double rtol, atol;
     double r_isl;        // FP32C: steps starting at r < r_isl run in FP64
     double s_eq, c_eq;   // sin, cos of pi/2 as the host libm returns them
+    double s2_cam, c2_cam;  // sin^2, cos^2 of th_cam (per-ray Carter-constant bound)
     // observer tetrad: p^mu components as functions of the local direction n
     //   p^t   = ct0 + n_phi * ct_ph
     //   p^r   = n_r * cr
