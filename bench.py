@@ -31,13 +31,14 @@ sys.path.insert(0, ROOT)
 METRIC = "Kerr rays/s @1080p a=0.9, FP32-comp vs FP64 at ≤1e-5 err; adjoint grads/s"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
-# HBM bytes of one FP32C render launch on the 1920x1080x4 frame (with ray
-# records), measured with rocprofv3 FETCH_SIZE and WRITE_SIZE and scaled by
-# what the counters read for a 256 MiB device copy (FETCH x2.00, WRITE x1.00;
-# profiles/kerr_render_1920x1080x4_kernel_trace.txt): 121.1 MB read + 667.7 MB
-# written.  The algorithmic bytes are the 24.9 MB image and 265.4 MB of ray
-# records; the rest is register-spill scratch of the persistent lanes.
-FP32C_HBM_BYTES_1080P4 = (59138.4 * 2.0 + 652010.8 * 1.0) * 1024.0
+# HBM bytes of one FP32C render launch on the 1920x1080x4 frame (default
+# 4-wave variant, with ray records), measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE and scaled by what the counters read for a 256 MiB device copy
+# (FETCH x2.00, WRITE x1.00; profiles/kerr_render_1920x1080x4_kernel_trace.txt):
+# 96.0 MB read + 588.2 MB written.  The algorithmic bytes are the 24.9 MB
+# image and 265.4 MB of ray records; the rest is register-spill scratch
+# around the rare FP64 attempts.
+FP32C_HBM_BYTES_1080P4 = (46868.6 * 2.0 + 574382.0 * 1.0) * 1024.0
 
 # Algorithmic operation counts per unit (add, sub, mul, div, sqrt, rcp = 1;
 # fma = 2; library sin/cos/pow counted apart), derived from the kernel source:
@@ -130,6 +131,12 @@ def parse():
     ap.add_argument("--crossing", default="hermite", choices=["hermite", "linear"])
     ap.add_argument("--waves-per-simd", type=int, default=0, choices=[0, 2, 3, 4],
                     help="FP32C kernel variant (0 = the library default, 4 waves per SIMD)")
+    ap.add_argument("--fp64c-waves-per-simd", type=int, default=0, choices=[0, 2, 3],
+                    help="FP64 comparator kernel variant (0 = the library default, 3 waves per SIMD; "
+                         "2 = the compiler's register budget)")
+    ap.add_argument("--fp64-waves-per-simd", type=int, default=0, choices=[0, 1, 2],
+                    help="reference-formula FP64 kernel variant (0 = the library default, 2 waves per "
+                         "SIMD; 1 = the compiler's register budget)")
     ap.add_argument("--linear-check", default="960x540x1",
                     help="WxHxSPP of the linear-crossing comparison; 'none' skips it")
     ap.add_argument("--adjoint", type=int, default=1,
@@ -835,18 +842,19 @@ def main() -> int:
     res = {}
     for prec in precs:
         k, w = (args.steps, args.warmup) if prec == "fp32c" or len(precs) == 1 else (args.fp64_steps, 0)
-        res[prec] = render_case(prec, args.crossing, args.width, args.height, args.spp, k, w,
-                                args.waves_per_simd)
+        waves_p = {"fp32c": args.waves_per_simd, "fp64c": args.fp64c_waves_per_simd,
+                   "fp64": args.fp64_waves_per_simd}[prec]
+        res[prec] = render_case(prec, args.crossing, args.width, args.height, args.spp, k, w, waves_p)
     head = "fp32c" if "fp32c" in res else precs[0]
     R = res[head]
     value = R["rays"] / R["wall"]
     waves = args.waves_per_simd or 4
     if head != "fp32c":
-        kernel = "kerr_render<StepF64, adaptive>"
+        kernel = "kerr_render_capped<StepF64, adaptive, 2 waves>"
     elif waves == 2:
         kernel = "kerr_render<StepF32C, adaptive>"
     else:
-        kernel = f"kerr_render_f32c_capped<adaptive, {waves} waves>"
+        kernel = f"kerr_render_capped<StepF32C, adaptive, {waves} waves>"
     achieved = (R["flops32"] + 2.0 * R["flops64"]) / R["mean_s"] / 1e12
     # the traffic constant was measured on the default kernel variant
     head_1080p4 = head == "fp32c" and waves == 4 and (args.width, args.height, args.spp) == (1920, 1080, 4)
@@ -878,7 +886,7 @@ def main() -> int:
             "frac": achieved / PEAK_TFLOPS["fp32"],
             "traffic": FP32C_HBM_BYTES_1080P4 if head_1080p4 else None,
             "traffic_note": ("HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE on this workload "
-                             "(copy-calibrated); mostly register-spill scratch of the persistent lanes"
+                             "(copy-calibrated); beyond the image and ray records, register-spill scratch around the rare FP64 attempts"
                              if head_1080p4 else "measured only for the 1920x1080x4 FP32C launch"),
             "bytes_algorithmic": 12 * R["npix"] + 32 * R["rays"] // args.steps,
             "kernel": kernel,

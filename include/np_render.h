@@ -78,11 +78,14 @@ typedef struct np_render_params {
     uint32_t seed;            /* jitter seed; 0 gives the reference's hash keys */
     double  fp64_radius;      /* NP_PREC_FP32C only: steps that start inside this radius (absolute
                                  units) run in FP64; 0 = auto, max(30 M, 1.5 disk_rout) */
-    int32_t waves_per_simd;   /* NP_PREC_FP32C kernel variant: 0 or 4 = capped at 128 VGPRs
-                                 (4 waves per SIMD; the float attempt runs without spills, the
-                                 rare FP64 branch spills), 3 = capped at 168 VGPRs (3 waves),
-                                 2 = the compiler's own register budget (2 waves per SIMD);
-                                 other values are rejected */
+    int32_t waves_per_simd;   /* kernel variant: the waves per SIMD it runs at, 0 = the default
+                                 (the fastest measured).  NP_PREC_FP32C: 0 or 4 = capped at 128
+                                 VGPRs (the float attempt runs without spills, the rare FP64
+                                 branch spills), 3 = capped at 168 VGPRs, 2 = the compiler's own
+                                 register budget.  NP_PREC_FP64_CARTER: 0 or 3 = capped at 168
+                                 VGPRs, 2 = the compiler's budget.  NP_PREC_FP64: 0 or 2 = capped
+                                 at 256 VGPRs, 1 = the compiler's budget (256 VGPRs + AGPRs).
+                                 Other values are rejected (NP_EINVAL) */
     int32_t crossing;         /* np_crossing: disk-crossing radius between accepted steps
                                  (reference: linear in theta) */
     int32_t estimate_every_step; /* NP_PREC_FP32C, adaptive: 0 = a ray's steps at the maximum

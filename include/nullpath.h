@@ -124,10 +124,10 @@ void np_pass_graph_destroy(np_pass_graph* graph);
 
 /*
  * Launch-shape choices of the batch renderer, picked by np_autotune_batch.
- * block_size: 64, 128 or 256 threads; waves_per_simd: FP32C kernel variant
- * (4 = 128-VGPR cap, 3 = 168-VGPR cap, 2 = compiler budget); refill_chunk: consecutive pixels
- * a lane takes per refill; fp64_radius: FP32C radius (absolute units) of
- * FP64 steps, 0 = each frame's own.
+ * block_size: 64, 128 or 256 threads; waves_per_simd: kernel variant as in
+ * np_render_params (FP32C: 4 = 128-VGPR cap, 3 = 168-VGPR cap, 2 = compiler budget), 0 = each
+ * frame's own; refill_chunk: consecutive pixels a lane takes per refill; fp64_radius: FP32C
+ * radius (absolute units) of FP64 steps, 0 = each frame's own.
  */
 typedef struct np_tuning {
     int32_t block_size;

@@ -487,7 +487,15 @@ __device__ __forceinline__ int after_accept(const FrameConst& F, typename S::Ray
         const double denom = fabs(d_prev) + fabs(d_curr) + 1e-30;
         const double alpha = fabs(d_prev) / denom;
         double r_cross = R.prev_r + alpha * (r - R.prev_r);
-        if (F.crossing == 1 && sign_change) {
+        // The cubic Hermite radius differs from the chord between the two
+        // accepted states by at most 4/27 (|n0 - dr| + |n1 - dr|), n = h
+        // dr/dlambda at either end, dr = r - prev_r.  Beyond the disk a null
+        // ray has |dr/dlambda| <= 1.3 E and h <= HMAX, so when both ends lie
+        // more than 2 HMAX (1 + 2E) outside r_out the refined radius cannot
+        // reach the disk either: the crossing adds nothing and is left at the
+        // chord (no FP64 end derivative).
+        const double r_refine = F.rout + 2.0 * HMAX * (1.0 + 2.0 * fabs(S::pt(R)));
+        if (F.crossing == 1 && sign_change && fmin(R.prev_r, r) <= r_refine) {
             double h, dr0, dth0, dr1, dth1;
             S::cross_derivs(F, R, h, dr0, dth0, dr1, dth1);
             r_cross = hermite_crossing(h, R.prev_r, R.prev_th, dr0, dth0, r, th, dr1, dth1, alpha, r_cross);
@@ -1670,17 +1678,19 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
 #define NP_KERNEL_PASS frames, nframes, spp, out, accumulate, inv_max3, counters, records, work, chunk
 
 // Kernel entry points.  Variant 0 leaves the register budget to the
-// compiler; the FP32C variants W = 3 and W = 4 ask for W waves per SIMD,
-// which caps the kernel at 168 or 128 VGPRs.
+// compiler; variant W asks for W waves per SIMD, which caps the kernel at
+// 512 / W VGPRs (rounded down to the allocation granule): FP32C W = 3, 4
+// (168, 128), the FP64 comparator W = 3, the reference-formula FP64 path
+// W = 2 (256, no AGPRs).
 template <class S, bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
     kerr_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
 }
 
-template <bool ADAPTIVE, bool INV, int W>
+template <class S, bool ADAPTIVE, bool INV, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
-kerr_render_f32c_capped(NP_KERNEL_ARGS) {
-    kerr_body<StepF32C, ADAPTIVE, INV>(NP_KERNEL_PASS);
+kerr_render_capped(NP_KERNEL_ARGS) {
+    kerr_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
 }
 
 #undef NP_KERNEL_ARGS
@@ -1692,26 +1702,31 @@ const void* kernel_ptr(bool inv) {
                : reinterpret_cast<const void*>(&kerr_render<S, A, false>);
 }
 
-template <bool A, int W>
+template <class S, bool A, int W>
 const void* kernel_ptr_capped(bool inv) {
-    return inv ? reinterpret_cast<const void*>(&kerr_render_f32c_capped<A, true, W>)
-               : reinterpret_cast<const void*>(&kerr_render_f32c_capped<A, false, W>);
+    return inv ? reinterpret_cast<const void*>(&kerr_render_capped<S, A, true, W>)
+               : reinterpret_cast<const void*>(&kerr_render_capped<S, A, false, W>);
 }
 
 template <bool A>
-const void* kernel_ptr_f32c(bool inv, int variant) {
-    if (variant == 4) return kernel_ptr_capped<A, 4>(inv);
-    if (variant == 3) return kernel_ptr_capped<A, 3>(inv);
-    return kernel_ptr<StepF32C, A>(inv);
+const void* kernel_ptr_variant(int precision, bool inv, int variant) {
+    if (precision == NP_PREC_FP32C) {
+        if (variant == 4) return kernel_ptr_capped<StepF32C, A, 4>(inv);
+        if (variant == 3) return kernel_ptr_capped<StepF32C, A, 3>(inv);
+        return kernel_ptr<StepF32C, A>(inv);
+    }
+    if (precision == NP_PREC_FP64_CARTER) {
+        if (variant == 3) return kernel_ptr_capped<StepC64, A, 3>(inv);
+        return kernel_ptr<StepC64, A>(inv);
+    }
+    if (variant == 2) return kernel_ptr_capped<StepF64, A, 2>(inv);
+    return kernel_ptr<StepF64, A>(inv);
 }
 
-// variant (FrameConst::variant): FP32C waves-per-SIMD cap, 0 = none.
+// variant (FrameConst::variant): waves-per-SIMD cap, 0 = none.
 const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
-    if (precision == NP_PREC_FP32C)
-        return adaptive ? kernel_ptr_f32c<true>(inv, variant) : kernel_ptr_f32c<false>(inv, variant);
-    if (precision == NP_PREC_FP64_CARTER)
-        return adaptive ? kernel_ptr<StepC64, true>(inv) : kernel_ptr<StepC64, false>(inv);
-    return adaptive ? kernel_ptr<StepF64, true>(inv) : kernel_ptr<StepF64, false>(inv);
+    return adaptive ? kernel_ptr_variant<true>(precision, inv, variant)
+                    : kernel_ptr_variant<false>(precision, inv, variant);
 }
 
 __global__ void init_frame_record(const BatchFrame rec, BatchFrame* __restrict__ dst,

@@ -147,7 +147,8 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, int sampl
         if (q.precision < NP_PREC_FP64 || q.precision > NP_PREC_FP64_CARTER) return NP_EINVAL;
         if (q.check_inv) return NP_EINVAL;
         if (t) {
-            q.waves_per_simd = t->waves_per_simd;
+            // 0 keeps the frame's own kernel variant
+            if (t->waves_per_simd != 0) q.waves_per_simd = t->waves_per_simd;
             if (t->fp64_radius > 0.0) q.fp64_radius = t->fp64_radius;
         }
         int err = make_frame(&q, &out[i].F);
@@ -165,7 +166,7 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, int sampl
 }
 
 int check_tuning(const np_tuning* tuning, np_tuning* t) {
-    *t = tuning ? *tuning : np_tuning{256, 3, 1, 0.0};
+    *t = tuning ? *tuning : np_tuning{256, 0, 1, 0.0};
     if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256) return NP_EINVAL;
     if (t->refill_chunk < 1 || t->refill_chunk > 1024) return NP_EINVAL;
     return NP_OK;

@@ -60,7 +60,14 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     if (!(std::fabs(p->spin) < 1.0f)) return NP_EINVAL;
     if (!(p->fov_y_deg > 0.0f && p->fov_y_deg < 179.0f)) return NP_EINVAL;
     if (p->max_steps < 0 || p->disk_max_orders < 1) return NP_EINVAL;
-    if (p->waves_per_simd != 0 && (p->waves_per_simd < 2 || p->waves_per_simd > 4)) return NP_EINVAL;
+    // kernel variants (waves per SIMD) that exist for each precision
+    {
+        const int w = p->waves_per_simd;
+        const bool ok = w == 0 || (p->precision == NP_PREC_FP32C && w >= 2 && w <= 4) ||
+                        (p->precision == NP_PREC_FP64_CARTER && (w == 2 || w == 3)) ||
+                        (p->precision == NP_PREC_FP64 && (w == 1 || w == 2));
+        if (!ok) return NP_EINVAL;
+    }
     if (p->integrator == NP_INTEGRATOR_ADAPTIVE && !(p->rtol > 0.0 && p->atol >= 0.0)) return NP_EINVAL;
 
     // render.cu:131-139 (float arithmetic)
@@ -103,10 +110,17 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->rtol = p->rtol;
     f->atol = p->atol;
     f->crossing = p->crossing == NP_CROSSING_HERMITE ? 1 : 0;
-    // FP32C kernel variant = waves-per-SIMD cap: 4 by default, 3, or 0 (the
-    // compiler's register budget, asked for as 2)
-    f->variant = (p->precision != NP_PREC_FP32C || p->waves_per_simd == 2) ? 0
-                 : (p->waves_per_simd == 3) ? 3 : 4;
+    // Kernel variant = the waves per SIMD the kernel runs at; 0 picks the
+    // fastest measured.  FP32C: 4 (default) or 3, capped; 2 is the compiler's
+    // own register budget.  FP64 comparator: 3 (default), capped; 2 the
+    // compiler's budget.  Reference-formula FP64: 2 (default), capped; 1 the
+    // compiler's budget (256 VGPRs + AGPRs).
+    if (p->precision == NP_PREC_FP32C)
+        f->variant = (p->waves_per_simd == 2) ? 0 : (p->waves_per_simd == 3) ? 3 : 4;
+    else if (p->precision == NP_PREC_FP64_CARTER)
+        f->variant = (p->waves_per_simd == 2) ? 0 : 3;
+    else
+        f->variant = (p->waves_per_simd == 1) ? 0 : 2;
     f->r_isl = p->fp64_radius > 0.0 ? p->fp64_radius : std::fmax(30.0 * M, 1.5 * p->disk_rout);
     if (p->estimate_every_step != 0 && p->estimate_every_step != 1) return NP_EINVAL;
     f->lazy = p->estimate_every_step == 0 ? 1 : 0;

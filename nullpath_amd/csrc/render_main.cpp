@@ -21,7 +21,7 @@
 //                              FILE.json = size, samples done, parameter signature)
 //   --resume FILE             (continue a single-frame render from a --state file)
 //   --fp64-radius X           (fp32c: FP64 steps inside X M; default max(30, 1.5 disk-rout))
-//   --waves-per-simd 2|3|4    (fp32c kernel variant)
+//   --waves-per-simd N        (kernel variant: waves per SIMD, 0 = default)
 //   --crossing linear|hermite (disk-crossing radius; default linear as the reference)
 //   --estimate-every-step     (fp32c: no far-field skip of the error estimate)
 #include <hip/hip_runtime.h>

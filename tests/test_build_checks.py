@@ -13,4 +13,4 @@ def test_code_objects_hold_kernels_only():
     nb.check_no_device_calls()
     kernels, others = nb.device_functions(os.path.join(nb.OBJ_DIR, "kerr_render.hip.o"))
     assert others == [] and len(kernels) >= 16
-    assert any("kerr_render_f32c_capped" in k for k in kernels)
+    assert any("kerr_render_capped" in k for k in kernels)
