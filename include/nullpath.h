@@ -63,6 +63,19 @@ typedef struct np_ray_record {
     double  r_end;     /* r when the ray stopped */
 } np_ray_record;
 
+/* Optional per-sample end values of the integrated coordinates (extension):
+ * the coordinate time t (dt/dlambda = g^{tt} p_t + g^{t phi} p_phi,
+ * bh_kerr.h:679-727, integrated with the state from t = 0 at the camera, as
+ * the reference's RK4 does), the azimuth phi, and the affine parameter
+ * lambda (the sum of the accepted step sizes), all at the last accepted
+ * state of the sample.  NP_PREC_FP32C accumulates t, phi and lambda in
+ * float-float with two-sum updates. */
+typedef struct np_ray_times {
+    double t;
+    double phi;
+    double lambda;
+} np_ray_times;
+
 /*
  * Trace `spp` samples (sample ids sample_offset .. sample_offset+spp-1) of
  * every pixel in rows [y0, y0+rows) of the frame described by *p, and write
@@ -80,6 +93,20 @@ typedef struct np_ray_record {
 int np_render_tile(const np_render_params* p, int y0, int rows, int spp, int sample_offset,
                    float* d_rgb, float* d_inv_max3, np_trace_counters* d_counters,
                    np_ray_record* d_records, void* stream);
+
+/*
+ * np_render_tile that also integrates the coordinate time and the affine
+ * parameter and writes one np_ray_times per traced sample to d_times
+ * (device, rows*width*spp entries, the indexing of d_records).  It has no
+ * counterpart in the reference, whose integrator advances t with the state
+ * (bh_kerr.h:734-774, 987-1032) but whose renderer never reads it.  The
+ * image, records and counters are those of np_render_tile; the launch runs
+ * the precision's default kernel variant (p->waves_per_simd is ignored).
+ * NP_EINVAL when d_times is NULL or p->check_inv is set.
+ */
+int np_render_tile_times(const np_render_params* p, int y0, int rows, int spp, int sample_offset,
+                         float* d_rgb, np_trace_counters* d_counters, np_ray_record* d_records,
+                         np_ray_times* d_times, void* stream);
 
 /*
  * Progressive render of a whole frame into a host accumulation buffer

@@ -95,6 +95,24 @@ def render_tile(p: RenderParams, out, y0=0, rows=None, spp=1, sample_offset=0, i
     check(code, "np_render_tile")
 
 
+def render_tile_times(p: RenderParams, out, times, y0=0, rows=None, spp=1, sample_offset=0,
+                      counters=None, records=None, stream=None) -> None:
+    """np_render_tile_times: render_tile that also writes each sample's end
+    coordinate time, azimuth and affine parameter into `times` (device buffer
+    of rows*width*spp np_ray_times, 24 bytes each; view it with
+    ray_times_dtype())."""
+    rows = p.height - y0 if rows is None else rows
+    if out.numel() < rows * p.width * 3 or out.dtype.itemsize != 4 or not out.is_contiguous():
+        raise NullPathError("out must be a contiguous float32 device tensor of rows*width*3")
+    if times.numel() * times.element_size() < rows * p.width * spp * 24 or not times.is_contiguous():
+        raise NullPathError("times must be a contiguous device buffer of rows*width*spp*24 bytes")
+    code = lib().np_render_tile_times(p, y0, rows, spp, sample_offset, out.data_ptr(),
+                                      None if counters is None else counters.data_ptr(),
+                                      None if records is None else records.data_ptr(),
+                                      times.data_ptr(), _stream_ptr(stream))
+    check(code, "np_render_tile_times")
+
+
 def new_counters(device="cuda"):
     """A zeroed device buffer for np_trace_counters (uint64 fields as int64)."""
     import torch

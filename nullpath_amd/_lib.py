@@ -32,7 +32,7 @@ class RenderParams(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
         ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
-        ("crossing", C.c_int32), ("estimate_every_step", C.c_int32),
+        ("crossing", C.c_int32), ("estimate_every_step", C.c_int32), ("renorm_pr", C.c_int32),
     ]
 
     def copy(self) -> "RenderParams":
@@ -86,6 +86,12 @@ def ray_record_dtype():
                      ("attempts", np.int32), ("min_r", np.float64), ("r_end", np.float64)])
 
 
+# np_ray_times as a numpy dtype (24 bytes)
+def ray_times_dtype():
+    import numpy as np
+    return np.dtype([("t", np.float64), ("phi", np.float64), ("lam", np.float64)])
+
+
 def lib() -> C.CDLL:
     """Load libnullpath.so (raises NullPathError when it is absent)."""
     global _lib
@@ -129,6 +135,8 @@ def _declare(L: C.CDLL) -> None:
     L.np_status_string.restype = C.c_char_p
     L.np_render_tile.argtypes = [C.POINTER(RenderParams), i32, i32, i32, i32, vp, vp, vp, vp, vp]
     L.np_render_tile.restype = i32
+    L.np_render_tile_times.argtypes = [C.POINTER(RenderParams), i32, i32, i32, i32, vp, vp, vp, vp, vp]
+    L.np_render_tile_times.restype = i32
     L.np_render_accumulate.argtypes = [C.POINTER(RenderParams), i32, i32, i32, i32, vp, vp]
     L.np_render_accumulate.restype = i32
     i64, d = C.c_int64, C.c_double

@@ -36,19 +36,21 @@ constexpr double HMAX = 0.2;      // BH_RK_HMAX (bh_common.h:253)
 constexpr double SAFETY = 0.9;    // BH_RK_SAFETY (bh_common.h:236)
 constexpr float STEP0 = 0.01f;    // BH_INTEGRATION_STEP_DEFAULT (bh_common.h:213)
 
+// Right-hand side of the integrated coordinates and momenta; t (dt/dlambda)
+// is formed by every field but only the kernels that report the end time
+// (TIME) integrate it -- elsewhere it is dead code.
 struct Deriv {
-    double r, th, ph, pr, pth;
+    double r, th, ph, pr, pth, t;
 };
 
 // computeGeodesicDerivativesKerrD (bh_kerr.h:679-727) with the metric
 // blocks of bh_kerr.h:600-617 and 624-672 inlined, one sincos shared.
-// g^tt only feeds dt/dlambda, which the renderer never reads.
 __device__ __forceinline__ Deriv kerr_rhs(const double M, const double a, const double r_plus,
                                           const double r, const double th, const double pt,
                                           const double pr, const double pth, const double pph) {
     Deriv d;
     if (r <= r_plus + 1e-12 || r <= 1e-12) {
-        d.r = d.th = d.ph = d.pr = d.pth = 0.0;
+        d.r = d.th = d.ph = d.pr = d.pth = d.t = 0.0;
         return d;
     }
     double s, c;
@@ -63,6 +65,8 @@ __device__ __forceinline__ Deriv kerr_rhs(const double M, const double a, const 
     const double gthth = 1.0 / SigmaC;
     const double gtphi = -2.0 * M * a * r / fmax(1e-12, SigmaC * Delta);
     const double gphiphi = (Delta - a2 * st2c) / fmax(1e-12, SigmaC * Delta * st2c);
+    const double Ac = (r2 + a2) * (r2 + a2) - a2 * Delta * st2c;
+    const double gtt = -Ac / fmax(1e-12, SigmaC * Delta);
     // kerr_blocks_contra_derivs_d
     const double s2 = (s * s > 1e-30 ? s * s : 1e-30);
     const double B = r2 + a2;
@@ -99,6 +103,7 @@ __device__ __forceinline__ Deriv kerr_rhs(const double M, const double a, const 
     const double gphiphi_r = (N_r * Den - N * Den_r) / Den2;
     const double gphiphi_th = (N_th * Den - N * Den_th) / Den2;
 
+    d.t = gtt * pt + gtphi * pph;
     d.r = grr * pr;
     d.th = gthth * pth;
     d.ph = gtphi * pt + gphiphi * pph;
@@ -120,28 +125,33 @@ __device__ __forceinline__ Deriv kerr_rhs(const double M, const double a, const 
 }
 
 struct Y5 {
-    double r, th, ph, pr, pth;
+    double r, th, ph, pr, pth, t;
 };
 
 // Classic RK4 step from y with a precomputed k1 = f(y); the stage
 // combination k1 + 2k2 + 2k3 + k4 is accumulated with the compensated sum
 // in the reference's order (bh_rk4_sum, bh_common.h:485-493) as the stages
 // are produced, so only the running sums stay live.
+template <bool TIME>
 __device__ __forceinline__ Y5 rk4_from_k1(const FrameConst& F, const Y5& y, const Deriv& k1,
                                           const double pt, const double pph, const double h,
                                           int& rhs_count) {
-    Klein<double> sr, sth, sph, spr, spth;
+    Klein<double> sr, sth, sph, spr, spth, st;
     sr.add(k1.r); sth.add(k1.th); sph.add(k1.ph); spr.add(k1.pr); spth.add(k1.pth);
+    if (TIME) st.add(k1.t);
     const double hh = 0.5 * h;
     Deriv k = kerr_rhs(F.M, F.a, F.r_plus, y.r + hh * k1.r, y.th + hh * k1.th, pt,
                        y.pr + hh * k1.pr, y.pth + hh * k1.pth, pph);
     sr.add(2.0 * k.r); sth.add(2.0 * k.th); sph.add(2.0 * k.ph); spr.add(2.0 * k.pr); spth.add(2.0 * k.pth);
+    if (TIME) st.add(2.0 * k.t);
     k = kerr_rhs(F.M, F.a, F.r_plus, y.r + hh * k.r, y.th + hh * k.th, pt,
                  y.pr + hh * k.pr, y.pth + hh * k.pth, pph);
     sr.add(2.0 * k.r); sth.add(2.0 * k.th); sph.add(2.0 * k.ph); spr.add(2.0 * k.pr); spth.add(2.0 * k.pth);
+    if (TIME) st.add(2.0 * k.t);
     k = kerr_rhs(F.M, F.a, F.r_plus, y.r + h * k.r, y.th + h * k.th, pt,
                  y.pr + h * k.pr, y.pth + h * k.pth, pph);
     sr.add(k.r); sth.add(k.th); sph.add(k.ph); spr.add(k.pr); spth.add(k.pth);
+    if (TIME) st.add(k.t);
     rhs_count += 3;
     const double h6 = h / 6.0;
     Y5 o;
@@ -150,6 +160,7 @@ __device__ __forceinline__ Y5 rk4_from_k1(const FrameConst& F, const Y5& y, cons
     o.ph = y.ph + h6 * sph.result();
     o.pr = y.pr + h6 * spr.result();
     o.pth = y.pth + h6 * spth.result();
+    o.t = TIME ? y.t + h6 * st.result() : 0.0;
     return o;
 }
 
@@ -158,12 +169,14 @@ __device__ __forceinline__ Y5 rk4_from_k1(const FrameConst& F, const Y5& y, cons
 // remaining right-hand sides run in a uniform, non-unrolled stage loop so
 // the kernel holds one copy of the RHS code and one stage's temporaries;
 // every product, compensated sum and update is the one rk4_step_d makes.
+template <bool TIME>
 __device__ __forceinline__ void step_doubling_f64(const FrameConst& F, const Y5& y, const Deriv& k1,
                                                   const double pt, const double pph, const double h,
                                                   Y5& full, Y5& half1, Y5& half2) {
     const double h2 = 0.5 * h;
-    Klein<double> sr, sth, sph, spr, spth;
+    Klein<double> sr, sth, sph, spr, spth, st;
     sr.add(k1.r); sth.add(k1.th); sph.add(k1.ph); spr.add(k1.pr); spth.add(k1.pth);
+    if (TIME) st.add(k1.t);
     Deriv kp = k1;
     Y5 base = y;
 #pragma clang loop unroll(disable)
@@ -183,9 +196,14 @@ __device__ __forceinline__ void step_doubling_f64(const FrameConst& F, const Y5&
             sr = Klein<double>(); sth = Klein<double>(); sph = Klein<double>();
             spr = Klein<double>(); spth = Klein<double>();
             sr.add(k.r); sth.add(k.th); sph.add(k.ph); spr.add(k.pr); spth.add(k.pth);
+            if (TIME) {
+                st = Klein<double>();
+                st.add(k.t);
+            }
         } else {
             const double w = (j == 2 || j == 5 || j == 9) ? 1.0 : 2.0;
             sr.add(w * k.r); sth.add(w * k.th); sph.add(w * k.ph); spr.add(w * k.pr); spth.add(w * k.pth);
+            if (TIME) st.add(w * k.t);
         }
         kp = k;
         if (j == 2 || j == 5 || j == 9) {
@@ -197,11 +215,16 @@ __device__ __forceinline__ void step_doubling_f64(const FrameConst& F, const Y5&
             o.ph = b.ph + h6 * sph.result();
             o.pr = b.pr + h6 * spr.result();
             o.pth = b.pth + h6 * spth.result();
+            o.t = TIME ? b.t + h6 * st.result() : 0.0;
             if (j == 2) {
                 full = o;
                 sr = Klein<double>(); sth = Klein<double>(); sph = Klein<double>();
                 spr = Klein<double>(); spth = Klein<double>();
                 sr.add(k1.r); sth.add(k1.th); sph.add(k1.ph); spr.add(k1.pr); spth.add(k1.pth);
+                if (TIME) {
+                    st = Klein<double>();
+                    st.add(k1.t);
+                }
                 kp = k1;
             } else if (j == 5) {
                 half1 = o;
@@ -576,6 +599,7 @@ struct StepF64 {
         Deriv k1;
         double pt, pph;   // read by every right-hand side
         double h_adapt, h;
+        double lam;       // affine parameter (TIME kernels)
         bool k1_valid;
     };
 
@@ -586,6 +610,8 @@ struct StepF64 {
         R.y.r = F.r_cam;
         R.y.th = F.th_cam;
         R.y.ph = F.ph_cam;
+        R.y.t = 0.0;
+        R.lam = 0.0;
         R.h_adapt = 0.0;
         R.h = 0.0;
         R.k1_valid = false;
@@ -594,6 +620,12 @@ struct StepF64 {
 
     __device__ static double pt(const Ray& R) { return R.pt; }
     __device__ static double pph(const Ray& R) { return R.pph; }
+    __device__ static double t(const Ray& R) { return R.y.t; }
+    __device__ static double phi(const Ray& R) { return R.y.ph; }
+    __device__ static double lam(const Ray& R) { return R.lam; }
+    // an accepted step of size R.h advances the affine parameter
+    __device__ static void advance(Ray& R) { R.lam += R.h; }
+    __device__ static void post_accept(const FrameConst&, Ray&) {}
 
     __device__ static double r(const Ray& R) { return R.y.r; }
     __device__ static double th(const Ray& R) { return R.y.th; }
@@ -616,7 +648,7 @@ struct StepF64 {
 
     // One integrator attempt.  Returns 1 accepted, 0 rejected (retry),
     // 2 failed (the ray is captured: horizon, pole or rejection budget).
-    template <bool ADAPTIVE>
+    template <bool ADAPTIVE, bool TIME>
     __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& /*rhs64*/,
                                   int& /*lazy*/) {
         const double Rs = F.Rs_d;
@@ -631,7 +663,7 @@ struct StepF64 {
             R.k1 = k1;
             R.k1_valid = false;
             R.h = h;
-            R.y = rk4_from_k1(F, R.y, k1, R.pt, R.pph, h, rhs_count);
+            R.y = rk4_from_k1<TIME>(F, R.y, k1, R.pt, R.pph, h, rhs_count);
             const bool ok = R.y.r > F.r_plus + 1e-9 && R.y.th > 1e-9 && R.y.th < NP_PI_D - 1e-9;
             return ok ? 1 : 2;
         }
@@ -651,7 +683,7 @@ struct StepF64 {
         }
         // rk4_adaptive_step_d (bh_kerr.h:1058-1105)
         Y5 full, half1, half2;
-        step_doubling_f64(F, R.y, R.k1, R.pt, R.pph, h, full, half1, half2);
+        step_doubling_f64<TIME>(F, R.y, R.k1, R.pt, R.pph, h, full, half1, half2);
         rhs_count += 10;
         const double lim = NP_PI_D - 1e-9;
         const bool ok1 = full.r > F.r_plus + 1e-12 && full.th > 1e-9 && full.th < lim;
@@ -746,11 +778,11 @@ __device__ __forceinline__ float ff_diff(ff x, ff y) {
 }
 
 struct DerivF {
-    float r, th, ph, pr, pth;
+    float r, th, ph, pr, pth, t;
 };
 
 struct FF5 {
-    ff r, th, ph, pr, pth;
+    ff r, th, ph, pr, pth, t;
 };
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -766,7 +798,7 @@ __device__ __forceinline__ DerivF kerr_rhs_f32_sc(const FrameConst& F, float r, 
                                                   float thd, float pr, float pth, float E, float L) {
     DerivF d;
     if (r <= F.rpf_guard) {
-        d.r = d.th = d.ph = d.pr = d.pth = 0.0f;
+        d.r = d.th = d.ph = d.pr = d.pth = d.t = 0.0f;
         return d;
     }
     const float d2 = thd * thd;
@@ -790,6 +822,8 @@ __device__ __forceinline__ DerivF kerr_rhs_f32_sc(const FrameConst& F, float r, 
     d.ph = (a * (2.0f * M * r * E - a * L) * iD + L * is2) * iS;
     d.pr = ((r - M) * Q * iD - P * X * (iD * iD)) * iS;
     d.pth = c * s * w * (L * is2 + a * E) * iS;
+    // dt/dlambda = ((r^2 + a^2) P / Delta + a (L - a E sin^2)) / Sigma
+    d.t = ((r2 + a2) * P * iD + a * (L - a * E * s2)) * iS;
     return d;
 }
 
@@ -808,7 +842,7 @@ __device__ __forceinline__ Deriv kerr_rhs_c64(const FrameConst& F, double r, dou
                                               double pth, double E, double L) {
     Deriv d;
     if (r <= F.r_plus + 1e-12) {
-        d.r = d.th = d.ph = d.pr = d.pth = 0.0;
+        d.r = d.th = d.ph = d.pr = d.pth = d.t = 0.0;
         return d;
     }
     double s, c;
@@ -829,12 +863,13 @@ __device__ __forceinline__ Deriv kerr_rhs_c64(const FrameConst& F, double r, dou
     d.ph = (a * (2.0 * M * r * E - a * L) * iD + L * is2) * iS;
     d.pr = ((r - M) * Q * iD - P * X * (iD * iD)) * iS;
     d.pth = c * s * w * (L * is2 + a * E) * iS;
+    d.t = ((r2 + a2) * P * iD + a * (L - a * E * s2)) * iS;
     return d;
 }
 
 // FP64 image of a float-float state.
 __device__ __forceinline__ Y5 ff5_to_y5(const FF5& y) {
-    return Y5{ff_to_d(y.r), ff_to_d(y.th), ff_to_d(y.ph), ff_to_d(y.pr), ff_to_d(y.pth)};
+    return Y5{ff_to_d(y.r), ff_to_d(y.th), ff_to_d(y.ph), ff_to_d(y.pr), ff_to_d(y.pth), ff_to_d(y.t)};
 }
 
 __device__ __forceinline__ bool in_domain_d(const FrameConst& F, const Y5& y) {
@@ -850,12 +885,13 @@ __device__ __forceinline__ bool in_domain_d(const FrameConst& F, const Y5& y) {
 // domain.
 // Returns true when all three results are inside the domain; k1r, k1th are
 // the r and theta components of f(y).
+template <bool TIME>
 __device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5& y, const double E,
                                                   const double L, const double h, Y5& half2,
                                                   double& k1r, double& k1th) {
     Deriv S, kp;
     Y5 base = ff5_to_y5(y);
-    S.r = S.th = S.ph = S.pr = S.pth = 0.0;
+    S.r = S.th = S.ph = S.pr = S.pth = S.t = 0.0;
     kp = S;
     bool ok = true;
     uint32_t* row = lane_row();
@@ -875,6 +911,7 @@ __device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5
             S = k;
         } else {
             S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
+            if (TIME) S.t += w * k.t;
         }
         kp = k;
         if (st == 3) {
@@ -885,6 +922,7 @@ __device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5
             o.ph = base.ph + h6 * S.ph;
             o.pr = base.pr + h6 * S.pr;
             o.pth = base.pth + h6 * S.pth;
+            o.t = TIME ? base.t + h6 * S.t : 0.0;
             ok = ok && in_domain_d(F, o);
             if (run == 0) {
                 st_d(row, LS_FULL + 0, o.r);
@@ -903,6 +941,7 @@ __device__ __forceinline__ bool step_doubling_c64(const FrameConst& F, const FF5
     return ok;
 }
 
+template <bool TIME>
 __device__ __forceinline__ Y5 rk4_c64(const FrameConst& F, const Y5& y, const double E, const double L,
                                       const double h, double& k1r, double& k1th) {
     const double hh = 0.5 * h;
@@ -922,9 +961,11 @@ __device__ __forceinline__ Y5 rk4_c64(const FrameConst& F, const Y5& y, const do
     o.ph = y.ph + h6 * (k1.ph + 2.0 * (k2.ph + k3.ph) + k4.ph);
     o.pr = y.pr + h6 * (k1.pr + 2.0 * (k2.pr + k3.pr) + k4.pr);
     o.pth = y.pth + h6 * (k1.pth + 2.0 * (k2.pth + k3.pth) + k4.pth);
+    o.t = TIME ? y.t + h6 * (k1.t + 2.0 * (k2.t + k3.t) + k4.t) : 0.0;
     return o;
 }
 
+template <bool TIME>
 __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const DerivF& k1, float E,
                                        float L, float h, int& rhs_count) {
     const float hh = 0.5f * h;
@@ -945,6 +986,7 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
     o.ph = ff_add(y.ph, h6 * (k1.ph + 2.0f * (k2.ph + k3.ph) + k4.ph));
     o.pr = ff_add(y.pr, h6 * (k1.pr + 2.0f * (k2.pr + k3.pr) + k4.pr));
     o.pth = ff_add(y.pth, h6 * (k1.pth + 2.0f * (k2.pth + k3.pth) + k4.pth));
+    if (TIME) o.t = ff_add(y.t, h6 * (k1.t + 2.0f * (k2.t + k3.t) + k4.t));
     return o;
 }
 
@@ -958,6 +1000,7 @@ __device__ __forceinline__ FF5 rk4_f32(const FrameConst& F, const FF5& y, const 
 // full_step (wave-uniform) = false skips stages 0-2 (the full step), for a
 // wave whose lanes all skip the error estimate; inc, full_r and full_th are
 // then left unset.
+template <bool TIME>
 __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5& y, const DerivF& k1,
                                                   const float s0, const float c0, const float E,
                                                   const float L, const float h, const bool full_step,
@@ -980,6 +1023,7 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
         } else {
             const float w = (j == 2 || j == 5 || j == 9) ? 1.0f : 2.0f;
             S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
+            if (TIME) S.t += w * k.t;
         }
         kp = k;
         if (j == 2) {
@@ -998,6 +1042,7 @@ __device__ __forceinline__ void step_doubling_f32(const FrameConst& F, const FF5
             o.ph = ff_add(b.ph, h6 * S.ph);
             o.pr = ff_add(b.pr, h6 * S.pr);
             o.pth = ff_add(b.pth, h6 * S.pth);
+            if (TIME) o.t = ff_add(b.t, h6 * S.t);
             if (j == 5) {
                 half1 = o;
                 br = ff_f(o.r); bthl = o.th.lo; bpr = ff_f(o.pr); bpth = ff_f(o.pth);
@@ -1043,8 +1088,9 @@ constexpr float kLazyError = 1e-3f;
 
 struct StepF32C {
     struct Ray : RayBase {
-        FF5 y;
+        FF5 y;            // y.t, the coordinate time, is integrated by TIME kernels only
         DerivF k1;
+        ff lam;           // affine parameter, compensated sum of the accepted steps (TIME)
         float E, L, h_adapt, h;
         // sqrt(Q + a^2 E^2) >= |p_theta| anywhere on the ray (Q = Carter's
         // constant): bounds how far one step can move theta
@@ -1073,6 +1119,8 @@ struct StepF32C {
         R.y.ph = ff_from_d(F.ph_cam);
         R.y.pr = ff_from_d(pr);
         R.y.pth = ff_from_d(pth);
+        R.y.t = ff{0.0f, 0.0f};
+        R.lam = ff{0.0f, 0.0f};
         R.E = (float)(-pt);
         R.L = (float)pph;
         R.h_adapt = 0.0f;
@@ -1092,9 +1140,36 @@ struct StepF32C {
     __device__ static double th(const Ray& R) { return ff_to_d(R.y.th); }
     __device__ static double pr(const Ray& R) { return ff_to_d(R.y.pr); }
     __device__ static double pth(const Ray& R) { return ff_to_d(R.y.pth); }
+    __device__ static double t(const Ray& R) { return ff_to_d(R.y.t); }
+    __device__ static double phi(const Ray& R) { return ff_to_d(R.y.ph); }
+    __device__ static double lam(const Ray& R) { return ff_to_d(R.lam); }
+    // an accepted step of size R.h advances the affine parameter (two-sum)
+    __device__ static void advance(Ray& R) { R.lam = ff_add(R.lam, R.h); }
+
+    // With F.renorm, p_r is set from the constraint H = 0 after every
+    // accepted step: in the Carter form 2 Sigma H = Delta p_r^2 - P^2/Delta +
+    // p_theta^2 + (L/sin - a E sin)^2, so Delta^2 p_r^2 = P^2 - Delta K with
+    // K = p_theta^2 + (L/sin - a E sin)^2.  Only where radial motion
+    // dominates (4 Delta^2 p_r^2 >= P^2): near a radial turning point the
+    // square root would amplify the rounding of K.  FP64 on the float-float
+    // state; the sign of p_r is kept.
+    __device__ static void post_accept(const FrameConst& F, Ray& R) {
+        if (!F.renorm) return;
+        const double r = ff_to_d(R.y.r), th = ff_to_d(R.y.th), pr = ff_to_d(R.y.pr), pth = ff_to_d(R.y.pth);
+        const double E = -pt(R), L = pph(R), a = F.a, a2 = a * a;
+        double s, c;
+        sincos_d(th, s, c);
+        const double Delta = r * r - 2.0 * F.M * r + a2;
+        const double P = (r * r + a2) * E - a * L;
+        const double w = L / s - a * E * s;
+        const double Rr = P * P - Delta * (pth * pth + w * w);
+        if (Delta > 0.0 && Rr > 0.0 && 4.0 * Delta * Delta * pr * pr >= P * P)
+            R.y.pr = ff_from_d(copysign(sqrt(Rr), pr) / Delta);
+    }
 
     __device__ static FF5 to_ff5(const Y5& y) {
-        return FF5{ff_from_d(y.r), ff_from_d(y.th), ff_from_d(y.ph), ff_from_d(y.pr), ff_from_d(y.pth)};
+        return FF5{ff_from_d(y.r), ff_from_d(y.th), ff_from_d(y.ph), ff_from_d(y.pr), ff_from_d(y.pth),
+                   ff_from_d(y.t)};
     }
 
     // Step size and (r, theta) derivatives at both ends of the step just
@@ -1119,11 +1194,12 @@ struct StepF32C {
     // retry loop of render.cu:193-205) on the FP64 image of the state.
     // Step sizes are held in float (a 6e-8 relative change of h); so is the
     // step factor.
+    template <bool TIME>
     __device__ static int attempt64(const FrameConst& F, Ray& R, int& rhs64) {
         const double h = (double)R.h;
         Y5 half2;
         double k1r, k1th;
-        const bool ok = step_doubling_c64(F, R.y, -pt(R), pph(R), h, half2, k1r, k1th);
+        const bool ok = step_doubling_c64<TIME>(F, R.y, -pt(R), pph(R), h, half2, k1r, k1th);
         rhs64 += 12;
         float h_try = R.h;
         float fac = 0.0f;
@@ -1163,7 +1239,7 @@ struct StepF32C {
         return 0;
     }
 
-    template <bool ADAPTIVE>
+    template <bool ADAPTIVE, bool TIME>
     __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& rhs64,
                                   int& lazy_att) {
         const float hmin = (float)HMIN, hmax = (float)HMAX;
@@ -1174,7 +1250,7 @@ struct StepF32C {
             R.h = h;
             if (ff_to_d(R.y.r) < F.r_isl) {
                 double k1r, k1th;
-                R.y = to_ff5(rk4_c64(F, ff5_to_y5(R.y), -pt(R), pph(R), (double)h, k1r, k1th));
+                R.y = to_ff5(rk4_c64<TIME>(F, ff5_to_y5(R.y), -pt(R), pph(R), (double)h, k1r, k1th));
                 R.k1.r = (float)k1r;
                 R.k1.th = (float)k1th;
                 rhs64 += 4;
@@ -1182,7 +1258,7 @@ struct StepF32C {
                 const DerivF k1 = kerr_rhs_f32(F, rf, R.y.th.hi, R.y.th.lo, ff_f(R.y.pr), ff_f(R.y.pth), R.E, R.L);
                 rhs_count += 1;
                 R.k1 = k1;
-                R.y = rk4_f32(F, R.y, k1, R.E, R.L, h, rhs_count);
+                R.y = rk4_f32<TIME>(F, R.y, k1, R.E, R.L, h, rhs_count);
             }
             const double rr = ff_to_d(R.y.r), tt = ff_to_d(R.y.th);
             const bool ok = rr > F.r_plus + 1e-9 && tt > 1e-9 && tt < NP_PI_D - 1e-9;
@@ -1199,7 +1275,7 @@ struct StepF32C {
         }
         if (R.mode != 0) {
             R.skip = 0;
-            return attempt64(F, R, rhs64);
+            return attempt64<TIME>(F, R, rhs64);
         }
         const float h = R.h;
         float s0, c0;
@@ -1227,7 +1303,7 @@ struct StepF32C {
         // the full step runs when any float lane of the wave estimates its
         // error (a uniform branch: a lane's own result never depends on it)
         const bool full_step = __ballot(!lazy) != 0;
-        step_doubling_f32(F, R.y, R.k1, s0, c0, R.E, R.L, h, full_step, inc, full_r, full_th, half1,
+        step_doubling_f32<TIME>(F, R.y, R.k1, s0, c0, R.E, R.L, h, full_step, inc, full_r, full_th, half1,
                           half2);
         rhs_count += full_step ? 10 : 7;
         lazy_att = full_step ? 0 : 1;
@@ -1280,6 +1356,7 @@ struct StepF32C {
 // all in double.  It exists to time the precision alone: FP32C against this
 // path differs only in the arithmetic type.
 // ---------------------------------------------------------------------------
+template <bool TIME>
 __device__ __forceinline__ void step_doubling_c64_lazy(const FrameConst& F, const Y5& y, const Deriv& k1,
                                                        const double E, const double L, const double h,
                                                        const bool full_step, Y5& full, Y5& half1,
@@ -1299,6 +1376,7 @@ __device__ __forceinline__ void step_doubling_c64_lazy(const FrameConst& F, cons
         } else {
             const double w = (j == 2 || j == 5 || j == 9) ? 1.0 : 2.0;
             S.r += w * k.r; S.th += w * k.th; S.ph += w * k.ph; S.pr += w * k.pr; S.pth += w * k.pth;
+            if (TIME) S.t += w * k.t;
         }
         kp = k;
         if (j == 2 || j == 5 || j == 9) {
@@ -1310,6 +1388,7 @@ __device__ __forceinline__ void step_doubling_c64_lazy(const FrameConst& F, cons
             o.ph = base.ph + h6 * S.ph;
             o.pr = base.pr + h6 * S.pr;
             o.pth = base.pth + h6 * S.pth;
+            o.t = TIME ? base.t + h6 * S.t : 0.0;
             if (j == 2) {
                 full = o;
                 S = k1;
@@ -1330,11 +1409,17 @@ struct StepC64 {
         Deriv k1;
         double pt, pph;   // read by every right-hand side
         double h_adapt, h, qb;
+        double lam;       // affine parameter (TIME kernels)
         bool k1_valid, skip;
     };
 
     __device__ static double pt(const Ray& R) { return R.pt; }
     __device__ static double pph(const Ray& R) { return R.pph; }
+    __device__ static double t(const Ray& R) { return R.y.t; }
+    __device__ static double phi(const Ray& R) { return R.y.ph; }
+    __device__ static double lam(const Ray& R) { return R.lam; }
+    __device__ static void advance(Ray& R) { R.lam += R.h; }
+    __device__ static void post_accept(const FrameConst&, Ray&) {}
 
     template <bool INV>
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
@@ -1343,6 +1428,8 @@ struct StepC64 {
         R.y.r = F.r_cam;
         R.y.th = F.th_cam;
         R.y.ph = F.ph_cam;
+        R.y.t = 0.0;
+        R.lam = 0.0;
         R.h_adapt = 0.0;
         R.h = 0.0;
         R.k1_valid = false;
@@ -1369,7 +1456,7 @@ struct StepC64 {
         dth1 = R.k1.th;
     }
 
-    template <bool ADAPTIVE>
+    template <bool ADAPTIVE, bool TIME>
     __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& /*rhs64*/,
                                   int& lazy_att) {
         const double E = -R.pt, L = R.pph;
@@ -1378,7 +1465,7 @@ struct StepC64 {
             const double scale = fmin(fmax(1.0, R.y.r / (0.5 * Rs)), 100.0);
             const double h = (double)STEP0 * scale;
             double k1r, k1th;
-            R.y = rk4_c64(F, R.y, E, L, h, k1r, k1th);
+            R.y = rk4_c64<TIME>(F, R.y, E, L, h, k1r, k1th);
             R.k1.r = k1r;
             R.k1.th = k1th;
             R.k1_valid = false;
@@ -1409,7 +1496,7 @@ struct StepC64 {
         R.skip = lazy;
         const bool full_step = __ballot(!lazy) != 0;
         Y5 full, half1, half2;
-        step_doubling_c64_lazy(F, R.y, R.k1, E, L, h, full_step, full, half1, half2);
+        step_doubling_c64_lazy<TIME>(F, R.y, R.k1, E, L, h, full_step, full, half1, half2);
         rhs_count += full_step ? 10 : 7;
         lazy_att = full_step ? 0 : 1;
         const double lim = NP_PI_D - 1e-9;
@@ -1470,13 +1557,16 @@ __device__ __forceinline__ const FrameConst& cold(const BatchFrame* B) {
     return p->F;
 }
 
-template <class S, bool ADAPTIVE, bool INV>
+// TIME: also integrate the coordinate time and the affine parameter and
+// write each sample's end t, phi and lambda to times (same indexing as the
+// ray records).
+template <class S, bool ADAPTIVE, bool INV, bool TIME>
 __device__ __forceinline__ void
 kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int spp,
           float* __restrict__ out_base, const int accumulate,
           float* __restrict__ inv_max3, np_trace_counters* __restrict__ counters,
           np_ray_record* __restrict__ records_base, unsigned long long* __restrict__ work,
-          const int chunk) {
+          const int chunk, np_ray_times* __restrict__ times) {
     const int wave_global = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     int fid = __builtin_amdgcn_readfirstlane(wave_global % nframes);
     typename S::Ray R;
@@ -1553,9 +1643,11 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
             if (has) {
                 int rhs_count = 0;
                 row[LS_ATT_RAY] += 1;
-                const int st = S::template attempt<ADAPTIVE>(F, R, rhs_count, rhs64, lazy_att);
+                const int st = S::template attempt<ADAPTIVE, TIME>(F, R, rhs_count, rhs64, lazy_att);
                 c_rhs += (uint32_t)(rhs_count + rhs64);
                 int cls = -1;
+                if (st == 1) S::post_accept(F, R);
+                if (TIME && st == 1) S::advance(R);
                 if (st == 2) cls = NP_RAY_CAPTURED;
                 else if (st == 1) cls = after_accept<S, INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));
                 if (cls >= 0) {
@@ -1593,6 +1685,13 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
                         rec.min_r = R.min_r;
                         rec.r_end = S::r(R);
                         records_base[B->rec_off + pix * (uint64_t)spp + (uint64_t)sample] = rec;
+                    }
+                    if (TIME) {
+                        np_ray_times tm;
+                        tm.t = S::t(R);
+                        tm.phi = S::phi(R);
+                        tm.lambda = S::lam(R);
+                        times[B->rec_off + pix * (uint64_t)spp + (uint64_t)sample] = tm;
                     }
                     if ((int)sample + 1 < spp) {
                         row[LS_SAMPLE] = sample + 1;
@@ -1674,8 +1773,8 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
     const BatchFrame* __restrict__ frames, const int nframes, const int spp,                    \
         float* __restrict__ out, const int accumulate, float* __restrict__ inv_max3,            \
         np_trace_counters* __restrict__ counters, np_ray_record* __restrict__ records,          \
-        unsigned long long* __restrict__ work, const int chunk
-#define NP_KERNEL_PASS frames, nframes, spp, out, accumulate, inv_max3, counters, records, work, chunk
+        unsigned long long* __restrict__ work, const int chunk, np_ray_times* __restrict__ times
+#define NP_KERNEL_PASS frames, nframes, spp, out, accumulate, inv_max3, counters, records, work, chunk, times
 
 // Kernel entry points.  Variant 0 leaves the register budget to the
 // compiler; variant W asks for W waves per SIMD, which caps the kernel at
@@ -1684,13 +1783,13 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
 // W = 2 (256, no AGPRs).
 template <class S, bool ADAPTIVE, bool INV>
 __global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
-    kerr_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
+    kerr_body<S, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
 }
 
-template <class S, bool ADAPTIVE, bool INV, int W>
+template <class S, bool ADAPTIVE, bool INV, int W, bool TIME = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 kerr_render_capped(NP_KERNEL_ARGS) {
-    kerr_body<S, ADAPTIVE, INV>(NP_KERNEL_PASS);
+    kerr_body<S, ADAPTIVE, INV, TIME>(NP_KERNEL_PASS);
 }
 
 #undef NP_KERNEL_ARGS
@@ -1723,8 +1822,24 @@ const void* kernel_ptr_variant(int precision, bool inv, int variant) {
     return kernel_ptr<StepF64, A>(inv);
 }
 
-// variant (FrameConst::variant): waves-per-SIMD cap, 0 = none.
-const void* select_kernel(int precision, bool adaptive, bool inv, int variant) {
+// The kernels that also integrate t and lambda (TIME): each precision's
+// default variant, without the invariant monitor.
+template <bool A>
+const void* kernel_ptr_time(int precision) {
+    if (precision == NP_PREC_FP32C)
+        return reinterpret_cast<const void*>(&kerr_render_capped<StepF32C, A, false, 4, true>);
+    if (precision == NP_PREC_FP64_CARTER)
+        return reinterpret_cast<const void*>(&kerr_render_capped<StepC64, A, false, 3, true>);
+    return reinterpret_cast<const void*>(&kerr_render_capped<StepF64, A, false, 2, true>);
+}
+
+// variant (FrameConst::variant): waves-per-SIMD cap, 0 = none.  time:
+// the TIME kernel of the precision (nullptr with the invariant monitor).
+const void* select_kernel(int precision, bool adaptive, bool inv, int variant, bool time) {
+    if (time) {
+        if (inv) return nullptr;
+        return adaptive ? kernel_ptr_time<true>(precision) : kernel_ptr_time<false>(precision);
+    }
     return adaptive ? kernel_ptr_variant<true>(precision, inv, variant)
                     : kernel_ptr_variant<false>(precision, inv, variant);
 }
@@ -1767,19 +1882,24 @@ size_t lane_lds_bytes(bool inv, int block) {
 int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
                 int variant, int spp, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
-                unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk) {
+                unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk,
+                np_ray_times* d_times) {
     if (block <= 0 || block > 256 || block % 64 != 0) return NP_EINVAL;
-    const void* fn = select_kernel(precision, adaptive, inv, variant);
+    const void* fn = select_kernel(precision, adaptive, inv, variant, d_times != nullptr);
+    if (!fn) return NP_EINVAL;
     void* args[] = {&d_frames, &nframes, &spp, &d_out, &accumulate, &d_inv, &d_counters, &d_records,
-                    &d_work, &chunk};
+                    &d_work, &chunk, &d_times};
     const hipError_t e =
         hipLaunchKernel(fn, dim3(grid), dim3(block), args, lane_lds_bytes(inv, block), stream);
     return (e == hipSuccess && hipGetLastError() == hipSuccess) ? NP_OK : NP_ELAUNCH;
 }
 
-int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu) {
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, select_kernel(precision, adaptive, inv, variant), block, lane_lds_bytes(inv, block));
+int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu,
+                   bool time) {
+    const void* fn = select_kernel(precision, adaptive, inv, variant, time);
+    if (!fn) return NP_EINVAL;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block,
+                                                                      lane_lds_bytes(inv, block));
     return e == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

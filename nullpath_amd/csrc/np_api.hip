@@ -98,7 +98,7 @@ int persistent_grid(DeviceState* d, int per_cu, int block, uint64_t items) {
 
 int render_launch(const np_render_params* p, int y0, int rows, int spp, int sample_offset,
                   float* d_rgb, int accumulate, float* d_inv, np_trace_counters* d_counters,
-                  np_ray_record* d_records, hipStream_t stream) {
+                  np_ray_record* d_records, hipStream_t stream, np_ray_times* d_times = nullptr) {
     if (!p || !d_rgb || spp < 0 || rows < 0 || y0 < 0 || sample_offset < 0) return NP_EINVAL;
     if (p->precision < NP_PREC_FP64 || p->precision > NP_PREC_FP64_CARTER) return NP_EINVAL;
     FrameConst F;
@@ -125,10 +125,11 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     const bool inv = p->check_inv != 0;
     const int block = 256;
     int per_cu = 0;
-    if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu) != NP_OK) per_cu = 1;
+    if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu, d_times != nullptr) != NP_OK)
+        per_cu = 1;
     const int grid = persistent_grid(d, per_cu, block, npix);
     return launch_kerr(d_rec, 1, p->precision, adaptive, inv, F.variant, spp, d_rgb, accumulate, d_inv,
-                       d_counters, d_records, slot, stream, grid, block, 1);
+                       d_counters, d_records, slot, stream, grid, block, 1, d_times);
 }
 
 __global__ void zero_u64_n(unsigned long long* p, int n) {
@@ -565,6 +566,14 @@ int np_render_tile(const np_render_params* p, int y0, int rows, int spp, int sam
                    np_ray_record* d_records, void* stream) {
     return np::render_launch(p, y0, rows, spp, sample_offset, d_rgb, 0, d_inv_max3, d_counters,
                              d_records, (hipStream_t)stream);
+}
+
+int np_render_tile_times(const np_render_params* p, int y0, int rows, int spp, int sample_offset,
+                         float* d_rgb, np_trace_counters* d_counters, np_ray_record* d_records,
+                         np_ray_times* d_times, void* stream) {
+    if (!p || !d_times || p->check_inv) return NP_EINVAL;
+    return np::render_launch(p, y0, rows, spp, sample_offset, d_rgb, 0, nullptr, d_counters, d_records,
+                             (hipStream_t)stream, d_times);
 }
 
 int np_render_pass(const np_render_params* p, int sample_offset, int spp, float* d_accum,

@@ -124,6 +124,8 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->r_isl = p->fp64_radius > 0.0 ? p->fp64_radius : std::fmax(30.0 * M, 1.5 * p->disk_rout);
     if (p->estimate_every_step != 0 && p->estimate_every_step != 1) return NP_EINVAL;
     f->lazy = p->estimate_every_step == 0 ? 1 : 0;
+    if (p->renorm_pr != 0 && p->renorm_pr != 1) return NP_EINVAL;
+    f->renorm = p->precision == NP_PREC_FP32C ? p->renorm_pr : 0;
     f->r_lazyf = (float)(100.0 * M);
     f->s_eq = std::sin(3.141592653589793238462643383279502884 * 0.5);
     f->c_eq = std::cos(3.141592653589793238462643383279502884 * 0.5);

@@ -45,6 +45,7 @@ struct FrameConst {
     // their error estimate
     float r_lazyf;
     int32_t lazy;
+    int32_t renorm;       // FP32C: p_r from H = 0 after accepted steps
 };
 
 // Returns 0 on success, NP_EINVAL when the parameters are outside what the

@@ -24,6 +24,7 @@
 //   --waves-per-simd N        (kernel variant: waves per SIMD, 0 = default)
 //   --crossing linear|hermite (disk-crossing radius; default linear as the reference)
 //   --estimate-every-step     (fp32c: no far-field skip of the error estimate)
+//   --renorm-pr               (fp32c: p_r from H = 0 after accepted steps)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,6 +82,7 @@ struct Settings {
         unsigned seed = 0;
         int max_steps = 10000, waves_per_simd = 0;
         bool estimate_every_step = false;
+        bool renorm_pr = false;
     } ext;
 };
 
@@ -147,6 +149,7 @@ std::vector<Flag> flag_table() {
     val({"--fp64-radius"}, [](S& s, const char* v) { s.ext.fp64_radius_m = as_double(v); });
     val({"--waves-per-simd"}, [](S& s, const char* v) { s.ext.waves_per_simd = as_int(v); });
     sw({"--estimate-every-step"}, [](S& s, const char*) { s.ext.estimate_every_step = true; });
+    sw({"--renorm-pr"}, [](S& s, const char*) { s.ext.renorm_pr = true; });
     return t;
 }
 
@@ -280,7 +283,7 @@ bool save_floats(const std::string& path, const float* v, size_t n) {
 uint64_t signature(const np_render_params& p) {
     uint64_t h = 1469598103934665603ull;
     const unsigned char* b = reinterpret_cast<const unsigned char*>(&p);
-    const size_t n = offsetof(np_render_params, estimate_every_step) + sizeof(p.estimate_every_step);
+    const size_t n = offsetof(np_render_params, renorm_pr) + sizeof(p.renorm_pr);
     for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
     return h;
 }
@@ -465,6 +468,7 @@ np_render_params to_params(Settings& s) {
     p.waves_per_simd = s.ext.waves_per_simd;
     p.crossing = folded(s.ext.crossing) == "hermite" ? NP_CROSSING_HERMITE : NP_CROSSING_LINEAR;
     p.estimate_every_step = s.ext.estimate_every_step ? 1 : 0;
+    p.renorm_pr = s.ext.renorm_pr ? 1 : 0;
     if (!s.ext.disk_rin.empty())
         p.disk_rin = folded(s.ext.disk_rin) == "isco" ? np_isco_radius((double)s.camera.spin) * M
                                                        : std::strtod(s.ext.disk_rin.c_str(), nullptr) * M;

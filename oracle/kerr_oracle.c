@@ -501,6 +501,7 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
     float inv[3] = {0, 0, 0};
     int step;
     double h_acc = 0.0;   /* size of the last accepted step */
+    double lam = 0.0;     /* affine parameter: sum of the accepted steps */
     for (step = 0; step < p->max_steps; ++step) {
         double scale = fmin(fmax(1.0, state[R_] / (0.5 * (double)Rs)), 100.0);
         int ok = 0;
@@ -529,6 +530,7 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
         }
         if (!ok) { captured = 1; break; }
         ++steps;
+        lam += h_acc;
         min_r = fmin(min_r, state[R_]);
         double d_prev = prev[TH_] - BH_PI_D * 0.5;
         double d_curr = state[TH_] - BH_PI_D * 0.5;
@@ -606,6 +608,11 @@ void orc_render_ray(const np_render_params* p, int x, int y, uint32_t sample_id,
     info->attempts = attempts;
     info->rhs_evals = rhs;
     info->min_r = min_r;
+    info->t_end = state[T_];
+    info->phi_end = state[PH_];
+    info->lam_end = lam;
+    info->r_end = state[R_];
+    info->th_end = state[TH_];
     memcpy(info->inv, inv, sizeof(inv));
     memcpy(info->rgb, color, sizeof(color));
 }

@@ -67,6 +67,11 @@ typedef struct orc_ray_info {
     int32_t ncross, pad_; /* equatorial crossings detected (in the disk or not), at most 4 kept */
     double  xr[4];        /* their radii, as the crossing rule computes them */
     double  p0[4];        /* covariant p_t, p_r, p_theta, p_phi at the camera */
+    double  t_end;        /* coordinate time t of the last accepted state (t = 0 at the camera) */
+    double  phi_end;      /* its azimuth phi */
+    double  lam_end;      /* affine parameter: the sum of the accepted step sizes */
+    double  r_end;        /* r of the last accepted state */
+    double  th_end;       /* its theta */
 } orc_ray_info;
 
 /* Camera-space unit direction of sample `sample_id` of pixel (x, y)

@@ -31,7 +31,7 @@ class RenderParams(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("disk_rin", C.c_double),
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
         ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
-        ("crossing", C.c_int32), ("estimate_every_step", C.c_int32),
+        ("crossing", C.c_int32), ("estimate_every_step", C.c_int32), ("renorm_pr", C.c_int32),
     ]
 
 
@@ -63,6 +63,8 @@ class RayInfo(C.Structure):
         ("attempts", C.c_int64), ("rhs_evals", C.c_int64), ("min_r", C.c_double),
         ("inv", C.c_float * 3), ("rgb", C.c_float * 3),
         ("ncross", C.c_int32), ("pad_", C.c_int32), ("xr", C.c_double * 4), ("p0", C.c_double * 4),
+        ("t_end", C.c_double), ("phi_end", C.c_double), ("lam_end", C.c_double), ("r_end", C.c_double),
+        ("th_end", C.c_double),
     ]
 
 
@@ -70,7 +72,8 @@ RAYINFO_DTYPE = np.dtype([
     ("cls", np.int32), ("orders", np.int32), ("steps", np.int64), ("attempts", np.int64),
     ("rhs_evals", np.int64), ("min_r", np.float64), ("inv", np.float32, 3),
     ("rgb", np.float32, 3), ("ncross", np.int32), ("pad_", np.int32), ("xr", np.float64, 4),
-    ("p0", np.float64, 4),
+    ("p0", np.float64, 4), ("t_end", np.float64), ("phi_end", np.float64), ("lam_end", np.float64),
+    ("r_end", np.float64), ("th_end", np.float64),
 ], align=True)
 
 

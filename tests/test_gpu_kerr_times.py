@@ -1,0 +1,192 @@
+"""GPU parity of the end coordinate time t, azimuth phi and affine parameter
+lambda (np_render_tile_times) against the CPU oracle.
+
+The oracle integrates t with the state (bh_kerr.h:734-774, 987-1032) and
+sums the accepted steps into lambda; its end values are pinned by quadrature
+at a = 0 (tests/test_oracle_kerr_times.py).
+
+* Fixed steps (h = 0.01 min(r / Rs, 100), a function of the state only):
+  every path runs the oracle's step sequence, so t, phi and lambda are
+  compared directly -- for FP32C this checks the float-float two-sum
+  accumulation itself.
+* Adaptive steps: the accepted step sizes come from error estimates that
+  differ in the last bits (FP64: FMA contraction and device libm) or in
+  float rounding (FP32C), so a ray can end at a slightly different lambda
+  even with the oracle's step count.  t and phi are compared after removing
+  the first-order effect of that difference, (lambda_gpu - lambda_oracle)
+  times dt/dlambda and dphi/dlambda at the oracle's end state.
+
+Only escaped rays whose step count equals the oracle's and whose closest
+approach stays beyond 5 Schwarzschild radii enter the bounds: near the
+photon orbit the geodesic flow amplifies any rounding difference by orders
+of magnitude, and t grows without bound towards the horizon.
+"""
+import numpy as np
+import pytest
+import torch
+
+import nullpath_amd as npa
+from nullpath_amd._lib import NullPathError, ray_record_dtype, ray_times_dtype
+
+
+def _run(p, spp, times=True):
+    n = p.width * p.height
+    out = torch.zeros(n * 3, dtype=torch.float32, device="cuda")
+    cnt = npa.new_counters()
+    rec = torch.zeros(n * spp * 32, dtype=torch.uint8, device="cuda")
+    print(f"[gpu] np_render_tile{'_times' if times else ''} {p.width}x{p.height} spp={spp} "
+          f"precision={p.precision} integrator={p.integrator} spin={p.spin}", flush=True)
+    if times:
+        tm = torch.zeros(n * spp * 24, dtype=torch.uint8, device="cuda")
+        npa.render_tile_times(p, out, tm, spp=spp, counters=cnt, records=rec)
+    else:
+        npa.render_tile(p, out, spp=spp, counters=cnt, records=rec)
+    torch.cuda.synchronize()
+    recs = rec.cpu().numpy().view(ray_record_dtype())
+    tms = tm.cpu().numpy().view(ray_times_dtype()) if times else None
+    return out.cpu().numpy().reshape(-1, 3), recs, tms
+
+
+def _oracle(oracle, p, spp):
+    q = oracle.default_params()
+    for name, _ in npa.RenderParams._fields_:
+        setattr(q, name, getattr(p, name))
+    ys, xs = np.mgrid[0:p.height, 0:p.width]
+    return oracle.render_pixels(q, xs.ravel(), ys.ravel(), 0, spp)
+
+
+def _params(precision, integrator=1, **kw):
+    base = dict(width=24, height=16, spin=0.9, theta_cam=npa.deg2rad_f32(80.0), fov_y_deg=20.0,
+                integrator=integrator, disk_max_orders=3, rtol=1e-8, precision=precision,
+                max_steps=200000)
+    base.update(kw)
+    return npa.default_params(**base)
+
+
+def _rates(p, info):
+    """dt/dlambda and dphi/dlambda (g^{tt} p_t + g^{t phi} p_phi, g^{t phi} p_t +
+    g^{phi phi} p_phi) at the oracle's end states, in float64 numpy."""
+    M = float(np.float32(p.mass))
+    a = float(np.float32(p.spin) * np.float32(p.mass))
+    r, th = info["r_end"], info["th_end"]
+    E, L = -info["p0"][:, 0], info["p0"][:, 3]
+    s2 = np.sin(th) ** 2
+    Sig = r * r + a * a * np.cos(th) ** 2
+    Del = r * r - 2 * M * r + a * a
+    A = (r * r + a * a) ** 2 - a * a * Del * s2
+    tdot = (E * A - 2 * M * a * r * L) / (Sig * Del)
+    pdot = (2 * M * a * r * E + (Del - a * a * s2) * L / s2) / (Sig * Del)
+    return tdot, pdot
+
+
+def _compare(p, g_rec, g_tm, o_info, label, correct):
+    same = (g_rec["steps"] == o_info["steps"]) & (g_rec["cls"] == o_info["cls"])
+    far = same & (o_info["cls"] == 0) & (o_info["min_r"] > 5.0 * 2.0 * p.mass)
+    dl = g_tm["lam"] - o_info["lam_end"]
+    dt = g_tm["t"] - o_info["t_end"]
+    dp = g_tm["phi"] - o_info["phi_end"]
+    if correct:
+        tdot, pdot = _rates(p, o_info)
+        dt = dt - dl * tdot
+        dp = dp - dl * pdot
+    t_err = np.abs(dt) / np.abs(o_info["t_end"])
+    l_err = np.abs(dl) / o_info["lam_end"]
+    f_err = np.abs(dp)
+    print(f"[{label}] {same.sum()} rays with the oracle's step count, {far.sum()} escaped beyond 5 Rs: "
+          f"max rel err t {t_err[far].max():.3g} (median {np.median(t_err[far]):.3g}), lambda "
+          f"{l_err[far].max():.3g}, abs err phi {f_err[far].max():.3g}", flush=True)
+    assert same.mean() > 0.9 and far.sum() > 100
+    return far, t_err, l_err, f_err
+
+
+# measured on MI355X (rel t, rel lambda, abs phi), with about 10x margin:
+#   fp64 fixed 9e-15 / 1e-14 / 2e-14; fp32c 6e-8 / 8e-8 / 3e-8; the FP64
+#   comparator 8e-8 / 8e-8 / 3e-8.  The Carter-form field equals Hamilton's
+#   equations only on H = 0, and the reference's camera launches rays with
+#   |H| ~ 1e-8 E^2, so both Carter paths follow a trajectory ~1e-8 away from
+#   the oracle's, and the fixed step h ~ r inherits that difference.
+FIXED_TOL = {0: (1e-13, 1e-13, 1e-12), 1: (5e-7, 5e-7, 3e-7), 2: (5e-7, 5e-7, 3e-7)}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_fixed_step_end_times_match_oracle(oracle, device, precision):
+    """The oracle's step sequence: t, phi, lambda compared directly."""
+    p = _params(precision, 0)
+    g_rgb, g_rec, g_tm = _run(p, 1)
+    o_rgb, o_info = _oracle(oracle, p, 1)
+    far, t_err, l_err, f_err = _compare(p, g_rec, g_tm, o_info, f"fixed precision={precision}", False)
+    tt, tl, tf = FIXED_TOL[precision]
+    assert t_err[far].max() < tt and l_err[far].max() < tl and f_err[far].max() < tf
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_adaptive_end_times_match_oracle(oracle, device, precision):
+    p = _params(precision, 1)
+    g_rgb, g_rec, g_tm = _run(p, 1)
+    o_rgb, o_info = _oracle(oracle, p, 1)
+    far, t_err, l_err, f_err = _compare(p, g_rec, g_tm, o_info, f"adaptive precision={precision}", True)
+    tol = 1e-9 if precision == 0 else 1e-7
+    assert t_err[far].max() < tol and f_err[far].max() < tol * 10
+    # the kernel that also integrates t renders what np_render_tile renders
+    b_rgb, b_rec, _ = _run(p, 1, times=False)
+    assert np.array_equal(b_rec["cls"], g_rec["cls"])
+    assert np.mean(b_rec["steps"] == g_rec["steps"]) > 0.99
+
+
+@pytest.mark.gpu
+def test_times_entry_point_rejects_bad_arguments(device):
+    p = _params(1, check_inv=1)
+    out = torch.zeros(p.width * p.height * 3, dtype=torch.float32, device="cuda")
+    tm = torch.zeros(p.width * p.height * 24, dtype=torch.uint8, device="cuda")
+    with pytest.raises(NullPathError):
+        npa.render_tile_times(p, out, tm)
+    code = npa.lib().np_render_tile_times(_params(1), 0, 4, 1, 0, out.data_ptr(), None, None, None, None)
+    assert code == -1
+
+
+def _end_h_max(p):
+    """The invariant monitor's |H| maximum; with the disk moved out of reach
+    no crossing is counted, so it is the maximum over the escaped rays' end
+    states (render.cu:327-335)."""
+    n = p.width * p.height
+    out = torch.zeros(n * 3, dtype=torch.float32, device="cuda")
+    inv = torch.zeros(3, dtype=torch.float32, device="cuda")
+    print(f"[gpu] np_render_tile check_inv renorm_pr={p.renorm_pr}", flush=True)
+    npa.render_tile(p, out, spp=1, inv=inv)
+    torch.cuda.synchronize()
+    return float(inv.cpu().numpy()[0])
+
+
+@pytest.mark.gpu
+def test_fp32c_pr_renormalisation(oracle, device):
+    """renorm_pr = 1 (FP32C): p_r set from H = 0 after every accepted step
+    where radial motion dominates.  The escaped rays' end |H| drops, and the
+    image meets the oracle contract of the headline tests (class, crossing
+    count and 1e-5 radiance outside the band) on the 1920x1080 workload
+    zoomed onto the disk."""
+    from test_gpu_oracle_parity import check_against_oracle
+    def far_disk(renorm):
+        q = npa.kerr_disk_params(width=48, height=27, fov=30.0, seed=1, precision=1, check_inv=1,
+                                 renorm_pr=renorm)
+        q.disk_rin, q.disk_rout = 1e6, 2e6
+        return q
+
+    h_plain = _end_h_max(far_disk(0))
+    h_renorm = _end_h_max(far_disk(1))
+    print(f"end |H| max: plain {h_plain:.3e}, renormalised {h_renorm:.3e}", flush=True)
+    assert h_renorm < 0.1 * h_plain
+    kw = dict(width=96, height=54, fov=1.6, seed=1, crossing=1, precision=1, renorm_pr=1)
+    n = 96 * 54
+    out = torch.zeros(n * 3, dtype=torch.float32, device="cuda")
+    rec = torch.zeros(n * 32, dtype=torch.uint8, device="cuda")
+    p = npa.kerr_disk_params(**kw)
+    print("[gpu] np_render_tile renorm_pr=1 disk zoom", flush=True)
+    npa.render_tile(p, out, spp=1, records=rec)
+    torch.cuda.synchronize()
+    o_rgb, o_info = _oracle(oracle, p, 1)
+    disk, _ = check_against_oracle(p, 1, out.cpu().numpy().reshape(-1, 3),
+                                   rec.cpu().numpy().view(ray_record_dtype()), o_rgb, o_info,
+                                   "fp32c renorm_pr")
+    assert disk > 100
