@@ -137,6 +137,14 @@ def parse():
     ap.add_argument("--fp64-waves-per-simd", type=int, default=0, choices=[0, 1, 2],
                     help="reference-formula FP64 kernel variant (0 = the library default, 2 waves per "
                          "SIMD; 1 = the compiler's register budget)")
+    ap.add_argument("--times", type=int, default=0,
+                    help="1 = the render legs also integrate t and the affine parameter "
+                         "(np_render_tile_times) -- a cost measurement, not the headline")
+    ap.add_argument("--renorm-pr", type=int, default=0, choices=[0, 1],
+                    help="1 = FP32C renormalises p_r from H = 0 after accepted steps")
+    ap.add_argument("--full-contract", type=int, default=1,
+                    help="1 = also time the FP32C frame with t and lambda integrated and p_r "
+                         "renormalised (one step), beside the headline")
     ap.add_argument("--linear-check", default="960x540x1",
                     help="WxHxSPP of the linear-crossing comparison; 'none' skips it")
     ap.add_argument("--adjoint", type=int, default=1,
@@ -175,7 +183,7 @@ def copy_calibration(mib: int) -> int:
 
 
 def render_case(prec: str, crossing: str, width: int, height: int, spp: int, steps: int,
-                warmup: int, waves_per_simd: int = 0) -> dict:
+                warmup: int, waves_per_simd: int = 0, times: bool = False, renorm_pr: int = 0) -> dict:
     import torch
 
     import nullpath_amd as npa
@@ -188,14 +196,22 @@ def render_case(prec: str, crossing: str, width: int, height: int, spp: int, ste
     rays = npix * spp
     p = npa.kerr_disk_params(width=width, height=height, spin=0.9, theta_deg=80.0, fov=55.0,
                              rtol=1e-8, max_orders=3, q=3.0, seed=1, precision=pid, crossing=cr,
-                             waves_per_simd=waves_per_simd)
+                             waves_per_simd=waves_per_simd, renorm_pr=renorm_pr)
     out = torch.zeros(npix * 3, dtype=torch.float32, device=dev)
     cnt = npa.new_counters(dev)
     rec = torch.zeros(rays * 32, dtype=torch.uint8, device=dev)
+    tm = torch.zeros(rays * 24 if times else 1, dtype=torch.uint8, device=dev)
+
+    def launch(**kw):
+        if times:
+            npa.render_tile_times(p, out, tm, spp=spp, **kw)
+        else:
+            npa.render_tile(p, out, spp=spp, **kw)
+
     npa.render_tile(npa.kerr_disk_params(width=64, height=36, seed=1, precision=pid, crossing=cr),
                     out, spp=1)
     for _ in range(warmup):
-        npa.render_tile(p, out, spp=spp)
+        launch()
     torch.cuda.synchronize()
     # events on the stream the kernels are launched on (torch's current stream)
     stream = torch.cuda.current_stream()
@@ -205,7 +221,7 @@ def render_case(prec: str, crossing: str, width: int, height: int, spp: int, ste
     for k in range(steps):
         cnt.zero_()
         ev[k][0].record(stream)
-        npa.render_tile(p, out, spp=spp, counters=cnt, records=rec, stream=stream)
+        launch(counters=cnt, records=rec, stream=stream)
         ev[k][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -844,7 +860,8 @@ def main() -> int:
         k, w = (args.steps, args.warmup) if prec == "fp32c" or len(precs) == 1 else (args.fp64_steps, 0)
         waves_p = {"fp32c": args.waves_per_simd, "fp64c": args.fp64c_waves_per_simd,
                    "fp64": args.fp64_waves_per_simd}[prec]
-        res[prec] = render_case(prec, args.crossing, args.width, args.height, args.spp, k, w, waves_p)
+        res[prec] = render_case(prec, args.crossing, args.width, args.height, args.spp, k, w, waves_p,
+                                bool(args.times), args.renorm_pr)
     head = "fp32c" if "fp32c" in res else precs[0]
     R = res[head]
     value = R["rays"] / R["wall"]
@@ -877,6 +894,8 @@ def main() -> int:
                          f"<=3 crossings, {args.crossing} crossing radius"),
             "precision": head,
             "rays_per_step": R["rays"] // args.steps,
+            "integrates_t_and_lambda": bool(args.times),
+            "renorm_pr": args.renorm_pr,
         },
         "roofline": {
             "bound": "valu",
@@ -913,6 +932,20 @@ def main() -> int:
             lin["size"] = args.linear_check
             lin["speedup"] = (lb["rays"] / lb["wall"]) / (la["rays"] / la["wall"])
             line["config"]["fp32c_vs_fp64_linear_crossing"] = lin
+    if args.full_contract and "fp32c" in res and not args.times and not args.renorm_pr:
+        # the same frame with every item of the FP32C precision contract on:
+        # t and lambda integrated (np_render_tile_times) and p_r renormalised
+        # from H = 0; its image against the FP64 path's
+        fc = render_case("fp32c", args.crossing, args.width, args.height, args.spp, 1, 0,
+                         args.waves_per_simd, True, 1)
+        full = {"rays_per_s": fc["rays"] / fc["wall"], "kernel_ms": fc["kern_ms"],
+                "time_vs_headline": fc["mean_s"] / res["fp32c"]["mean_s"]}
+        if "fp64" in res:
+            full["speedup_over_fp64"] = full["rays_per_s"] / (res["fp64"]["rays"] / res["fp64"]["wall"])
+            full["vs_fp64"] = compare(res["fp64"], fc)
+        if "fp64c" in res:
+            full["speedup_over_fp64c"] = full["rays_per_s"] / (res["fp64c"]["rays"] / res["fp64c"]["wall"])
+        line["config"]["fp32c_full_contract"] = full
 
     if args.adjoint:
         line["config"]["adjoint"] = bench_adjoint(args)

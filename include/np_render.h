@@ -94,8 +94,8 @@ typedef struct np_render_params {
                                  pole stays clear of the step; there the truncation error of a
                                  step is ~1e-5 of the tolerance, far from the 0.59 that would
                                  shrink the next step; 1 = every step estimates its error */
-    int32_t renorm_pr;        /* NP_PREC_FP32C: 1 = after every accepted step, set p_r from the
-                                 constraint H = 0 (Delta^2 p_r^2 = P^2 - Delta (p_theta^2 +
+    int32_t renorm_pr;        /* NP_PREC_FP32C: 1 = after every 16th accepted step, set p_r from
+                                 the constraint H = 0 (Delta^2 p_r^2 = P^2 - Delta (p_theta^2 +
                                  (L/sin - a E sin)^2), in FP64 on the float-float state) where
                                  radial motion dominates (Delta p_r^2 >= P^2 / (4 Delta)), so the
                                  square root is well conditioned; 0 = off (the reference never

@@ -625,7 +625,8 @@ struct StepF64 {
     __device__ static double lam(const Ray& R) { return R.lam; }
     // an accepted step of size R.h advances the affine parameter
     __device__ static void advance(Ray& R) { R.lam += R.h; }
-    __device__ static void post_accept(const FrameConst&, Ray&) {}
+    static constexpr unsigned kRenormEvery = 16;
+    __device__ static void post_accept(const FrameConst&, Ray&, bool) {}
 
     __device__ static double r(const Ray& R) { return R.y.r; }
     __device__ static double th(const Ray& R) { return R.y.th; }
@@ -1146,15 +1147,19 @@ struct StepF32C {
     // an accepted step of size R.h advances the affine parameter (two-sum)
     __device__ static void advance(Ray& R) { R.lam = ff_add(R.lam, R.h); }
 
-    // With F.renorm, p_r is set from the constraint H = 0 after every
-    // accepted step: in the Carter form 2 Sigma H = Delta p_r^2 - P^2/Delta +
-    // p_theta^2 + (L/sin - a E sin)^2, so Delta^2 p_r^2 = P^2 - Delta K with
-    // K = p_theta^2 + (L/sin - a E sin)^2.  Only where radial motion
-    // dominates (4 Delta^2 p_r^2 >= P^2): near a radial turning point the
-    // square root would amplify the rounding of K.  FP64 on the float-float
-    // state; the sign of p_r is kept.
-    __device__ static void post_accept(const FrameConst& F, Ray& R) {
-        if (!F.renorm) return;
+    // With F.renorm, p_r is set from the constraint H = 0 after the steps a
+    // wave accepts on every kRenormEvery-th attempt iteration (`tick`, the
+    // same for all lanes, so the FP64 block runs for the whole wave at once
+    // instead of for some lane on nearly every iteration; the drift between
+    // two renormalisations is ~1e-10 E^2): in the Carter form 2 Sigma H =
+    // Delta p_r^2 - P^2/Delta + p_theta^2 + (L/sin - a E sin)^2, so Delta^2
+    // p_r^2 = P^2 - Delta K with K = p_theta^2 + (L/sin - a E sin)^2.  Only
+    // where radial motion dominates (4 Delta^2 p_r^2 >= P^2): near a radial
+    // turning point the square root would amplify the rounding of K.  FP64 on
+    // the float-float state; the sign of p_r is kept.
+    static constexpr unsigned kRenormEvery = 16;
+    __device__ static void post_accept(const FrameConst& F, Ray& R, bool tick) {
+        if (!F.renorm || !tick) return;
         const double r = ff_to_d(R.y.r), th = ff_to_d(R.y.th), pr = ff_to_d(R.y.pr), pth = ff_to_d(R.y.pth);
         const double E = -pt(R), L = pph(R), a = F.a, a2 = a * a;
         double s, c;
@@ -1419,7 +1424,8 @@ struct StepC64 {
     __device__ static double phi(const Ray& R) { return R.y.ph; }
     __device__ static double lam(const Ray& R) { return R.lam; }
     __device__ static void advance(Ray& R) { R.lam += R.h; }
-    __device__ static void post_accept(const FrameConst&, Ray&) {}
+    static constexpr unsigned kRenormEvery = 16;
+    __device__ static void post_accept(const FrameConst&, Ray&, bool) {}
 
     template <bool INV>
     __device__ static void start(const FrameConst& F, Ray& R, float n_r, float n_th, float n_ph,
@@ -1588,6 +1594,7 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
     // (scalar registers, from ballots).
     uint32_t c_rhs = 0;
     uint64_t w_att64 = 0, w_lazy = 0;
+    uint32_t iter = 0;   // attempt iterations of this wave (scalar)
     if (spp <= 0) done = true;
 
     // Outer loop: one frame at a time; inner loop: one integrator attempt per
@@ -1640,13 +1647,14 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
             }
             int rhs64 = 0, lazy_att = 0;
             bool ended = false;
+            ++iter;
             if (has) {
                 int rhs_count = 0;
                 row[LS_ATT_RAY] += 1;
                 const int st = S::template attempt<ADAPTIVE, TIME>(F, R, rhs_count, rhs64, lazy_att);
                 c_rhs += (uint32_t)(rhs_count + rhs64);
                 int cls = -1;
-                if (st == 1) S::post_accept(F, R);
+                if (st == 1) S::post_accept(F, R, (iter % S::kRenormEvery) == 0);
                 if (TIME && st == 1) S::advance(R);
                 if (st == 2) cls = NP_RAY_CAPTURED;
                 else if (st == 1) cls = after_accept<S, INV>(F, R, S::r(R), S::th(R), S::pr(R), S::pth(R));

@@ -161,7 +161,7 @@ def _end_h_max(p):
 
 @pytest.mark.gpu
 def test_fp32c_pr_renormalisation(oracle, device):
-    """renorm_pr = 1 (FP32C): p_r set from H = 0 after every accepted step
+    """renorm_pr = 1 (FP32C): p_r set from H = 0 after every 16th accepted step
     where radial motion dominates.  The escaped rays' end |H| drops, and the
     image meets the oracle contract of the headline tests (class, crossing
     count and 1e-5 radiance outside the band) on the 1920x1080 workload
