@@ -186,7 +186,7 @@ int np_render_batch(const np_render_params* frames, int nframes, int spp, int sa
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, kernels and results as np_render_batch; the
  * frame table and the per-frame work counters live in a fixed device array
- * of the library (about 49 KB per GPU, held until the process ends) instead
+ * of the library (about 51 KB per GPU, held until the process ends) instead
  * of a per-call allocation.  At most NP_NOALLOC_MAX_FRAMES frames
  * (NP_ECAPACITY beyond).  The caller still provides d_rgb and the optional
  * d_records / d_counters.  The call copies the frame table into that array
