@@ -1087,6 +1087,10 @@ __device__ __forceinline__ bool okf(const FrameConst& F, float r_hi, float th_hi
 // there the truncation error of a 0.2 M step is ~1e-5 of the tolerance.
 constexpr float kLazyError = 1e-3f;
 
+// Float error level above which a float attempt's rejection is final (no
+// FP64 re-run): twice the tolerance.
+constexpr float kClearReject = 2.0f;
+
 struct StepF32C {
     struct Ray : RayBase {
         FF5 y;            // y.t, the coordinate time, is integrated by TIME kernels only
@@ -1343,10 +1347,24 @@ struct StepF32C {
                 R.skip = F.lazy && far && e < kLazyError;
                 return 1;
             }
+            if (e > kClearReject) {
+                // A clear rejection: the noise floor is subtracted from every
+                // difference, so the float error level is a lower bound of
+                // the FP64 one and the step is rejected in both.  The retry
+                // runs in float with the step law of render.cu:193-205 on
+                // the float error level; no FP64 re-run is needed to decide.
+                const float le = __log2f(e);
+                R.h = fmaxf(h * fmaxf(0.1f, 0.9f * exp2f(-0.25f * le)), hmin);
+                R.attempt += 1;
+                R.skip = 0;
+                if (R.h <= hmin * 1.01f || R.attempt >= 8) return 2;
+                return 0;
+            }
         }
-        // A float attempt that leaves the domain or fails the tolerance is
-        // re-run in FP64 with the same step; that run decides acceptance,
-        // the next step size and the retry budget exactly as the FP64 path.
+        // A float attempt that leaves the domain or fails the tolerance by a
+        // margin within float rounding is re-run in FP64 with the same step;
+        // that run decides acceptance, the next step size and the retry
+        // budget exactly as the FP64 path.
         R.mode = 2;
         return 0;
     }
