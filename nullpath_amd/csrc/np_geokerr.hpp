@@ -431,17 +431,24 @@ __device__ __forceinline__ cplx csqrt_d(cplx z) {
     return mk(fabs(z.im) / (2.0 * t), copysign(t, z.im));
 }
 
-// LAGUER (geokerr_original.f:3527-3606); a[0..m], constant term first.
-__device__ inline void laguer(const cplx* a, int m, cplx& x) {
+// LAGUER (geokerr_original.f:3527-3606) for a polynomial of degree M;
+// a[0..M], constant term first.  The degree is a template parameter so every
+// coefficient index is a constant after unrolling: the coefficient and root
+// arrays of the callers stay in registers instead of private (scratch)
+// memory, with the same operations in the same order as the runtime-degree
+// loop of the Fortran.
+template <int M>
+__device__ __forceinline__ void laguer(const cplx* a, cplx& x) {
     const double frac[8] = {0.5, 0.25, 0.75, 0.13, 0.38, 0.62, 0.88, 1.0};
     const int MT = 10, MAXIT = 80;
     const double EPSS = 1e-15;
     for (int iter = 1; iter <= MAXIT; ++iter) {
-        cplx b = a[m];
+        cplx b = a[M];
         double err = cabs_d(b);
         cplx d = mk(0.0), f = mk(0.0);
         const double abx = cabs_d(x);
-        for (int j = m - 1; j >= 0; --j) {
+#pragma unroll
+        for (int j = M - 1; j >= 0; --j) {
             f = x * f + d;
             d = x * d + b;
             b = x * b + a[j];
@@ -452,14 +459,14 @@ __device__ inline void laguer(const cplx* a, int m, cplx& x) {
         const cplx g = d / b;
         const cplx g2 = g * g;
         const cplx h = g2 - 2.0 * (f / b);
-        const cplx sq = csqrt_d((double)(m - 1) * ((double)m * h - g2));
+        const cplx sq = csqrt_d((double)(M - 1) * ((double)M * h - g2));
         cplx gp = g + sq;
         const cplx gm = g - sq;
         const double abp = cabs_d(gp), abm = cabs_d(gm);
         if (abp < abm) gp = gm;
         cplx dx;
         if (fmax(abp, abm) > 0.0) {
-            dx = mk((double)m) / gp;
+            dx = mk((double)M) / gp;
         } else {
             // CEXP(CMPLX(LOG(1+ABX), ITER)) in default (single) precision
             const float lr = (float)log(1.0 + abx), th = (float)iter;
@@ -473,43 +480,73 @@ __device__ inline void laguer(const cplx* a, int m, cplx& x) {
     }
 }
 
-// ZROOTS (geokerr_original.f:3343-3441); roots sorted by real part.
-__device__ inline void zroots(const cplx* a, int m, cplx* roots) {
-    const double EPS = 1e-6;
-    cplx ad[8];
-    for (int j = 0; j <= m; ++j) ad[j] = a[j];
-    cplx xstart = mk(0.0);
-    int ncc = 0;
-    for (int j = m; j >= 1; --j) {
+// ZROOTS' deflation (geokerr_original.f:3343-3441) from degree J down to 1,
+// one level per template instance (J and the loop below are constants).
+template <int M, int J>
+__device__ __forceinline__ void zroots_deflate(cplx* ad, cplx* roots, cplx& xstart, int& ncc) {
+    if constexpr (J >= 1) {
+        const double EPS = 1e-6;
         cplx x = xstart;
-        if (j < m && roots[j].im != 0.0 && ncc == 0) {
+        if (J < M && roots[J].im != 0.0 && ncc == 0) {
             // the reference builds (Re r, -Re r), not the conjugate
-            xstart = mk(roots[j].re, -roots[j].re);
+            xstart = mk(roots[J].re, -roots[J].re);
             ncc = 1;
         } else {
             xstart = mk(0.0);
             ncc = 0;
         }
-        laguer(ad, j, x);
+        laguer<J>(ad, x);
         if (fabs(x.im) <= 2.0 * EPS * EPS * fabs(x.re)) x = mk(x.re);
-        roots[j - 1] = x;
-        cplx b = ad[j];
-        for (int jj = j - 1; jj >= 0; --jj) {
+        roots[J - 1] = x;
+        cplx b = ad[J];
+#pragma unroll
+        for (int jj = J - 1; jj >= 0; --jj) {
             const cplx c = ad[jj];
             ad[jj] = b;
             b = x * b + c;
         }
+        zroots_deflate<M, J - 1>(ad, roots, xstart, ncc);
     }
-    for (int j = 0; j < m; ++j) laguer(a, m, roots[j]);
-    for (int j = 1; j < m; ++j) {
-        const cplx x = roots[j];
-        int i;
-        for (i = j - 1; i >= 0; --i) {
-            if (roots[i].re <= x.re) break;
-            roots[i + 1] = roots[i];
+}
+
+// Straight insertion by real part (ZROOTS' PIKSRT-style final loop), with
+// the break of the scan as a predicate so every index is a constant: a root
+// moves up past every larger real part until one at or below it (a NaN
+// real part never stops the scan, as in the loop with the break).
+template <typename C, int M>
+__device__ __forceinline__ void insertion_sort_re(C* roots) {
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+        const C x = roots[j];
+        bool moving = true;
+#pragma unroll
+        for (int i = j - 1; i >= 0; --i) {
+            if (moving) {
+                if (roots[i].re <= x.re) {
+                    moving = false;
+                    roots[i + 1] = x;
+                } else {
+                    roots[i + 1] = roots[i];
+                }
+            }
         }
-        roots[i + 1] = x;
+        if (moving) roots[0] = x;
     }
+}
+
+// ZROOTS (geokerr_original.f:3343-3441) for degree M; roots sorted by real
+// part.
+template <int M>
+__device__ __forceinline__ void zroots(const cplx* a, cplx* roots) {
+    cplx ad[M + 1];
+#pragma unroll
+    for (int j = 0; j <= M; ++j) ad[j] = a[j];
+    cplx xstart = mk(0.0);
+    int ncc = 0;
+    zroots_deflate<M, M>(ad, roots, xstart, ncc);
+#pragma unroll
+    for (int j = 0; j < M; ++j) laguer<M>(a, roots[j]);
+    insertion_sort_re<cplx, M>(roots);
 }
 
 // ---------------- mixed-precision roots of the quartic U(u) ----------------
@@ -567,17 +604,20 @@ __device__ __forceinline__ cplxf csqrt_f(cplxf z) {
     return mkf(fabsf(z.im) * h, copysignf(t, z.im));
 }
 
-// LAGUER's iteration in float (same step control; float round-off bound).
-__device__ inline void laguer_f(const cplxf* a, int m, cplxf& x) {
+// LAGUER's iteration in float (same step control; float round-off bound),
+// degree M a template parameter as in laguer.
+template <int M>
+__device__ __forceinline__ void laguer_f(const cplxf* a, cplxf& x) {
     const float frac[8] = {0.5f, 0.25f, 0.75f, 0.13f, 0.38f, 0.62f, 0.88f, 1.0f};
     const int MT = 10, MAXIT = 80;
     const float EPSS = 1e-7f;
     for (int iter = 1; iter <= MAXIT; ++iter) {
-        cplxf b = a[m];
+        cplxf b = a[M];
         float err = cabs_f(b);
         cplxf d = mkf(0.0f), f = mkf(0.0f);
         const float abx = cabs_f(x);
-        for (int j = m - 1; j >= 0; --j) {
+#pragma unroll
+        for (int j = M - 1; j >= 0; --j) {
             f = x * f + d;
             d = x * d + b;
             b = x * b + a[j];
@@ -588,14 +628,14 @@ __device__ inline void laguer_f(const cplxf* a, int m, cplxf& x) {
         const cplxf g = d / b;
         const cplxf g2 = g * g;
         const cplxf h = g2 - 2.0f * (f / b);
-        const cplxf sq = csqrt_f((float)(m - 1) * ((float)m * h - g2));
+        const cplxf sq = csqrt_f((float)(M - 1) * ((float)M * h - g2));
         cplxf gp = g + sq;
         const cplxf gm = g - sq;
         const float abp = cabs_f(gp), abm = cabs_f(gm);
         if (abp < abm) gp = gm;
         cplxf dx;
         if (fmaxf(abp, abm) > 0.0f) {
-            dx = mkf((float)m) / gp;
+            dx = mkf((float)M) / gp;
         } else {
             const float e = expf(logf(1.0f + abx));
             dx = mkf(e * cosf((float)iter), e * sinf((float)iter));
@@ -607,36 +647,50 @@ __device__ inline void laguer_f(const cplxf* a, int m, cplxf& x) {
     }
 }
 
-__device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
-    cplxf ad[5];
-    for (int j = 0; j <= 4; ++j) ad[j] = mkf((float)a[j].re, (float)a[j].im);
-    cplxf xstart = mkf(0.0f);
-    int ncc = 0;
-    cplxf rf[4];
-    for (int j = 4; j >= 1; --j) {
+// The float deflation of zroots4_mixed from degree J down to 1; false when a
+// root is not usable (non-finite, or its imaginary part between 1e-5 and
+// 1e-2 of its modulus).
+template <int J>
+__device__ __forceinline__ bool deflate4_f(cplxf* ad, cplxf* rf, cplxf& xstart, int& ncc) {
+    if constexpr (J >= 1) {
         cplxf x = xstart;
-        if (j < 4 && rf[j].im != 0.0f && ncc == 0) {
-            xstart = mkf(rf[j].re, -rf[j].re);
+        if (J < 4 && rf[J].im != 0.0f && ncc == 0) {
+            xstart = mkf(rf[J].re, -rf[J].re);
             ncc = 1;
         } else {
             xstart = mkf(0.0f);
             ncc = 0;
         }
-        laguer_f(ad, j, x);
+        laguer_f<J>(ad, x);
         const float ax = cabs_f(x);
         if (!(ax > 0.0f) || !isfinite(ax)) return false;
         const float rim = fabsf(x.im) / ax;
         if (rim <= 1e-5f) x = mkf(x.re);
         else if (rim < 1e-2f) return false;
-        rf[j - 1] = x;
-        cplxf b = ad[j];
-        for (int jj = j - 1; jj >= 0; --jj) {
+        rf[J - 1] = x;
+        cplxf b = ad[J];
+#pragma unroll
+        for (int jj = J - 1; jj >= 0; --jj) {
             const cplxf c = ad[jj];
             ad[jj] = b;
             b = x * b + c;
         }
+        return deflate4_f<J - 1>(ad, rf, xstart, ncc);
     }
+    return true;
+}
+
+__device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
+    cplxf ad[5];
+#pragma unroll
+    for (int j = 0; j <= 4; ++j) ad[j] = mkf((float)a[j].re, (float)a[j].im);
+    cplxf xstart = mkf(0.0f);
+    int ncc = 0;
+    cplxf rf[4];
+    if (!deflate4_f<4>(ad, rf, xstart, ncc)) return false;
+#pragma unroll
     for (int i = 0; i < 4; ++i)
+#pragma unroll
         for (int j = i + 1; j < 4; ++j)
             if (rf[i].im == 0.0f && rf[j].im == 0.0f &&
                 fabsf(rf[i].re - rf[j].re) <= 1e-3f * fmaxf(fabsf(rf[i].re), fabsf(rf[j].re)))
@@ -644,10 +698,12 @@ __device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
     // three Newton steps in double on the full polynomial: from a float
     // root of a simple (separated) root they reach double precision; a real
     // root stays on the real axis
+#pragma unroll
     for (int j = 0; j < 4; ++j) {
         cplx x = mk((double)rf[j].re, (double)rf[j].im);
         for (int it = 0; it < 3; ++it) {
             cplx p = a[4], dp = mk(0.0);
+#pragma unroll
             for (int k = 3; k >= 0; --k) {
                 dp = x * dp + p;
                 p = x * p + a[k];
@@ -657,15 +713,7 @@ __device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
         }
         roots[j] = x;
     }
-    for (int j = 1; j < 4; ++j) {
-        const cplx x = roots[j];
-        int i;
-        for (i = j - 1; i >= 0; --i) {
-            if (roots[i].re <= x.re) break;
-            roots[i + 1] = roots[i];
-        }
-        roots[i + 1] = x;
-    }
+    insertion_sort_re<cplx, 4>(roots);
     return true;
 }
 
@@ -897,11 +945,11 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
             c[0] = mk(one); c[1] = mk(0.0); c[2] = mk(cc); c[3] = mk(dd); c[4] = mk(ee);
             if constexpr (MIXED) {
                 if (!zroots4_mixed(c, root)) {
-                    zroots(c, 4, root);
+                    zroots<4>(c, root);
                     v.fp64_roots = 1;
                 }
             } else {
-                zroots(c, 4, root);
+                zroots<4>(c, root);
             }
             int nreal = 0;
             for (int i = 0; i < 4; ++i) nreal += (root[i].im == 0.0);
@@ -938,7 +986,7 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
             co[0] = mk(one); co[1] = mk(-cc / se); co[2] = mk(-one);
             co[3] = mk(se * (two * cc / ee - (dd / ee) * (dd / ee)));
             co[4] = mk(-one); co[5] = mk(-cc / se); co[6] = mk(one);
-            zroots(co, 6, hroots);
+            zroots<6>(co, hroots);
             double h1 = 0.0;
             for (int i = 0; i < 6 && h1 == 0.0; ++i)
                 if (hroots[i].im == 0.0) h1 = hroots[i].re;
@@ -1232,7 +1280,7 @@ __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const 
         if constexpr (FIRSTPT) {
             cplx c[5], root[4];
             c[0] = mk(one); c[1] = mk(0.0); c[2] = mk(cc); c[3] = mk(dd); c[4] = mk(ee);
-            zroots(c, 4, root);
+            zroots<4>(c, root);
             int nreal = 0;
             for (int i = 0; i < 4; ++i) nreal += (root[i].im == 0.0);
             if (nreal == 2) {
@@ -1288,7 +1336,7 @@ __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const 
             co[0] = mk(one); co[1] = mk(-cc / se); co[2] = mk(-one);
             co[3] = mk(se * (two * cc / ee - (dd / ee) * (dd / ee)));
             co[4] = mk(-one); co[5] = mk(-cc / se); co[6] = mk(one);
-            zroots(co, 6, hr);
+            zroots<6>(co, hr);
             double h1 = 0.0;
             for (int i = 0; i < 6 && h1 == 0.0; ++i)
                 if (hr[i].im == 0.0) h1 = hr[i].re;
@@ -1315,7 +1363,7 @@ __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const 
             co[4] = mk(-one / ee);
             co[5] = mk(-cc / ee);
             co[6] = mk(one);
-            zroots(co, 6, hr);
+            zroots<6>(co, hr);
             double mn2 = 0.0;
             for (int i = 0; i < 6 && mn2 == 0.0; ++i)
                 if (hr[i].im == 0.0) mn2 = hr[i].re;

@@ -16,11 +16,11 @@ extern "C" void probe_roots(const double* coef, int n, int mixed, double* roots,
         int fb = 0;
         if (mixed) {
             if (!np::gk::zroots4_mixed(c, r)) {
-                np::gk::zroots(c, 4, r);
+                np::gk::zroots<4>(c, r);
                 fb = 1;
             }
         } else {
-            np::gk::zroots(c, 4, r);
+            np::gk::zroots<4>(c, r);
         }
         for (int j = 0; j < 4; ++j) {
             roots[8 * i + 2 * j] = r[j].re;
