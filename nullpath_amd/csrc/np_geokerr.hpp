@@ -647,47 +647,81 @@ __device__ __forceinline__ void laguer_f(const cplxf* a, cplxf& x) {
     }
 }
 
-// The float deflation of zroots4_mixed from degree J down to 1; false when a
-// root is not usable (non-finite, or its imaginary part between 1e-5 and
-// 1e-2 of its modulus).
-template <int J>
-__device__ __forceinline__ bool deflate4_f(cplxf* ad, cplxf* rf, cplxf& xstart, int& ncc) {
-    if constexpr (J >= 1) {
-        cplxf x = xstart;
-        if (J < 4 && rf[J].im != 0.0f && ncc == 0) {
-            xstart = mkf(rf[J].re, -rf[J].re);
-            ncc = 1;
-        } else {
-            xstart = mkf(0.0f);
-            ncc = 0;
-        }
-        laguer_f<J>(ad, x);
-        const float ax = cabs_f(x);
-        if (!(ax > 0.0f) || !isfinite(ax)) return false;
-        const float rim = fabsf(x.im) / ax;
-        if (rim <= 1e-5f) x = mkf(x.re);
-        else if (rim < 1e-2f) return false;
-        rf[J - 1] = x;
-        cplxf b = ad[J];
+// Float seeds of the four roots of U(u) = 1 + cc u^2 + dd u^3 + ee u^4 in
+// closed form, with no data-dependent iteration (so the lanes of a wave stay
+// together): with x = 1/u, x^4 U(1/x) = x^4 + cc x^2 + dd x + ee is a
+// depressed quartic.  Ferrari's method takes the largest root m of the
+// resolvent cubic m^3 + cc m^2 + (cc^2/4 - ee) m - dd^2/8 = 0 (Viete's
+// trigonometric form when it has three real roots, Cardano's otherwise) and
+// splits the quartic into x^2 - s x + (cc/2 + m + dd/(2s)) and x^2 + s x +
+// (cc/2 + m - dd/(2s)), s = sqrt(2m), whose constant terms multiply to ee.
+// False when m is not clearly positive (dd near 0: the split degenerates).
+__device__ __forceinline__ bool quartic_u_seeds_f(float cc, float dd, float ee, cplxf* rf) {
+    const float A = cc, B = 0.25f * cc * cc - ee, Cc = -0.125f * dd * dd;
+    const float A3 = A * (1.0f / 3.0f);
+    // depressed resolvent t^3 + P t + Q = 0, m = t - A/3
+    const float P = B - A * A3;
+    const float Q = (2.0f / 27.0f) * A * A * A - A3 * B + Cc;
+    const float disc = 0.25f * Q * Q + P * P * P * (1.0f / 27.0f);
+    float t;
+    if (disc <= 0.0f) {
+        const float sp = sqrt_f(fmaxf(-P * (1.0f / 3.0f), 0.0f));
+        const float c3 = (sp > 0.0f) ? fminf(fmaxf(-0.5f * Q * rcp_f(sp * sp * sp), -1.0f), 1.0f) : 0.0f;
+        t = 2.0f * sp * cosf(acosf(c3) * (1.0f / 3.0f));
+    } else {
+        const float w = cbrtf(-0.5f * Q - copysignf(sqrt_f(disc), Q));
+        t = (w != 0.0f) ? w - P * rcp_f(3.0f * w) : 0.0f;
+    }
+    const float m = t - A3;
+    if (!(m > 1e-4f * (fabsf(cc) + sqrt_f(fabsf(ee))))) return false;
+    const float sq = sqrt_f(2.0f * m);
+    const float hh = 0.5f * cc + m, kk = dd * rcp_f(2.0f * sq);
+    // the two constant terms multiply to ee: the one of smaller magnitude
+    // (where hh -+ kk cancels, e.g. the root u ~ -dd/ee of a tiny ee) from
+    // ee over the other
+    float c1 = hh + kk, c2 = hh - kk;
+    if (fabsf(c1) >= fabsf(c2)) c2 = ee * rcp_f(c1);
+    else c1 = ee * rcp_f(c2);
 #pragma unroll
-        for (int jj = J - 1; jj >= 0; --jj) {
-            const cplxf c = ad[jj];
-            ad[jj] = b;
-            b = x * b + c;
+    for (int qd = 0; qd < 2; ++qd) {
+        // x^2 + b x + c
+        const float b = (qd == 0) ? -sq : sq, c = (qd == 0) ? c1 : c2;
+        const float D = b * b - 4.0f * c;
+        cplxf x1, x2;
+        if (D >= 0.0f) {
+            const float r1 = -0.5f * (b + copysignf(sqrt_f(D), b));
+            x1 = mkf(r1);
+            x2 = mkf(c * rcp_f(r1));
+        } else {
+            const float im = 0.5f * sqrt_f(-D);
+            x1 = mkf(-0.5f * b, im);
+            x2 = mkf(-0.5f * b, -im);
         }
-        return deflate4_f<J - 1>(ad, rf, xstart, ncc);
+        // u = 1 / x
+        const float i1 = rcp_f(x1.re * x1.re + x1.im * x1.im), i2 = rcp_f(x2.re * x2.re + x2.im * x2.im);
+        rf[2 * qd] = mkf(x1.re * i1, -x1.im * i1);
+        rf[2 * qd + 1] = mkf(x2.re * i2, -x2.im * i2);
     }
     return true;
 }
 
+// The quartic's roots for GEOMU's compared mode: float closed-form seeds
+// (quartic_u_seeds_f), classified real or complex in float, then three
+// Newton steps in double on the full polynomial; false (the caller runs the
+// FP64 ZROOTS) where the float phase cannot decide: a seed that is not
+// finite, an imaginary part between 1e-5 and 1e-2 of the modulus, or two
+// real seeds within 1e-3 of each other.
 __device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
-    cplxf ad[5];
-#pragma unroll
-    for (int j = 0; j <= 4; ++j) ad[j] = mkf((float)a[j].re, (float)a[j].im);
-    cplxf xstart = mkf(0.0f);
-    int ncc = 0;
     cplxf rf[4];
-    if (!deflate4_f<4>(ad, rf, xstart, ncc)) return false;
+    if (!quartic_u_seeds_f((float)a[2].re, (float)a[3].re, (float)a[4].re, rf)) return false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float ax = cabs_f(rf[j]);
+        if (!(ax > 0.0f) || !isfinite(ax)) return false;
+        const float rim = fabsf(rf[j].im) / ax;
+        if (rim <= 1e-5f) rf[j] = mkf(rf[j].re);
+        else if (rim < 1e-2f) return false;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
