@@ -93,7 +93,7 @@ ADJ_BWD_STEP = ADJ_FWD_STEP + 4 + 3 * ADJ_RHS + 4 * ADJ_VJP + 76
 # (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)), i.e. the flops the
 # active lanes executed, division and square-root sequences included
 # (profiles/geomu_2p20_counters.txt).
-GEOMU_FLOPS_PER_RAY = 7456
+GEOMU_FLOPS_PER_RAY = 7092
 
 
 def valu_flops(c: dict, prec: str):
