@@ -436,7 +436,7 @@ def batch_frames(nframes: int = 64, seed: int = 4096):
 def bench_batch(spp: int = 2, cpu_pixels_per_frame: int = 0) -> dict:
     """One persistent launch over the 64-frame uneven batch, after the
     autotuner picked block size, refill chunk, FP32C kernel variant and
-    FP64 radius on a 1/8-resolution copy; beside it the same frames as 64
+    FP64 radius on a 1/4-resolution copy; beside it the same frames as 64
     separate single-frame launches with the default launch shape."""
     import torch
 
@@ -444,7 +444,7 @@ def bench_batch(spp: int = 2, cpu_pixels_per_frame: int = 0) -> dict:
     frames = batch_frames()
     total = sum(f.width * f.height for f in frames)
     t0 = time.perf_counter()
-    best, trials = npa.autotune_batch(frames, spp=spp, subsample=8)
+    best, trials = npa.autotune_batch(frames, spp=spp, subsample=4)
     tune_s = time.perf_counter() - t0
     out = torch.zeros(3 * total, dtype=torch.float32, device="cuda")
     cnt = npa.new_counters()

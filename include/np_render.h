@@ -100,6 +100,15 @@ typedef struct np_render_params {
                                  radial motion dominates (Delta p_r^2 >= P^2 / (4 Delta)), so the
                                  square root is well conditioned; 0 = off (the reference never
                                  renormalises) */
+    int32_t defer_fp64;       /* NP_PREC_FP32C, adaptive: whether a lane whose next attempt runs in
+                                 FP64 (inside fp64_radius, or the re-run of a float attempt) waits
+                                 until enough lanes of its wave need one, so the wave runs its FP64
+                                 branch for several lanes at once.  Changes no result (each ray's
+                                 attempts are the same), only the time.  1 = defer, -1 = never,
+                                 0 = the library's choice: defer when the FP64 sphere seen from the
+                                 camera spans more than a tenth of the vertical half field of view
+                                 (asin(fp64 radius / camera radius) > 0.1 fov_y/2), where many
+                                 rays run FP64 steps */
 } np_render_params;
 
 /* Fill *p with the reference renderer's defaults (render.cu:15-46 with
@@ -119,6 +128,7 @@ static inline void np_render_params_default(np_render_params* p) {
     p->crossing = NP_CROSSING_LINEAR;
     p->estimate_every_step = 0;
     p->renorm_pr = 0;
+    p->defer_fp64 = 0;
 }
 
 /* Kerr prograde ISCO radius in units of M (Bardeen, Press & Teukolsky 1972,

@@ -154,13 +154,15 @@ void np_pass_graph_destroy(np_pass_graph* graph);
  * block_size: 64, 128 or 256 threads; waves_per_simd: kernel variant as in
  * np_render_params (FP32C: 4 = 128-VGPR cap, 3 = 168-VGPR cap, 2 = compiler budget), 0 = each
  * frame's own; refill_chunk: consecutive pixels a lane takes per refill; fp64_radius: FP32C
- * radius (absolute units) of FP64 steps, 0 = each frame's own.
+ * radius (absolute units) of FP64 steps, 0 = each frame's own; defer_fp64: as in
+ * np_render_params (1 = defer FP64 attempts, -1 = never), 0 = each frame's own.
  */
 typedef struct np_tuning {
     int32_t block_size;
     int32_t waves_per_simd;
     int32_t refill_chunk;
     double  fp64_radius;
+    int32_t defer_fp64;
 } np_tuning;
 
 /*

@@ -482,7 +482,7 @@ def render_batch(frames, out, spp=1, sample_offset=0, records=None, counters=Non
     check(code, "np_render_batch_noalloc" if noalloc else "np_render_batch")
 
 
-def autotune_batch(frames, spp=1, subsample=8, max_trials=64):
+def autotune_batch(frames, spp=1, subsample=4, max_trials=64):
     """np_autotune_batch: returns (best Tuning, [(Tuning dict, ms), ...])."""
     import ctypes as C
 

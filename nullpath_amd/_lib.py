@@ -33,6 +33,7 @@ class RenderParams(C.Structure):
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
         ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
         ("crossing", C.c_int32), ("estimate_every_step", C.c_int32), ("renorm_pr", C.c_int32),
+        ("defer_fp64", C.c_int32),
     ]
 
     def copy(self) -> "RenderParams":
@@ -63,7 +64,8 @@ class Tuning(C.Structure):
     """np_tuning (include/nullpath.h)."""
 
     _fields_ = [("block_size", C.c_int32), ("waves_per_simd", C.c_int32),
-                ("refill_chunk", C.c_int32), ("fp64_radius", C.c_double)]
+                ("refill_chunk", C.c_int32), ("fp64_radius", C.c_double),
+                ("defer_fp64", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -626,6 +626,7 @@ struct StepF64 {
     // an accepted step of size R.h advances the affine parameter
     __device__ static void advance(Ray& R) { R.lam += R.h; }
     static constexpr unsigned kRenormEvery = 16;
+    static constexpr bool kDefer = false;
     __device__ static void post_accept(const FrameConst&, Ray&, bool) {}
 
     __device__ static double r(const Ray& R) { return R.y.r; }
@@ -1091,6 +1092,12 @@ constexpr float kLazyError = 1e-3f;
 // FP64 re-run): twice the tolerance.
 constexpr float kClearReject = 2.0f;
 
+// Deferred FP64 attempts of the FP32C path (StepF32C::kDefer): the wave runs
+// its FP64 block once kDeferLanes lanes wait for it or kDeferIters
+// iterations after it last ran.
+constexpr int kDeferLanes = 24;
+constexpr uint32_t kDeferIters = 16;
+
 struct StepF32C {
     struct Ray : RayBase {
         FF5 y;            // y.t, the coordinate time, is integrated by TIME kernels only
@@ -1162,6 +1169,30 @@ struct StepF32C {
     // turning point the square root would amplify the rounding of K.  FP64 on
     // the float-float state; the sign of p_r is kept.
     static constexpr unsigned kRenormEvery = 16;
+
+    // FP64 attempts (mode 1 and 2) may be deferred by the persistent loop:
+    // a lane whose next attempt is FP64 waits until enough lanes of its wave
+    // need one, or a few iterations have passed, so the wave runs the FP64
+    // block for several lanes at once instead of beside every float
+    // iteration.  The ray's own sequence of attempts is unchanged.
+    static constexpr bool kDefer = true;
+    __device__ static bool pending64(const Ray& R) { return R.mode != 0; }
+
+    // step-size choice at the start of a step (render.cu:185-191) and the
+    // precision of its attempts: FP64 inside r_isl.  Idempotent until the
+    // attempt runs.
+    __device__ static void start_step(const FrameConst& F, Ray& R) {
+        if (R.attempt == 0 && R.mode != 2) {
+            const float hmin = (float)HMIN, hmax = (float)HMAX;
+            const float rf = ff_f(R.y.r);
+            const float scale = fminf(fmaxf(1.0f, rf / (0.5f * F.Rs)), 100.0f);
+            const float h_base = fminf(fmaxf(STEP0 * scale, hmin), hmax);
+            const float h = (R.h_adapt > 0.0f) ? R.h_adapt : h_base;
+            R.h = fminf(fmaxf(h, hmin), hmax);
+            R.mode = (ff_to_d(R.y.r) < F.r_isl) ? 1 : 0;
+        }
+    }
+
     __device__ static void post_accept(const FrameConst& F, Ray& R, bool tick) {
         if (!F.renorm || !tick) return;
         const double r = ff_to_d(R.y.r), th = ff_to_d(R.y.th), pr = ff_to_d(R.y.pr), pth = ff_to_d(R.y.pth);
@@ -1248,7 +1279,8 @@ struct StepF32C {
         return 0;
     }
 
-    template <bool ADAPTIVE, bool TIME>
+    // STARTED: start_step has run for this attempt (the caller deferred it).
+    template <bool ADAPTIVE, bool TIME, bool STARTED = false>
     __device__ static int attempt(const FrameConst& F, Ray& R, int& rhs_count, int& rhs64,
                                   int& lazy_att) {
         const float hmin = (float)HMIN, hmax = (float)HMAX;
@@ -1273,15 +1305,7 @@ struct StepF32C {
             const bool ok = rr > F.r_plus + 1e-9 && tt > 1e-9 && tt < NP_PI_D - 1e-9;
             return ok ? 1 : 2;
         }
-        if (R.attempt == 0 && R.mode != 2) {
-            // step-size choice at the start of a step (render.cu:185-191)
-            const float rf = ff_f(R.y.r);
-            const float scale = fminf(fmaxf(1.0f, rf / (0.5f * F.Rs)), 100.0f);
-            const float h_base = fminf(fmaxf(STEP0 * scale, hmin), hmax);
-            const float h = (R.h_adapt > 0.0f) ? R.h_adapt : h_base;
-            R.h = fminf(fmaxf(h, hmin), hmax);
-            R.mode = (ff_to_d(R.y.r) < F.r_isl) ? 1 : 0;
-        }
+        if (!STARTED) start_step(F, R);
         if (R.mode != 0) {
             R.skip = 0;
             return attempt64<TIME>(F, R, rhs64);
@@ -1443,6 +1467,7 @@ struct StepC64 {
     __device__ static double lam(const Ray& R) { return R.lam; }
     __device__ static void advance(Ray& R) { R.lam += R.h; }
     static constexpr unsigned kRenormEvery = 16;
+    static constexpr bool kDefer = false;
     __device__ static void post_accept(const FrameConst&, Ray&, bool) {}
 
     template <bool INV>
@@ -1584,7 +1609,7 @@ __device__ __forceinline__ const FrameConst& cold(const BatchFrame* B) {
 // TIME: also integrate the coordinate time and the affine parameter and
 // write each sample's end t, phi and lambda to times (same indexing as the
 // ray records).
-template <class S, bool ADAPTIVE, bool INV, bool TIME>
+template <class S, bool ADAPTIVE, bool INV, bool TIME, bool DEFER_KERNEL>
 __device__ __forceinline__ void
 kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int spp,
           float* __restrict__ out_base, const int accumulate,
@@ -1613,6 +1638,10 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
     uint32_t c_rhs = 0;
     uint64_t w_att64 = 0, w_lazy = 0;
     uint32_t iter = 0;   // attempt iterations of this wave (scalar)
+    uint32_t wait64 = 0; // iterations since the deferred FP64 block last ran (scalar)
+    // kernels built with deferral (DEFER_KERNEL) defer in frames with
+    // F.defer64 set; the others carry none of its bookkeeping
+    constexpr bool DEFER = DEFER_KERNEL && S::kDefer && ADAPTIVE;
     if (spp <= 0) done = true;
 
     // Outer loop: one frame at a time; inner loop: one integrator attempt per
@@ -1666,10 +1695,36 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
             int rhs64 = 0, lazy_att = 0;
             bool ended = false;
             ++iter;
-            if (has) {
+            bool act = has;
+            if constexpr (DEFER) {
+                bool p64 = false;
+                if (has) {
+                    S::start_step(F, R);
+                    p64 = S::pending64(R);
+                }
+                // F.defer64: lanes whose next attempt is FP64 sit out this
+                // iteration unless kDeferLanes of them wait, kDeferIters
+                // iterations have passed since the FP64 block last ran, no
+                // lane has a float attempt to make, or the frame has no
+                // pixels left for this wave (its last rays then finish at
+                // full speed) -- a wave-uniform decision
+                const uint64_t pend = __ballot(p64);
+                if (F.defer64 && pend) {
+                    ++wait64;
+                    const bool run64 = __ballot(has && !p64) == 0 || __ballot(done) != 0 ||
+                                       __popcll(pend) >= kDeferLanes || wait64 >= kDeferIters;
+                    if (run64) wait64 = 0;
+                    else act = has && !p64;
+                }
+            }
+            if (act) {
                 int rhs_count = 0;
                 row[LS_ATT_RAY] += 1;
-                const int st = S::template attempt<ADAPTIVE, TIME>(F, R, rhs_count, rhs64, lazy_att);
+                int st;
+                if constexpr (DEFER)   // start_step has run above
+                    st = S::template attempt<ADAPTIVE, TIME, true>(F, R, rhs_count, rhs64, lazy_att);
+                else
+                    st = S::template attempt<ADAPTIVE, TIME>(F, R, rhs_count, rhs64, lazy_att);
                 c_rhs += (uint32_t)(rhs_count + rhs64);
                 int cls = -1;
                 if (st == 1) S::post_accept(F, R, (iter % S::kRenormEvery) == 0);
@@ -1807,15 +1862,15 @@ kerr_body(const BatchFrame* __restrict__ frames, const int nframes, const int sp
 // 512 / W VGPRs (rounded down to the allocation granule): FP32C W = 3, 4
 // (168, 128), the FP64 comparator W = 3, the reference-formula FP64 path
 // W = 2 (256, no AGPRs).
-template <class S, bool ADAPTIVE, bool INV>
+template <class S, bool ADAPTIVE, bool INV, bool DEFER = false>
 __global__ void __launch_bounds__(256) kerr_render(NP_KERNEL_ARGS) {
-    kerr_body<S, ADAPTIVE, INV, false>(NP_KERNEL_PASS);
+    kerr_body<S, ADAPTIVE, INV, false, DEFER>(NP_KERNEL_PASS);
 }
 
-template <class S, bool ADAPTIVE, bool INV, int W, bool TIME = false>
+template <class S, bool ADAPTIVE, bool INV, int W, bool TIME = false, bool DEFER = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 kerr_render_capped(NP_KERNEL_ARGS) {
-    kerr_body<S, ADAPTIVE, INV, TIME>(NP_KERNEL_PASS);
+    kerr_body<S, ADAPTIVE, INV, TIME, DEFER>(NP_KERNEL_PASS);
 }
 
 #undef NP_KERNEL_ARGS
@@ -1834,7 +1889,17 @@ const void* kernel_ptr_capped(bool inv) {
 }
 
 template <bool A>
-const void* kernel_ptr_variant(int precision, bool inv, int variant) {
+const void* kernel_ptr_variant(int precision, bool inv, int variant, bool defer) {
+    if constexpr (A) {
+        // FP32C with deferred FP64 attempts (adaptive steps, no monitor)
+        if (precision == NP_PREC_FP32C && defer && !inv) {
+            if (variant == 4)
+                return reinterpret_cast<const void*>(&kerr_render_capped<StepF32C, A, false, 4, false, true>);
+            if (variant == 3)
+                return reinterpret_cast<const void*>(&kerr_render_capped<StepF32C, A, false, 3, false, true>);
+            return reinterpret_cast<const void*>(&kerr_render<StepF32C, A, false, true>);
+        }
+    }
     if (precision == NP_PREC_FP32C) {
         if (variant == 4) return kernel_ptr_capped<StepF32C, A, 4>(inv);
         if (variant == 3) return kernel_ptr_capped<StepF32C, A, 3>(inv);
@@ -1861,13 +1926,15 @@ const void* kernel_ptr_time(int precision) {
 
 // variant (FrameConst::variant): waves-per-SIMD cap, 0 = none.  time:
 // the TIME kernel of the precision (nullptr with the invariant monitor).
-const void* select_kernel(int precision, bool adaptive, bool inv, int variant, bool time) {
+// defer: the FP32C kernel that defers FP64 attempts in frames with
+// FrameConst::defer64 set (adaptive, no monitor, no TIME).
+const void* select_kernel(int precision, bool adaptive, bool inv, int variant, bool time, bool defer) {
     if (time) {
         if (inv) return nullptr;
         return adaptive ? kernel_ptr_time<true>(precision) : kernel_ptr_time<false>(precision);
     }
-    return adaptive ? kernel_ptr_variant<true>(precision, inv, variant)
-                    : kernel_ptr_variant<false>(precision, inv, variant);
+    return adaptive ? kernel_ptr_variant<true>(precision, inv, variant, defer)
+                    : kernel_ptr_variant<false>(precision, inv, variant, defer);
 }
 
 __global__ void init_frame_record(const BatchFrame rec, BatchFrame* __restrict__ dst,
@@ -1909,9 +1976,9 @@ int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool ada
                 int variant, int spp, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk,
-                np_ray_times* d_times) {
+                np_ray_times* d_times, bool defer) {
     if (block <= 0 || block > 256 || block % 64 != 0) return NP_EINVAL;
-    const void* fn = select_kernel(precision, adaptive, inv, variant, d_times != nullptr);
+    const void* fn = select_kernel(precision, adaptive, inv, variant, d_times != nullptr, defer);
     if (!fn) return NP_EINVAL;
     void* args[] = {&d_frames, &nframes, &spp, &d_out, &accumulate, &d_inv, &d_counters, &d_records,
                     &d_work, &chunk, &d_times};
@@ -1921,8 +1988,8 @@ int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool ada
 }
 
 int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu,
-                   bool time) {
-    const void* fn = select_kernel(precision, adaptive, inv, variant, time);
+                   bool time, bool defer) {
+    const void* fn = select_kernel(precision, adaptive, inv, variant, time, defer);
     if (!fn) return NP_EINVAL;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block,
                                                                       lane_lds_bytes(inv, block));

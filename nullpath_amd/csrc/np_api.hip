@@ -96,6 +96,17 @@ int persistent_grid(DeviceState* d, int per_cu, int block, uint64_t items) {
     return (int)grid;
 }
 
+// The library's choice of deferred FP64 attempts (FrameConst::defer64 = 1)
+// also needs many rays per lane of the persistent grid: a deferred ray runs
+// its FP64 steps a few times slower, and with only a few rays per lane the
+// launch ends when its slowest lanes do.  Measured on 960x540 (2 rays per
+// lane: 7-20 % slower with deferral) and 1920x1080x2 (16: up to 40 % faster).
+constexpr uint64_t kDeferMinRaysPerLane = 8;
+
+void settle_defer(FrameConst& F, uint64_t rays, uint64_t lanes) {
+    if (F.defer64 == 1 && rays < kDeferMinRaysPerLane * lanes) F.defer64 = 0;
+}
+
 int render_launch(const np_render_params* p, int y0, int rows, int spp, int sample_offset,
                   float* d_rgb, int accumulate, float* d_inv, np_trace_counters* d_counters,
                   np_ray_record* d_records, hipStream_t stream, np_ray_times* d_times = nullptr) {
@@ -109,6 +120,20 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     const int y_end = std::min(y0 + rows, p->height);
     const uint64_t npix = y_end > y0 ? (uint64_t)(y_end - y0) * (uint64_t)p->width : 0ull;
     if (npix == 0 || spp == 0) return NP_OK;
+    const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
+    const bool inv = p->check_inv != 0;
+    const int block = 256;
+    int per_cu = 0;
+    if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu, d_times != nullptr) != NP_OK)
+        per_cu = 1;
+    int grid = persistent_grid(d, per_cu, block, npix);
+    settle_defer(F, npix * (uint64_t)spp, (uint64_t)grid * (uint64_t)block);
+    const bool defer = F.defer64 != 0 && d_times == nullptr;
+    if (defer) {
+        if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu, false, true) != NP_OK)
+            per_cu = 1;
+        grid = persistent_grid(d, per_cu, block, npix);
+    }
     BatchFrame rec;
     rec.F = F;
     rec.npix = npix;
@@ -121,15 +146,8 @@ int render_launch(const np_render_params* p, int y0, int rows, int spp, int samp
     BatchFrame* d_rec = nullptr;
     err = take_frame_slot(d, stream, rec, &d_rec, &slot);
     if (err != NP_OK) return err;
-    const bool adaptive = p->integrator == NP_INTEGRATOR_ADAPTIVE;
-    const bool inv = p->check_inv != 0;
-    const int block = 256;
-    int per_cu = 0;
-    if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu, d_times != nullptr) != NP_OK)
-        per_cu = 1;
-    const int grid = persistent_grid(d, per_cu, block, npix);
     return launch_kerr(d_rec, 1, p->precision, adaptive, inv, F.variant, spp, d_rgb, accumulate, d_inv,
-                       d_counters, d_records, slot, stream, grid, block, 1, d_times);
+                       d_counters, d_records, slot, stream, grid, block, 1, d_times, defer);
 }
 
 __global__ void zero_u64_n(unsigned long long* p, int n) {
@@ -151,6 +169,7 @@ int batch_frames(const np_render_params* frames, int nframes, int spp, int sampl
             // 0 keeps the frame's own kernel variant
             if (t->waves_per_simd != 0) q.waves_per_simd = t->waves_per_simd;
             if (t->fp64_radius > 0.0) q.fp64_radius = t->fp64_radius;
+            if (t->defer_fp64 != 0) q.defer_fp64 = t->defer_fp64;
         }
         int err = make_frame(&q, &out[i].F);
         if (err != NP_OK) return err;
@@ -173,6 +192,29 @@ int check_tuning(const np_tuning* tuning, np_tuning* t) {
     return NP_OK;
 }
 
+// Persistent grid of a batch launch (blocks).
+uint64_t batch_grid(const np_render_params* frames, const BatchFrame* host0, int cus, uint64_t total_px,
+                    const np_tuning& t, bool defer) {
+    const bool adaptive = frames[0].integrator == NP_INTEGRATOR_ADAPTIVE;
+    int per_cu = 0;
+    if (occupancy_kerr(frames[0].precision, adaptive, false, host0->F.variant, t.block_size, &per_cu, false,
+                       defer) != NP_OK)
+        per_cu = 1;
+    if (per_cu <= 0) per_cu = 1;
+    uint64_t grid = (uint64_t)per_cu * (uint64_t)cus;
+    const uint64_t need = (total_px + (uint64_t)t.block_size - 1) / (uint64_t)t.block_size;
+    if (need < grid) grid = need;
+    if (grid < 1) grid = 1;
+    return grid;
+}
+
+// The deferral choice of every frame of a batch, from the whole launch.
+void settle_batch_defer(const np_render_params* frames, BatchFrame* host, int nframes, int cus, int spp,
+                        uint64_t total_px, const np_tuning& t) {
+    const uint64_t lanes = batch_grid(frames, host, cus, total_px, t, false) * (uint64_t)t.block_size;
+    for (int i = 0; i < nframes; ++i) settle_defer(host[i].F, total_px * (uint64_t)spp, lanes);
+}
+
 // Persistent batch launch over a frame table and cleared per-frame counters
 // already in device memory.
 int batch_kernel(const np_render_params* frames, int nframes, const BatchFrame* host0,
@@ -183,17 +225,12 @@ int batch_kernel(const np_render_params* frames, int nframes, const BatchFrame* 
     if (hipGetLastError() != hipSuccess) return NP_ELAUNCH;
     const bool adaptive = frames[0].integrator == NP_INTEGRATOR_ADAPTIVE;
     const int variant = host0->F.variant;
-    int per_cu = 0;
-    if (occupancy_kerr(frames[0].precision, adaptive, false, variant, t.block_size, &per_cu) != NP_OK)
-        per_cu = 1;
-    if (per_cu <= 0) per_cu = 1;
-    uint64_t grid = (uint64_t)per_cu * (uint64_t)cus;
-    const uint64_t need = (total_px + (uint64_t)t.block_size - 1) / (uint64_t)t.block_size;
-    if (need < grid) grid = need;
-    if (grid < 1) grid = 1;
+    bool defer = false;
+    for (int i = 0; i < nframes; ++i) defer = defer || host0[i].F.defer64 != 0;
+    const uint64_t grid = batch_grid(frames, host0, cus, total_px, t, defer);
     return launch_kerr(d_frames, nframes, frames[0].precision, adaptive, false, variant, spp, d_rgb, 0,
                        nullptr, d_counters, d_records, d_work, stream, (int)grid, t.block_size,
-                       t.refill_chunk);
+                       t.refill_chunk, nullptr, defer);
 }
 
 int batch_launch(const np_render_params* frames, int nframes, int spp, int sample_offset,
@@ -210,6 +247,7 @@ int batch_launch(const np_render_params* frames, int nframes, int spp, int sampl
     DeviceState* d = device_state(&err);
     if (!d) return err;
     if (total_px == 0 || spp == 0) return NP_OK;
+    settle_batch_defer(frames, host.data(), nframes, d->cus, spp, total_px, t);
     const size_t table = host.size() * sizeof(BatchFrame);
     const size_t bytes = table + (size_t)nframes * sizeof(unsigned long long);
     void* buf = nullptr;
@@ -303,6 +341,7 @@ int batch_launch_noalloc(const np_render_params* frames, int nframes, int spp, i
     err = batch_frames(frames, nframes, spp, sample_offset, &t, s->stage, &total_px);
     if (err != NP_OK) return err;
     if (total_px == 0 || spp == 0) return NP_OK;
+    settle_batch_defer(frames, s->stage, nframes, s->cus, spp, total_px, t);
     BatchFrame* d_frames = reinterpret_cast<BatchFrame*>(s->ptr);
     unsigned long long* d_work = reinterpret_cast<unsigned long long*>(s->ptr + kScratchTable);
     // the staging buffer is reused by the next call: wait for the copy
@@ -379,7 +418,14 @@ int create_pass_graph(const np_render_params* p, int spp, int passes, float* d_a
     const int block = 256;
     int per_cu = 0;
     if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu) != NP_OK) per_cu = 1;
-    const int grid = persistent_grid(d, per_cu, block, rec.npix);
+    int grid = persistent_grid(d, per_cu, block, rec.npix);
+    settle_defer(rec.F, rec.npix * (uint64_t)spp, (uint64_t)grid * (uint64_t)block);
+    const bool defer = rec.F.defer64 != 0;
+    if (defer) {
+        if (occupancy_kerr(p->precision, adaptive, inv, F.variant, block, &per_cu, false, true) != NP_OK)
+            per_cu = 1;
+        grid = persistent_grid(d, per_cu, block, rec.npix);
+    }
     err = NP_OK;
     if (hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking) != hipSuccess) {
         g->capture = nullptr;
@@ -396,7 +442,7 @@ int create_pass_graph(const np_render_params* p, int spp, int passes, float* d_a
             if (cerr == NP_OK)
                 cerr = launch_kerr(g->d_rec, 1, p->precision, adaptive, inv, F.variant, spp, d_accum, 1,
                                    inv ? d_inv : nullptr, nullptr, nullptr, g->d_work, g->capture, grid,
-                                   block, 1);
+                                   block, 1, nullptr, defer);
         }
         // end the capture in every case, so the stream leaves capture mode
         if (hipStreamEndCapture(g->capture, &g->graph) != hipSuccess) {
@@ -504,10 +550,23 @@ int autotune_batch(const np_render_params* frames, int nframes, int spp, int sub
     if (err == NP_OK && hipDeviceSynchronize() != hipSuccess) err = NP_ELAUNCH;
     double best_ms = 1e300;
     int n = 0;
+    int derr = NP_OK;
+    DeviceState* dev = err == NP_OK ? device_state(&derr) : nullptr;
+    if (!dev && err == NP_OK) err = derr;
+    std::vector<BatchFrame> full((size_t)nframes);
+    std::vector<np_render_params> trial_frames(sub);
     for (const auto& t : cand) {
+        if (err != NP_OK || !dev) break;
+        // each trial defers FP64 attempts in the frames where the full-size
+        // launch under this tuning would (the subsampled batch alone has too
+        // few rays per lane for the library's choice)
+        uint64_t full_px = 0;
+        err = batch_frames(frames, nframes, spp, 0, &t, full.data(), &full_px);
         if (err != NP_OK) break;
+        settle_batch_defer(frames, full.data(), nframes, dev->cus, spp, full_px, t);
+        for (int i = 0; i < nframes; ++i) trial_frames[i].defer_fp64 = full[i].F.defer64 ? 1 : -1;
         if (hipEventRecord(e0, nullptr) != hipSuccess) { err = NP_ELAUNCH; break; }
-        err = batch_launch(sub.data(), nframes, spp, 0, d_rgb, nullptr, nullptr, &t, nullptr);
+        err = batch_launch(trial_frames.data(), nframes, spp, 0, d_rgb, nullptr, nullptr, &t, nullptr);
         if (err != NP_OK) break;
         float ms = 0.0f;
         if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||

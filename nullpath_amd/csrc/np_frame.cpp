@@ -126,6 +126,16 @@ int make_frame(const np_render_params* p, FrameConst* f) {
     f->lazy = p->estimate_every_step == 0 ? 1 : 0;
     if (p->renorm_pr != 0 && p->renorm_pr != 1) return NP_EINVAL;
     f->renorm = p->precision == NP_PREC_FP32C ? p->renorm_pr : 0;
+    if (p->defer_fp64 < -1 || p->defer_fp64 > 1) return NP_EINVAL;
+    {
+        // the library's choice: the FP64 sphere's angular radius from the
+        // camera against the vertical half field of view
+        const double half_fov = 0.5 * (double)p->fov_y_deg * 3.141592653589793238462643383279502884 / 180.0;
+        const double ang = std::asin(std::fmin(1.0, f->r_isl / (double)r_cam));
+        const bool wide = ang > 0.1 * half_fov;
+        const int on = p->defer_fp64 == 1 ? 2 : (p->defer_fp64 == 0 && wide) ? 1 : 0;
+        f->defer64 = (p->precision == NP_PREC_FP32C && p->integrator == NP_INTEGRATOR_ADAPTIVE) ? on : 0;
+    }
     f->r_lazyf = (float)(100.0 * M);
     f->s_eq = std::sin(3.141592653589793238462643383279502884 * 0.5);
     f->c_eq = std::cos(3.141592653589793238462643383279502884 * 0.5);

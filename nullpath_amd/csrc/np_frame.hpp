@@ -46,6 +46,10 @@ struct FrameConst {
     float r_lazyf;
     int32_t lazy;
     int32_t renorm;       // FP32C: p_r from H = 0 after accepted steps
+    // FP32C, adaptive: FP64 attempts deferred in the wave.  0 = no; 2 = yes
+    // (asked for); 1 = yes (the library's choice), which the launch turns
+    // off when the persistent grid has few rays per lane (settle_defer)
+    int32_t defer64;
 };
 
 // Returns 0 on success, NP_EINVAL when the parameters are outside what the

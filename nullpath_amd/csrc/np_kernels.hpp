@@ -28,13 +28,15 @@ struct BatchFrame {
 // pixels a lane takes per refill.
 // d_times (optional): the launch runs the kernel that also integrates t and
 // lambda and writes one np_ray_times per sample (inv must be false).
+// defer: the FP32C kernel that defers FP64 attempts in the frames whose
+// FrameConst::defer64 is set (adaptive steps, inv and d_times unset).
 int launch_kerr(const BatchFrame* d_frames, int nframes, int precision, bool adaptive, bool inv,
                 int variant, int spp, float* d_out, int accumulate, float* d_inv,
                 np_trace_counters* d_counters, np_ray_record* d_records,
                 unsigned long long* d_work, hipStream_t stream, int grid, int block, int chunk,
-                np_ray_times* d_times = nullptr);
+                np_ray_times* d_times = nullptr, bool defer = false);
 int occupancy_kerr(int precision, bool adaptive, bool inv, int variant, int block, int* blocks_per_cu,
-                   bool time = false);
+                   bool time = false, bool defer = false);
 
 // A cleared work-queue counter on the current device (cleared by a kernel
 // on `stream`, so the call may be graph-captured).

@@ -32,6 +32,7 @@ class RenderParams(C.Structure):
         ("disk_rout", C.c_double), ("emiss_q", C.c_float), ("seed", C.c_uint32),
         ("fp64_radius", C.c_double), ("waves_per_simd", C.c_int32),
         ("crossing", C.c_int32), ("estimate_every_step", C.c_int32), ("renorm_pr", C.c_int32),
+        ("defer_fp64", C.c_int32),
     ]
 
 

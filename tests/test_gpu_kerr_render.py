@@ -173,6 +173,27 @@ def test_fp32c_kernel_variants_agree(device):
         assert c3 == c, w
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("w", [4, 3, 2])
+def test_deferred_fp64_attempts_change_nothing(device, w):
+    """FP32C with the FP64 attempts deferred (defer_fp64 = 1: a lane whose
+    next attempt is FP64 waits for more such lanes of its wave) against never
+    deferred (-1), camera at 50 M where many rays run FP64 steps: every ray
+    makes the same attempts, so outputs, records and counters are bitwise
+    equal, in every kernel variant."""
+    runs = {}
+    for d in (-1, 1):
+        p = npa.kerr_disk_params(width=96, height=64, fov=30.0, seed=2, precision=1, crossing=1,
+                                 r_cam_Rs=25.0, waves_per_simd=w, defer_fp64=d)
+        runs[d] = _run_gpu(p, 2)
+    (i0, c0, r0), (i1, c1, r1) = runs[-1], runs[1]
+    print(f"[gpu] waves {w}: FP64 attempts {c0['attempts_f64']} of {c0['attempts']}", flush=True)
+    assert c0["attempts_f64"] > 0.01 * c0["attempts"]
+    assert np.array_equal(i0, i1)
+    assert np.array_equal(r0, r1)
+    assert c0 == c1
+
+
 def _batch_frames(precision):
     spins = [0.0, 0.5, 0.9, 0.998]
     sizes = [(40, 24), (32, 32), (48, 20), (24, 36)]
@@ -185,7 +206,9 @@ def _batch_frames(precision):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("tuning", [None, {"block_size": 128, "waves_per_simd": 2,
-                                           "refill_chunk": 4, "fp64_radius": 0.0}])
+                                           "refill_chunk": 4, "fp64_radius": 0.0},
+                                    {"block_size": 256, "waves_per_simd": 0, "refill_chunk": 1,
+                                     "fp64_radius": 0.0, "defer_fp64": 1}])
 def test_batch_equals_per_frame_renders(device, precision, tuning):
     """One persistent launch over frames of different size, spin and
     inclination gives, frame by frame, exactly the single-frame render."""
