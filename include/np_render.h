@@ -108,7 +108,9 @@ typedef struct np_render_params {
                                  0 = the library's choice: defer when the FP64 sphere seen from the
                                  camera spans more than a tenth of the vertical half field of view
                                  (asin(fp64 radius / camera radius) > 0.1 fov_y/2), where many
-                                 rays run FP64 steps */
+                                 rays run FP64 steps, and the launch has at least 8 rays per lane
+                                 of its persistent grid (with fewer, the deferred rays, a few
+                                 times slower, decide when the launch ends) */
 } np_render_params;
 
 /* Fill *p with the reference renderer's defaults (render.cu:15-46 with
