@@ -751,10 +751,25 @@ __device__ inline bool zroots4_mixed(const cplx* a, cplx* roots) {
     return true;
 }
 
-// SNCNDN (geokerr_original.f:3277-3340).
+// SNCNDN (geokerr_original.f:3277-3340).  Its two arrays of the descending
+// sequence, indexed by the iteration count, live on the device in LDS, one
+// column per thread (every kernel that reaches this runs blocks of at most
+// 256 threads): dynamically indexed, they would otherwise sit in private
+// (scratch) memory.  Same values, same order of operations.
 __device__ inline void sncndn(double uu, double emmc, double& sn, double& cn, double& dn) {
     const double CA = 3e-8;
-    double emc = emmc, u = uu, a, b, c = 0.0, d = 0.0, em[13], en[13];
+    double emc = emmc, u = uu, a, b, c = 0.0, d = 0.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    __shared__ double s_em[13 * 256], s_en[13 * 256];
+    double* const em_col = s_em + threadIdx.x;
+    double* const en_col = s_en + threadIdx.x;
+#define NP_EM(i) em_col[(i) * 256]
+#define NP_EN(i) en_col[(i) * 256]
+#else
+    double em[13], en[13];
+#define NP_EM(i) em[i]
+#define NP_EN(i) en[i]
+#endif
     int l = 0;
     if (emc != 0.0) {
         const bool bo = (emc < 0.0);
@@ -768,9 +783,9 @@ __device__ inline void sncndn(double uu, double emmc, double& sn, double& cn, do
         dn = 1.0;
         for (int i = 1; i <= 13; ++i) {
             l = i;
-            em[i - 1] = a;
+            NP_EM(i - 1) = a;
             emc = sqrt(emc);
-            en[i - 1] = emc;
+            NP_EN(i - 1) = emc;
             c = 0.5 * (a + emc);
             if (fabs(a - emc) <= CA * a) break;
             emc = a * emc;
@@ -782,10 +797,10 @@ __device__ inline void sncndn(double uu, double emmc, double& sn, double& cn, do
             a = cn / sn;
             c = a * c;
             for (int ii = l; ii >= 1; --ii) {
-                b = em[ii - 1];
+                b = NP_EM(ii - 1);
                 a = c * a;
                 c = dn * c;
-                dn = (en[ii - 1] + a) / (b + a);
+                dn = (NP_EN(ii - 1) + a) / (b + a);
                 a = c / b;
             }
             a = 1.0 / sqrt(c * c + 1.0);
@@ -803,6 +818,8 @@ __device__ inline void sncndn(double uu, double emmc, double& sn, double& cn, do
         dn = cn;
         sn = tanh(u);
     }
+#undef NP_EM
+#undef NP_EN
 }
 
 // ------------------------------- GEOMU --------------------------------------
