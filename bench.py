@@ -714,7 +714,8 @@ def bench_geokerr(n: int, s: int, cpu_geodesics: int) -> dict:
 
 
 def bench_geokerr_driver(n: int, cpu_geodesics: int) -> dict:
-    """The whole GEOKERR driver (np_geokerr_batch) in its two other
+    """The whole GEOKERR driver (np_geokerr_batch, split into per-sample
+    lanes; np_geokerr_batch_serial timed beside it) in its two other
     configurations: u-sampled with MUFILL (NUP 512, NPTS 64, KEXT 2) and
     mu-sampled (USEGEOR, NUP 512), both with PHIT, on n camera geodesics
     from u0 = 1e-4 (alpha, beta ~ U[-12, 12] M, a ~ U[0.05, 0.998],
@@ -751,19 +752,38 @@ def bench_geokerr_driver(n: int, cpu_geodesics: int) -> dict:
         rays = {"u0": t(np.full(n, 1e-4)), "uf": t(uf), "uout": t(np.full(n, 1e-4)), "mu0": t(mu0), "muf": t(muf),
                 "a": t(a), "l": t(lz), "q2": t(q2), "beta": t(be), "su": t(np.ones(n)), "sm": t(sm),
                 "tpm": t(tpm, torch.int32), "tpr": t(np.full(n, tpr), torch.int32)}
-        npa.geokerr_batch(opts, {k: v[:64] for k, v in rays.items()})    # warm-up
+        for serial in (False, True):   # warm-up
+            npa.geokerr_batch(opts, {k: v[:64] for k, v in rays.items()}, serial=serial)
         torch.cuda.synchronize()
         stream = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        res = npa.geokerr_batch(opts, rays, stream=stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1)
+
+        def timed(serial):
+            e0.record(stream)
+            r = npa.geokerr_batch(opts, rays, stream=stream, serial=serial)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return r, e0.elapsed_time(e1)
+
+        res, ms = timed(False)
+        ser, ms_serial = timed(True)
+        int_mismatch, float_rel = 0, 0.0
+        for k, v in res.items():
+            w = ser[k]
+            if v.dtype == torch.float64:
+                both_nan = torch.isnan(v) & torch.isnan(w)
+                den = torch.maximum(torch.maximum(v.abs(), w.abs()), torch.ones_like(v))
+                rel = torch.where(both_nan, torch.zeros_like(v), (v - w).abs() / den)
+                float_rel = max(float_rel, float(torch.nan_to_num(rel, nan=1.0).max().item()))
+            else:
+                int_mismatch += int((v != w).sum().item())
         written = int((res["tpri"] >= 0).sum().item())
         entry = {"opts": opts, "batch_ms": ms, "samples_written": written,
                  "samples_per_s": written / (ms / 1e3),
-                 "geodesics_with_mufill": int((res["nup_out"] > opts["nup"]).sum().item())}
+                 "geodesics_with_mufill": int((res["nup_out"] > opts["nup"]).sum().item()),
+                 "serial_batch_ms": ms_serial, "speedup_over_serial": ms_serial / ms,
+                 "vs_serial": {"integer_mismatches": int_mismatch, "float_max_rel_diff": float_rel}}
+        del ser
         if cpu_geodesics > 0:
             import os
             import time

@@ -375,8 +375,27 @@ typedef struct np_geokerr_samples {  /* device arrays */
     int32_t *tpm_out, *tpr_out, *ncase_out, *nup_out, *npts_out;   /* n each, or NULL */
 } np_geokerr_samples;
 
+/* np_geokerr_batch runs a geodesic's samples in parallel: one lane per
+ * geodesic runs the solves whose results later samples read (the first
+ * solve, the FIRSTPT sample, MUFILL's far-side solve, the last sample of
+ * each leg), then one lane per sample solves from its leg's starting state;
+ * a geodesic whose samples meet one of the driver's sticky states (GEOR
+ * leaving with NCASE = 0, GEOR's equal-root cubic branch, an input edit of
+ * U0 or UF) is rerun serially.  The results are those of the serial walk.
+ * USEGEOR samples after one where GEOR leaves its quartic case analysis
+ * (NCASE = 0, the Fortran's exit when the mu target lies beyond the u
+ * range) start again from the state that sample left.  It allocates a
+ * workspace of about 1.1 KB per geodesic (u-sampled) or 0.6 KB (USEGEOR),
+ * stream-ordered on `stream` where the device allows.
+ * NP_EALLOC when that fails. */
 int np_geokerr_batch(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
                      const np_geokerr_samples* out, void* stream);
+
+/* The same driver, one lane walking every sample of its geodesic in the
+ * Fortran's order; no workspace.  The form np_geokerr_batch is checked
+ * against, and its fallback for flagged geodesics. */
+int np_geokerr_batch_serial(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
+                            const np_geokerr_samples* out, void* stream);
 
 /* ---------------------------------------------------------------------
  * Schwarzschild equatorial sweeps (schwarzschild_blackhole.cu)

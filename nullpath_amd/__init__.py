@@ -260,7 +260,7 @@ def geor_batch(u0, mu0, muf, spin, L, Q2, su, sm, tpm, uf_in=None, stream=None):
     return uf, imu, tpr, ncase
 
 
-def geokerr_batch(opts, rays, stream=None):
+def geokerr_batch(opts, rays, stream=None, serial=False):
     """The whole GEOKERR driver (np_geokerr_batch) for n geodesics.
 
     opts: dict with nup, npts, kext, phit, usegeor, mufill, offset (the
@@ -269,7 +269,8 @@ def geokerr_batch(opts, rays, stream=None):
     beta, su, sm and int32 tpm, tpr.  Returns a dict of device tensors:
     ufi, mufi, dti, dphi, lambdai (float64) and tpmi, tpri (int32), each
     n x (nup + npts - 2 kext), and per geodesic u0_out, uf_out, muf_out,
-    tpm_out, tpr_out, ncase_out, nup_out, npts_out."""
+    tpm_out, tpr_out, ncase_out, nup_out, npts_out.  serial=True runs
+    np_geokerr_batch_serial (one lane walks each geodesic's samples)."""
     import torch
     from ._lib import GEOKERR_RAY_FIELDS, GeokerrOpts, GeokerrRays, GeokerrSamples
     o = GeokerrOpts(**{k: opts[k] for k in ("nup", "npts", "kext", "phit", "usegeor", "mufill", "offset")})
@@ -288,7 +289,9 @@ def geokerr_batch(opts, rays, stream=None):
                 for k in ("tpm_out", "tpr_out", "ncase_out", "nup_out", "npts_out")})
     r = GeokerrRays(**{k: rays[k].data_ptr() for k in GEOKERR_RAY_FIELDS})
     s = GeokerrSamples(**{k: v.data_ptr() for k, v in out.items()})
-    check(lib().np_geokerr_batch(C.byref(o), C.byref(r), n, C.byref(s), _stream_ptr(stream)), "np_geokerr_batch")
+    fn = lib().np_geokerr_batch_serial if serial else lib().np_geokerr_batch
+    check(fn(C.byref(o), C.byref(r), n, C.byref(s), _stream_ptr(stream)),
+          "np_geokerr_batch_serial" if serial else "np_geokerr_batch")
     return out
 
 

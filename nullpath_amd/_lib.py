@@ -150,6 +150,8 @@ def _declare(L: C.CDLL) -> None:
     L.np_geor_batch.restype = i32
     L.np_geokerr_batch.argtypes = [C.POINTER(GeokerrOpts), C.POINTER(GeokerrRays), i64, C.POINTER(GeokerrSamples), vp]
     L.np_geokerr_batch.restype = i32
+    L.np_geokerr_batch_serial.argtypes = L.np_geokerr_batch.argtypes
+    L.np_geokerr_batch_serial.restype = i32
     L.np_geomu_camera_batch.argtypes = [vp] * 5 + [d, i64] + [vp] * 5
     L.np_geomu_camera_batch.restype = i32
     L.np_geomu_camera_batch_mixed.argtypes = [vp] * 5 + [d, i64] + [vp] * 6

@@ -387,12 +387,31 @@ gk_phi_omega_kernel(const double* __restrict__ u0, const double* __restrict__ mu
 }
 
 // ------------------------ the whole GEOKERR driver ------------------------
-// One lane per geodesic running the Fortran's solves in its order
-// (geokerr_original.f:357-604), every by-reference argument and shared
-// variable carried as the Fortran carries it.  The u-sampled branch runs as
-// one loop over its steps (first leg, MUFILL's far-side solve, MUFILL
-// samples, the leg back from the turning point) so that GEOMU, GEOR and
-// each GEOPHITIME form are inlined once.
+// np_geokerr_batch runs the Fortran's solves (geokerr_original.f:357-604)
+// with every by-reference argument and shared variable carried as the
+// Fortran carries it.  The steps of one geodesic are written once, as
+// u_step (the u-sampled branch) and g_step (USEGEOR), and run in two forms:
+//
+// * serial (gk_full_u_kernel / gk_full_geor_kernel): one lane walks every
+//   step of its geodesic (np_geokerr_batch_serial);
+// * split (np_geokerr_batch): gk_fu_setup / gk_fg_setup, one lane per
+//   geodesic, run the steps whose results later steps read -- the first
+//   solve, the FIRSTPT sample, the last sample of the first leg, MUFILL's
+//   far-side solve, the last MUFILL sample and the last sample overall --
+//   and store the state each leg starts from; gk_fu_step / gk_fg_step then
+//   run every other step in a lane of its own from its leg's state.
+//
+// The split gives the serial results because a step with FIRSTPT off writes
+// only values that it, or its GEOPHITIME, recomputes before reading them:
+// each step of a leg reads the state the leg started from.  The exceptions
+// are the Fortran's sticky states, which a step detects and reports:
+//   - a solve that changes NCASE (GEOR leaving with NCASE = 0 when imu lies
+//     beyond the u range or muf in the forbidden mu band; later quartic
+//     solves then skip the case analysis);
+//   - GEOR's equal-root cubic branch, which reads the previous sample's UF;
+//   - the forbidden-region retry editing U0 or UF.
+// A geodesic with such a step is flagged, and gk_full_*_kernel<true> then
+// reruns the flagged geodesics serially over the same outputs.
 
 __device__ __forceinline__ void full_prefill(const np_geokerr_samples& s, size_t base, int total) {
     const double qnan = __builtin_nan("");
@@ -407,18 +426,34 @@ __device__ __forceinline__ void full_prefill(const np_geokerr_samples& s, size_t
     }
 }
 
-__device__ __forceinline__ void full_store(const np_geokerr_samples& s, size_t base, int total, int64_t idx1,
-                                           double u, double mu, double dt, double dphi, int tpm, int tpr,
-                                           double lam) {
-    const int64_t j = idx1 - 1;
+__device__ __forceinline__ void gap_store(const np_geokerr_samples& s, size_t at) {
+    const double qnan = __builtin_nan("");
+    s.ufi[at] = qnan;
+    s.mufi[at] = qnan;
+    s.dti[at] = qnan;
+    s.dphi[at] = qnan;
+    s.lambdai[at] = qnan;
+    s.tpmi[at] = -1;
+    s.tpri[at] = -1;
+}
+
+// One driver sample: its 1-based output index and values.
+struct GkSample {
+    int64_t idx;
+    double u, mu, dt, dphi, lam;
+    int32_t tpm, tpr;
+};
+
+__device__ __forceinline__ void full_store(const np_geokerr_samples& s, size_t base, int total, const GkSample& x) {
+    const int64_t j = x.idx - 1;
     if (j < 0 || j >= (int64_t)total) return;
-    s.ufi[base + j] = u;
-    s.mufi[base + j] = mu;
-    s.dti[base + j] = dt;
-    s.dphi[base + j] = dphi;
-    s.tpmi[base + j] = tpm;
-    s.tpri[base + j] = tpr;
-    s.lambdai[base + j] = lam;
+    s.ufi[base + j] = x.u;
+    s.mufi[base + j] = x.mu;
+    s.dti[base + j] = x.dt;
+    s.dphi[base + j] = x.dphi;
+    s.tpmi[base + j] = x.tpm;
+    s.tpri[base + j] = x.tpr;
+    s.lambdai[base + j] = x.lam;
 }
 
 __device__ __forceinline__ void full_finish(const np_geokerr_samples& s, int64_t g, double U0, double UF, double MUF,
@@ -433,143 +468,533 @@ __device__ __forceinline__ void full_finish(const np_geokerr_samples& s, int64_t
     if (s.npts_out) s.npts_out[g] = npts;
 }
 
-// USEGEOR: samples evenly in mu from MU0 to MUF (:421-467).  GEOR and
-// GEOMU always form their PHT pieces: they write only RF values that
-// GEOPHITIME alone reads, so PHIT = false changes no position.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-gk_full_geor_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s) {
-    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int total = o.nup + o.npts - 2 * o.kext;
-    const size_t base = (size_t)g * (size_t)total;
-    full_prefill(s, base, total);
-    GkVars v{};
-    const double mu0 = in.mu0[g], a = in.a[g], l = in.l[g], q2 = in.q2[g], su = in.su[g], sm = in.sm[g];
-    const double beta = in.beta ? in.beta[g] : 0.0;
-    double U0 = in.u0[g], UF = in.uf[g];
-    const double MUF = in.muf[g];
-    int TPM = in.tpm[g], TPR = in.tpr[g];
-    double IU = 0.0, dt = 0.0, dph = 0.0, lam = 0.0;
-    for (int k = 1; k <= o.nup; ++k) {
-        const double MUN = mu0 + ((double)k - o.offset) * (MUF - mu0) / (double)o.nup;
-        if (k >= 2) TPM = f_int((sgn1(mu0) * sm + 1.0) / 2.0);
-        if (mu0 != 0.0 || beta != 0.0) {
-            if (k == 1) geor_v<true, true>(v, U0, UF, mu0, MUN, a, l, q2, IU, TPM, TPR, su, sm);
-            else geor_v<false, true>(v, U0, UF, mu0, MUN, a, l, q2, IU, TPM, TPR, su, sm);
-            if (o.phit) {
-                const PhiTime p = geophitime<true>(v, U0, UF, mu0, MUN, a, l, q2, TPM, TPR, su, sm, IU);
-                dt = p.tmu + p.tu;
-                dph = p.phimu + p.phiu;
-                lam = p.lambda;
-            }
-        } else {
-            UF = U0;
-        }
-        full_store(s, base, total, k, UF, MUN, dt, dph, TPM, TPR, lam);
-    }
-    full_finish(s, g, U0, UF, MUF, TPM, TPR, v.ncase, o.nup, o.npts);
+// GEOR's equal-root cubic branch (u2 = u3 set exactly, rows 1/2): the only
+// GEOR branch that reads the UF it is given.
+__device__ __forceinline__ bool equal_root_cubic(const GkVars& v) {
+    return (v.ncase == 1 || v.ncase == 2) && v.u2 == v.u3;
 }
 
-// The u-sampled branch (:468-597) with the MUFILL switch (:539-575).
+// ---- USEGEOR: samples evenly in mu from MU0 to MUF (:421-467) ----
+// GEOR and GEOMU always form their PHT pieces: they write only RF values
+// that GEOPHITIME alone reads, so PHIT = false changes no position.
+
+struct GConst {
+    double mu0, a, l, q2, su, sm, beta, MUF;
+};
+
+struct GState {
+    GkVars v;
+    double U0, UF, IU, dt, dph, lam;
+    int32_t TPM, TPR;
+};
+
+struct FgRecord {
+    GConst c;
+    GState s1;   // after sample 1
+    GState se;   // after the first sample that left the quartic case analysis
+};
+
+__device__ __forceinline__ void g_init(const np_geokerr_rays& in, int64_t g, GConst& c, GState& st) {
+    c.mu0 = in.mu0[g]; c.a = in.a[g]; c.l = in.l[g]; c.q2 = in.q2[g];
+    c.su = in.su[g]; c.sm = in.sm[g];
+    c.beta = in.beta ? in.beta[g] : 0.0;
+    c.MUF = in.muf[g];
+    st.v = GkVars{};
+    st.U0 = in.u0[g]; st.UF = in.uf[g];
+    st.IU = 0.0; st.dt = 0.0; st.dph = 0.0; st.lam = 0.0;
+    st.TPM = in.tpm[g]; st.TPR = in.tpr[g];
+}
+
+// What a USEGEOR sample did to state later samples read.
+enum GEffect : int {
+    kNone = 0,      // nothing: later samples start from the same state
+    kExit = 1,      // GEOR left the quartic case analysis (NCASE = 0)
+    kChained = 2,   // the equal-root cubic branch, an edit of U0, a cubic case change
+};
+
+// Sample k (1-based).  FIRST: k may be 1 (GEOR with FIRSTPT on); without
+// it the caller guarantees k >= 2.  FP: GEOPHITIME with FIRSTPT on, as the
+// Fortran calls it in this branch; off, it reads the pieces sample 1 formed
+// (the u0 integrals of rows 1, 2, 7, 8 and the mu0 / turning-point pieces
+// of calcphitmu), which depend only on the geodesic and the case: the
+// caller guarantees sample 1 ran in the case this sample is in.  Returns
+// the sample's GEffect (k >= 2).
+template <bool FIRST, bool FP>
+__device__ __forceinline__ int g_step(const np_geokerr_opts& o, const GConst& c, GState& st, int k, GkSample& out) {
+    const double MUN = c.mu0 + ((double)k - o.offset) * (c.MUF - c.mu0) / (double)o.nup;
+    if (k >= 2) st.TPM = f_int((sgn1(c.mu0) * c.sm + 1.0) / 2.0);
+    int effect = kNone;
+    if (c.mu0 != 0.0 || c.beta != 0.0) {
+        const double u0_in = st.U0;
+        const int nc_in = st.v.ncase;
+        if (FIRST && k == 1)
+            geor_v<true, true>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su, c.sm);
+        else
+            geor_v<false, true>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su, c.sm);
+        if (k >= 2) {
+            const double ee = -c.a * c.a * c.q2;   // GEOR's test for the quartic case
+            if (st.U0 != u0_in || equal_root_cubic(st.v)) effect = kChained;
+            else if (st.v.ncase != nc_in) effect = (ee != 0.0 && st.v.ncase == 0) ? kExit : kChained;
+        }
+        if (o.phit) {
+            const PhiTime p = geophitime<FP>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.TPM, st.TPR, c.su,
+                                             c.sm, st.IU);
+            st.dt = p.tmu + p.tu;
+            st.dph = p.phimu + p.phiu;
+            st.lam = p.lambda;
+        }
+    } else {
+        st.UF = st.U0;
+    }
+    out = GkSample{k, st.UF, MUN, st.dt, st.dph, st.lam, st.TPM, st.TPR};
+    return effect;
+}
+
+__device__ __forceinline__ void g_finish(const np_geokerr_opts& o, const np_geokerr_samples& s, int64_t g,
+                                         const GConst& c, const GState& st) {
+    full_finish(s, g, st.U0, st.UF, c.MUF, st.TPM, st.TPR, st.v.ncase, o.nup, o.npts);
+}
+
+// Serial walk; ONLY_FLAGGED: the geodesics the split form flagged.
+template <bool ONLY_FLAGGED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-gk_full_u_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s) {
+gk_full_geor_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s,
+                    const int32_t* __restrict__ flags) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    if (ONLY_FLAGGED && !flags[g]) return;
+    const int total = o.nup + o.npts - 2 * o.kext;
+    const size_t base = (size_t)g * (size_t)total;
+    full_prefill(s, base, total);
+    GConst c;
+    GState st;
+    g_init(in, g, c, st);
+    GkSample x;
+    for (int k = 1; k <= o.nup; ++k) {
+        (void)g_step<true, true>(o, c, st, k, x);
+        full_store(s, base, total, x);
+    }
+    g_finish(o, s, g, c, st);
+}
+
+// Per geodesic: sample 1 (FIRSTPT on) and the state after it; the final
+// state from sample NUP run from there (right unless an earlier sample
+// changes the state, which the later kernels then handle).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_fg_setup_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s, FgRecord* __restrict__ rec,
+                   int32_t* __restrict__ chained, int32_t* __restrict__ first_exit) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int total = o.nup + o.npts - 2 * o.kext;
     const size_t base = (size_t)g * (size_t)total;
-    full_prefill(s, base, total);
-    GkVars v{};
-    const double uout = in.uout[g], mu0 = in.mu0[g], a = in.a[g], l = in.l[g], q2 = in.q2[g];
-    const double su = in.su[g], sm = in.sm[g], offset = o.offset;
-    const int nup = o.nup, kext = o.kext;
-    double U0 = in.u0[g], UF = in.uf[g], MUF = in.muf[g];
-    int TPM = in.tpm[g], TPR = in.tpr[g];
-    const double uplus = 1.0 / (1.0 + sqrt(1.0 - a * a));
-    double IU = 0.0, dt = 0.0, dph = 0.0, lam = 0.0;
+    GConst c;
+    GState st;
+    g_init(in, g, c, st);
+    GkSample x;
+    (void)g_step<true, true>(o, c, st, 1, x);
+    full_store(s, base, total, x);
+    rec[g].c = c;
+    rec[g].s1 = st;
+    if (o.nup >= 2) (void)g_step<false, false>(o, c, st, o.nup, x);   // its lane stores the sample
+    g_finish(o, s, g, c, st);
+    chained[g] = 0;
+    first_exit[g] = INT32_MAX;
+}
 
-    // first solve on (U0, UF): case, roots, cached u0 / mu0 pieces
-    geomu<true, true>(v, U0, UF, false, mu0, a, l, q2, TPM, TPR, su, sm, MUF, IU);
-    const int nc = v.ncase;
-    if ((nc > 2 && nc < 7) || (nc == 1 && su < 0.0) || (nc == 7 && su < 0.0) || (nc == 2 && su > 0.0) ||
-        (nc == 8 && su > 0.0))
-        TPR = 0;
+// Lane (g, slot): samples k = slot + 1, slot + 1 + width, ... (k >= 2) of
+// geodesic g from the state after sample 1, and the never-written output
+// slots beyond NUP.  A sample that leaves the quartic case analysis records
+// itself in first_exit[g] (the lowest wins); a chained state marks the
+// geodesic for the serial walk.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_fg_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samples s,
+                  const FgRecord* __restrict__ rec, int32_t* __restrict__ chained, int32_t* __restrict__ first_exit) {
+    const int total = o.nup + o.npts - 2 * o.kext;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * width;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = i / width;
+        const int64_t slot = i - g * width;
+        const size_t base = (size_t)g * (size_t)total;
+        for (int64_t j = slot; j < total; j += width)
+            if (j >= o.nup) gap_store(s, base + (size_t)j);
+        if (slot == 0) continue;
+        const GConst c = rec[g].c;
+        for (int64_t k = slot + 1; k <= o.nup; k += width) {
+            GState st = rec[g].s1;
+            GkSample x;
+            const int effect = g_step<false, false>(o, c, st, (int)k, x);
+            full_store(s, base, total, x);
+            if (effect == kExit) atomicMin(&first_exit[g], (int)k);
+            else if (effect == kChained) chained[g] = 1;
+        }
+    }
+}
+
+// Per geodesic whose samples left the quartic case analysis: the state
+// after the first such sample e (from the state after sample 1, as its lane
+// ran it), and the final state from there.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_fg_exit_kernel(np_geokerr_opts o, int64_t n, np_geokerr_samples s, FgRecord* __restrict__ rec,
+                  const int32_t* __restrict__ chained, const int32_t* __restrict__ first_exit) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int e = first_exit[g];
+    if (e > o.nup || chained[g]) return;
+    const GConst c = rec[g].c;
+    GState st = rec[g].s1;
+    GkSample x;
+    (void)g_step<false, false>(o, c, st, e, x);
+    rec[g].se = st;
+    if (e < o.nup) (void)g_step<false, false>(o, c, st, o.nup, x);
+    g_finish(o, s, g, c, st);
+}
+
+// Lane (g, slot): the samples after the first exit e, from the state after e.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_fg_restep_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samples s,
+                    const FgRecord* __restrict__ rec, int32_t* __restrict__ chained,
+                    const int32_t* __restrict__ first_exit) {
+    const int total = o.nup + o.npts - 2 * o.kext;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * width;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = i / width;
+        const int64_t slot = i - g * width;
+        const int e = first_exit[g];
+        if (e > o.nup || chained[g]) continue;
+        const size_t base = (size_t)g * (size_t)total;
+        const GConst c = rec[g].c;
+        for (int64_t k = slot + 1; k <= o.nup; k += width) {
+            if (k <= e) continue;
+            GState st = rec[g].se;
+            GkSample x;
+            const int effect = g_step<false, false>(o, c, st, (int)k, x);
+            full_store(s, base, total, x);
+            if (effect != kNone) chained[g] = 1;
+        }
+    }
+}
+
+// ---- the u-sampled branch (:468-597) with the MUFILL switch (:539-575) ----
+
+// Per geodesic, fixed by the first solve.
+struct UConst {
+    double uout, mu0, a, l, q2, su, sm, ub, du;
+    int64_t nfirst, nfill, nret, kfirst, nsteps;
+    int32_t kmax, npts2, go, back, fill, pad;
+};
+
+// The serial loop's variables.
+struct UState {
+    GkVars v;
+    double U0, UF, MUF, UN, MUN, IU, dt, dph, lam, muminus, muplus, mustart;
+    int32_t TPM, TPR, TPR1, TPM1, tpmstart, pad;
+};
+
+struct FuRecord {
+    UConst c;
+    UState s0;   // after the first leg's first sample (the FIRSTPT GEOPHITIME)
+    UState s1;   // after MUFILL's far-side solve
+    UState s2;   // where the leg back starts: after the last MUFILL sample, or the last first-leg sample
+};
+
+// The first solve on (U0, UF) -- case, roots, cached u0 / mu0 pieces -- and
+// the driver's sample counts (:468-537).
+__device__ __forceinline__ void u_first_solve(const np_geokerr_opts& o, const np_geokerr_rays& in, int64_t g,
+                                              UConst& c, UState& st) {
+    c = UConst{};
+    st = UState{};
+    c.uout = in.uout[g]; c.mu0 = in.mu0[g]; c.a = in.a[g]; c.l = in.l[g]; c.q2 = in.q2[g];
+    c.su = in.su[g]; c.sm = in.sm[g];
+    st.U0 = in.u0[g]; st.UF = in.uf[g]; st.MUF = in.muf[g];
+    st.TPM = in.tpm[g]; st.TPR = in.tpr[g];
+    const int nup = o.nup, kext = o.kext;
+    const double uplus = 1.0 / (1.0 + sqrt(1.0 - c.a * c.a));
+    geomu<true, true>(st.v, st.U0, st.UF, false, c.mu0, c.a, c.l, c.q2, st.TPM, st.TPR, c.su, c.sm, st.MUF, st.IU);
+    const int nc = st.v.ncase;
+    if ((nc > 2 && nc < 7) || (nc == 1 && c.su < 0.0) || (nc == 7 && c.su < 0.0) || (nc == 2 && c.su > 0.0) ||
+        (nc == 8 && c.su > 0.0))
+        st.TPR = 0;
     double ub;
     if (nc > 2 && nc < 7) {
-        ub = (su > 0.0) ? uplus : 0.0;
+        ub = (c.su > 0.0) ? uplus : 0.0;
     } else if (nc == 1 || nc == 7) {
-        ub = v.u2;
-        if (TPR == 1 && UF == ub) UF = uout;
+        ub = st.v.u2;
+        if (st.TPR == 1 && st.UF == ub) st.UF = c.uout;
     } else {
-        ub = v.u3;
-        if (TPR == 1 && UF == ub) UF = uout;
+        ub = st.v.u3;
+        if (st.TPR == 1 && st.UF == ub) st.UF = c.uout;
     }
-    double du = sgn1(ub - uout) * su * ((ub - uout) + (2.0 * TPR - 1.0) * (ub - UF));
+    double du = sgn1(ub - c.uout) * c.su * ((ub - c.uout) + (2.0 * st.TPR - 1.0) * (ub - st.UF));
     du = du / (double)nup;
-    int kmax = f_int((ub - uout) / du + offset);
+    int kmax = f_int((ub - c.uout) / du + o.offset);
     if (kmax > nup) kmax = nup;
     if (du == 0.0) kmax = 0;
-    if (!(sgn1(uout - ub) != sgn1(uout - U0) || uout == U0)) {
-        full_finish(s, g, U0, UF, MUF, TPM, TPR, v.ncase, 0, o.npts);
+    c.ub = ub;
+    c.du = du;
+    c.kmax = kmax;
+    c.go = (sgn1(c.uout - ub) != sgn1(c.uout - st.U0) || c.uout == st.U0) ? 1 : 0;
+    if (!c.go) return;
+    c.back = (st.TPR == 1) ? 1 : 0;
+    c.fill = (c.back && o.mufill) ? 1 : 0;
+    c.npts2 = c.fill ? o.npts - 2 * kext : o.npts;
+    c.nfirst = (kmax == 0) ? 0 : (kmax > 1 ? kmax : 1);
+    c.kfirst = (int64_t)kmax + 1 + kext;
+    if (c.kfirst < 1 - (int64_t)c.npts2) c.kfirst = 1 - (int64_t)c.npts2;
+    c.nret = c.back ? ((int64_t)nup - c.kfirst + 1 > 0 ? (int64_t)nup - c.kfirst + 1 : 0) : 0;
+    c.nfill = c.fill ? 1 + (int64_t)o.npts : 0;
+    c.nsteps = c.nfirst + c.nfill + c.nret;
+    st.TPR1 = 0;
+    st.TPM1 = st.TPM;
+    st.tpmstart = 0;
+    st.mustart = c.mu0;
+}
+
+// Step `it` of the u-sampled loop.  Phase 0: the first leg; 1: MUFILL's
+// far-side solve; 2: MUFILL samples; 3: the leg back from the turning
+// point.  Returns the phase; for phases 0, 2 and 3 `out` holds the sample.
+// The far-side solve reads MUFI/TPMI(KMAX-KEXT) as they stand at that
+// point: jb_in says whether that index lies inside the arrays, jb_mu and
+// jb_tpm are its values.  ALLOW_FP: the step may be one with FIRSTPT on in
+// its GEOPHITIME (the first sample, or the leg back when NUP = 1).  KIND:
+// kUStep or kMuStep when the caller guarantees a GEOMU step (phases 0, 1,
+// 3) or a GEOR step (phase 2), so that only that solver is inlined.
+enum UStepKind : int { kAnyStep = 0, kUStep = 1, kMuStep = 2 };
+
+template <bool ALLOW_FP, int KIND = kAnyStep>
+__device__ __forceinline__ int u_step(const np_geokerr_opts& o, const UConst& c, UState& st, int64_t it, bool jb_in,
+                                      double jb_mu, int jb_tpm, GkSample& out, bool& sticky) {
+    int phase;
+    int64_t k;
+    if (it < c.nfirst) { phase = 0; k = it + 1; }
+    else if (it < c.nfirst + c.nfill) { k = it - c.nfirst; phase = (k == 0) ? 1 : 2; }
+    else { phase = 3; k = c.kfirst + (it - c.nfirst - c.nfill); }
+    const int nc_in = st.v.ncase;
+    bool mu_step;
+    if constexpr (KIND == kMuStep) mu_step = true;
+    else if constexpr (KIND == kUStep) mu_step = false;
+    else mu_step = (phase == 2);
+    if (mu_step) {
+        const double u0_in = st.U0;
+        indep_muf(st.muminus, st.muplus, st.mustart, st.MUF, st.tpmstart, st.TPM1, c.sm, (int)k, o.npts, o.offset,
+                  st.MUN, st.TPM);
+        geor_v<false, true>(st.v, st.U0, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su, c.sm);
+        if (st.U0 != u0_in || equal_root_cubic(st.v)) sticky = true;
+    } else {
+        const double kk = (phase == 1) ? (double)((int64_t)c.kmax + o.kext + 1) : (double)k;
+        st.UN = (phase == 0) ? c.uout + (kk - o.offset) * c.du : 2.0 * c.ub - (c.uout + (kk - o.offset) * c.du);
+        double us = (phase == 0) ? st.U0 : st.UF;
+        const double us_in = us;
+        int tpmv = (phase == 1) ? st.TPM1 : st.TPM;
+        int tprv = (phase == 0) ? st.TPR1 : st.TPR;
+        geomu<false, true>(st.v, us, st.UN, false, c.mu0, c.a, c.l, c.q2, tpmv, tprv, c.su, c.sm, st.MUN, st.IU);
+        if (us != us_in) sticky = true;
+        if (phase == 0) st.U0 = us; else st.UF = us;
+        if (phase == 1) st.TPM1 = tpmv; else st.TPM = tpmv;
+        if (phase == 0) st.TPR1 = tprv; else st.TPR = tprv;
+    }
+    if (st.v.ncase != nc_in) sticky = true;
+    if (phase == 1) {
+        st.MUF = st.MUN;
+        findmroots(c.q2, c.l, c.a, c.mu0, st.muminus, st.muplus);
+        if (jb_in) {
+            st.mustart = jb_mu;
+            st.tpmstart = jb_tpm;
+        }
+        return phase;
+    }
+    if (o.phit) {
+        const bool fp = ALLOW_FP && ((phase == 0 && k == 1) || (phase == 3 && o.nup == 1));
+        const double us = (phase == 3) ? st.UF : st.U0;
+        const int tprv = (phase == 0) ? st.TPR1 : st.TPR;
+        PhiTime p;
+        if (ALLOW_FP && fp)
+            p = geophitime<true>(st.v, us, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.TPM, tprv, c.su, c.sm, st.IU);
+        else
+            p = geophitime<false>(st.v, us, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.TPM, tprv, c.su, c.sm, st.IU);
+        st.dt = p.tmu + p.tu;
+        st.dph = p.phimu + p.phiu;
+        st.lam = p.lambda;
+    }
+    out.idx = (phase == 0) ? k : (phase == 2) ? k + c.kmax - o.kext : k + c.npts2;
+    out.u = st.UN;
+    out.mu = st.MUN;
+    out.dt = st.dt;
+    out.dphi = st.dph;
+    out.lam = st.lam;
+    out.tpm = st.TPM;
+    out.tpr = (phase == 0) ? 0 : (phase == 2) ? st.TPR : 1;
+    return phase;
+}
+
+__device__ __forceinline__ void u_finish(const np_geokerr_opts& o, const np_geokerr_samples& s, int64_t g,
+                                         const UConst& c, const UState& st) {
+    if (!c.go) full_finish(s, g, st.U0, st.UF, st.MUF, st.TPM, st.TPR, st.v.ncase, 0, o.npts);
+    else full_finish(s, g, st.U0, st.UF, st.MUF, st.TPM, st.TPR, st.v.ncase, c.back ? o.nup + c.npts2 : o.nup,
+                     c.npts2);
+}
+
+// Output index ranges of the MUFILL samples and of the leg back.
+__device__ __forceinline__ bool in_fill_range(const np_geokerr_opts& o, const UConst& c, int64_t idx) {
+    return c.fill && o.npts >= 1 && idx >= 1 + (int64_t)c.kmax - o.kext &&
+           idx <= (int64_t)o.npts + c.kmax - o.kext;
+}
+__device__ __forceinline__ bool in_back_range(const np_geokerr_opts& o, const UConst& c, int64_t idx) {
+    return c.nret >= 1 && idx >= c.kfirst + c.npts2 && idx <= (int64_t)o.nup + c.npts2;
+}
+
+// Whether a sample of `phase` at idx is the last write there in the serial
+// order (later phases overwrite earlier ones; within a phase the indices
+// differ).
+__device__ __forceinline__ bool u_last_writer(const np_geokerr_opts& o, const UConst& c, int phase, int64_t idx) {
+    if (phase == 3) return true;
+    if (in_back_range(o, c, idx)) return false;
+    return phase == 2 || !in_fill_range(o, c, idx);
+}
+
+// Output slot idx (1-based) written by no step.
+__device__ __forceinline__ bool u_gap(const np_geokerr_opts& o, const UConst& c, int64_t idx) {
+    if (!c.go) return true;
+    return !(idx <= c.nfirst || in_fill_range(o, c, idx) || in_back_range(o, c, idx));
+}
+
+// Serial walk; ONLY_FLAGGED: the geodesics the split form flagged.
+template <bool ONLY_FLAGGED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_full_u_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s,
+                 const int32_t* __restrict__ flags) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    if (ONLY_FLAGGED && !flags[g]) return;
+    const int total = o.nup + o.npts - 2 * o.kext;
+    const size_t base = (size_t)g * (size_t)total;
+    full_prefill(s, base, total);
+    UConst c;
+    UState st;
+    u_first_solve(o, in, g, c, st);
+    bool sticky = false;
+    GkSample x;
+    for (int64_t it = 0; c.go && it < c.nsteps; ++it) {
+        // the far-side solve reads MUFI/TPMI(KMAX-KEXT) as the loop left them
+        const int64_t jb = (int64_t)c.kmax - o.kext;
+        const bool jb_in = (it == c.nfirst && c.fill) && jb >= 1 && jb <= (int64_t)total;
+        const double jb_mu = jb_in ? s.mufi[base + (size_t)(jb - 1)] : 0.0;
+        const int jb_tpm = jb_in ? s.tpmi[base + (size_t)(jb - 1)] : 0;
+        if (u_step<true>(o, c, st, it, jb_in, jb_mu, jb_tpm, x, sticky) != 1) full_store(s, base, total, x);
+    }
+    u_finish(o, s, g, c, st);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_fu_setup_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_samples s, FuRecord* __restrict__ rec,
+                   int32_t* __restrict__ flags) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int total = o.nup + o.npts - 2 * o.kext;
+    const size_t base = (size_t)g * (size_t)total;
+    UConst c;
+    UState st;
+    u_first_solve(o, in, g, c, st);
+    bool sticky = false;
+    GkSample x;
+    rec[g].c = c;
+    if (!c.go) {
+        u_finish(o, s, g, c, st);
+        flags[g] = 0;
         return;
     }
-    const bool back = (TPR == 1);
-    const bool fill = back && o.mufill;
-    const int npts2 = fill ? o.npts - 2 * kext : o.npts;
-    const int64_t nfirst = (kmax == 0) ? 0 : (kmax > 1 ? kmax : 1);
-    int64_t kfirst = (int64_t)kmax + 1 + kext;
-    if (kfirst < 1 - (int64_t)npts2) kfirst = 1 - (int64_t)npts2;
-    const int64_t nret = back ? ((int64_t)nup - kfirst + 1 > 0 ? (int64_t)nup - kfirst + 1 : 0) : 0;
-    const int64_t nfill = fill ? 1 + (int64_t)o.npts : 0;
-    const int64_t nsteps = nfirst + nfill + nret;
-
-    int TPR1 = 0, TPM1 = TPM, tpmstart = 0;
-    double UN = 0.0, MUN = 0.0, muminus = 0.0, muplus = 0.0, mustart = mu0;
-    for (int64_t it = 0; it < nsteps; ++it) {
-        // phase 0: first leg; 1: MUFILL far side; 2: MUFILL samples; 3: back
-        int phase;
-        int64_t k;
-        if (it < nfirst) { phase = 0; k = it + 1; }
-        else if (it < nfirst + nfill) { k = it - nfirst; phase = (k == 0) ? 1 : 2; }
-        else { phase = 3; k = kfirst + (it - nfirst - nfill); }
-        if (phase == 2) {
-            indep_muf(muminus, muplus, mustart, MUF, tpmstart, TPM1, sm, (int)k, o.npts, offset, MUN, TPM);
-            geor_v<false, true>(v, U0, UN, mu0, MUN, a, l, q2, IU, TPM, TPR, su, sm);
-        } else {
-            const double kk = (phase == 1) ? (double)((int64_t)kmax + kext + 1) : (double)k;
-            UN = (phase == 0) ? uout + (kk - offset) * du : 2.0 * ub - (uout + (kk - offset) * du);
-            double us = (phase == 0) ? U0 : UF;
-            int tpmv = (phase == 1) ? TPM1 : TPM;
-            int tprv = (phase == 0) ? TPR1 : TPR;
-            geomu<false, true>(v, us, UN, false, mu0, a, l, q2, tpmv, tprv, su, sm, MUN, IU);
-            if (phase == 0) U0 = us; else UF = us;
-            if (phase == 1) TPM1 = tpmv; else TPM = tpmv;
-            if (phase == 0) TPR1 = tprv; else TPR = tprv;
-        }
-        if (phase == 1) {
-            MUF = MUN;
-            findmroots(q2, l, a, mu0, muminus, muplus);
-            const int64_t jb = (int64_t)kmax - kext;   // the Fortran reads MUFI(KMAX-KEXT)
-            if (jb >= 1 && jb <= (int64_t)total) {
-                mustart = s.mufi[base + (size_t)(jb - 1)];
-                tpmstart = s.tpmi[base + (size_t)(jb - 1)];
-            }
-            continue;
-        }
-        if (o.phit) {
-            const bool fp = (phase == 0 && k == 1) || (phase == 3 && nup == 1);
-            const double us = (phase == 3) ? UF : U0;
-            const int tprv = (phase == 0) ? TPR1 : TPR;
-            const PhiTime p = fp ? geophitime<true>(v, us, UN, mu0, MUN, a, l, q2, TPM, tprv, su, sm, IU)
-                                 : geophitime<false>(v, us, UN, mu0, MUN, a, l, q2, TPM, tprv, su, sm, IU);
-            dt = p.tmu + p.tu;
-            dph = p.phimu + p.phiu;
-            lam = p.lambda;
-        }
-        const int64_t idx = (phase == 0) ? k : (phase == 2) ? k + kmax - kext : k + npts2;
-        const int tpri = (phase == 0) ? 0 : (phase == 2) ? TPR : 1;
-        full_store(s, base, total, idx, UN, MUN, dt, dph, TPM, tpri, lam);
+    // the first leg: sample 1 here (FIRSTPT), samples 2.. in their lanes
+    double mu_1 = 0.0;
+    int tpm_1 = 0;
+    if (c.nfirst >= 1) {
+        u_step<true>(o, c, st, 0, false, 0.0, 0, x, sticky);
+        if (u_last_writer(o, c, 0, x.idx)) full_store(s, base, total, x);
+        mu_1 = x.mu;
+        tpm_1 = x.tpm;
     }
-    full_finish(s, g, U0, UF, MUF, TPM, TPR, v.ncase, back ? nup + npts2 : nup, npts2);
+    rec[g].s0 = st;
+    UState last = st;   // after the first leg's last sample
+    if (c.nfirst >= 2) u_step<false>(o, c, last, c.nfirst - 1, false, 0.0, 0, x, sticky);
+    UState back = last;
+    if (c.fill) {
+        // MUFI/TPMI(KMAX-KEXT) when the far-side solve runs: written by the
+        // first leg's sample KMAX-KEXT, else still the prefilled NaN / -1
+        const int64_t jb = (int64_t)c.kmax - o.kext;
+        const bool jb_in = jb >= 1 && jb <= (int64_t)total;
+        double jb_mu = __builtin_nan("");
+        int jb_tpm = -1;
+        if (jb_in && jb <= c.nfirst) {
+            if (jb == 1) {
+                jb_mu = mu_1;
+                jb_tpm = tpm_1;
+            } else {
+                UState t = rec[g].s0;
+                u_step<false>(o, c, t, jb - 1, false, 0.0, 0, x, sticky);
+                jb_mu = x.mu;
+                jb_tpm = x.tpm;
+            }
+        }
+        UState far = last;
+        u_step<true>(o, c, far, c.nfirst, jb_in, jb_mu, jb_tpm, x, sticky);
+        rec[g].s1 = far;
+        back = far;
+        if (o.npts >= 1) u_step<false>(o, c, back, c.nfirst + o.npts, false, 0.0, 0, x, sticky);
+    }
+    rec[g].s2 = back;
+    UState fin = back;
+    if (c.nret >= 1 && o.nup == 1) {
+        // NUP = 1: every sample of the leg back has FIRSTPT on in its
+        // GEOPHITIME, so the leg runs here, in order
+        for (int64_t it = c.nfirst + c.nfill; it < c.nsteps; ++it) {
+            u_step<true>(o, c, fin, it, false, 0.0, 0, x, sticky);
+            full_store(s, base, total, x);
+        }
+    } else if (c.nret >= 1) {
+        // the last step overall, for the final state (its lane stores it)
+        u_step<true>(o, c, fin, c.nsteps - 1, false, 0.0, 0, x, sticky);
+    }
+    u_finish(o, s, g, c, fin);
+    flags[g] = sticky ? 1 : 0;
+}
+
+// MU = false: lane (g, slot) runs steps slot, slot + width, ... of geodesic
+// g on the first leg and the leg back (GEOMU) that the setup did not run,
+// each from the state its leg started from, and writes the output slots no
+// step writes (width >= the row length).  MU = true: lane (g, slot) runs
+// MUFILL samples slot + 1, slot + 1 + width, ... (GEOR) from the state
+// after the far-side solve.
+template <bool MU>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gk_fu_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samples s,
+                  const FuRecord* __restrict__ rec, int32_t* __restrict__ flags) {
+    const int total = o.nup + o.npts - 2 * o.kext;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * width;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = i / width;
+        const int64_t slot = i - g * width;
+        const size_t base = (size_t)g * (size_t)total;
+        const UConst c = rec[g].c;
+        bool sticky = false;
+        if (MU) {
+            for (int64_t k = slot + 1; c.go && c.fill && k <= o.npts; k += width) {
+                UState st = rec[g].s1;
+                GkSample x;
+                u_step<false, kMuStep>(o, c, st, c.nfirst + k, false, 0.0, 0, x, sticky);
+                if (u_last_writer(o, c, 2, x.idx)) full_store(s, base, total, x);
+            }
+        } else {
+            for (int64_t j = slot; j < total; j += width)
+                if (u_gap(o, c, j + 1)) gap_store(s, base + (size_t)j);
+            for (int64_t it = slot; c.go && it < c.nsteps; it += width) {
+                const int phase = (it < c.nfirst) ? 0 : (it < c.nfirst + c.nfill) ? ((it == c.nfirst) ? 1 : 2) : 3;
+                if (phase == 1 || phase == 2 || (phase == 0 && it == 0) || (phase == 3 && o.nup == 1)) continue;
+                UState st = (phase == 0) ? rec[g].s0 : rec[g].s2;
+                GkSample x;
+                u_step<false, kUStep>(o, c, st, it, false, 0.0, 0, x, sticky);
+                if (u_last_writer(o, c, phase, x.idx)) full_store(s, base, total, x);
+            }
+        }
+        if (sticky) flags[g] = 1;
+    }
 }
 
 int cus() {
@@ -667,27 +1092,114 @@ int np_geokerr_phi_omega_batch(const double* d_u0, const double* d_mu0, const do
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 
-int np_geokerr_batch(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
-                     const np_geokerr_samples* out, void* stream) {
-    using namespace np::gk;
+namespace {
+
+// Argument checks shared by both forms of the driver.
+int driver_args(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n, const np_geokerr_samples* out) {
     if (!opts || !rays || !out || n < 0) return NP_EINVAL;
     const np_geokerr_opts o = *opts;
     if (o.nup < 1 || o.npts < 0 || o.kext < 0) return NP_EINVAL;
     const int64_t total = (int64_t)o.nup + o.npts - 2 * (int64_t)o.kext;
     if (total < 1 || total > INT32_MAX) return NP_EINVAL;
     if (n == 0) return NP_OK;
-    const np_geokerr_rays r = *rays;
-    const np_geokerr_samples s = *out;
+    const np_geokerr_rays& r = *rays;
+    const np_geokerr_samples& s = *out;
     if (!r.u0 || !r.uf || !r.uout || !r.mu0 || !r.muf || !r.a || !r.l || !r.q2 || !r.su || !r.sm || !r.tpm ||
         !r.tpr)
         return NP_EINVAL;
     if (!s.ufi || !s.mufi || !s.dti || !s.dphi || !s.lambdai || !s.tpmi || !s.tpri) return NP_EINVAL;
-    if (n > INT64_MAX / total || (n + 255) / 256 > INT32_MAX) return NP_EINVAL;
+    // the split form's lanes: n x (NUP + NPTS + 2)
+    const int64_t width = (int64_t)o.nup + o.npts + 2;
+    if (n > INT64_MAX / width || (n + 255) / 256 > INT32_MAX) return NP_EINVAL;
+    return NP_OK;
+}
+
+void launch_serial(const np_geokerr_opts& o, const np_geokerr_rays& r, int64_t n, const np_geokerr_samples& s,
+                   hipStream_t st) {
+    using namespace np::gk;
     const dim3 grid((unsigned)((n + 255) / 256));
-    if (o.usegeor)
-        hipLaunchKernelGGL(gk_full_geor_kernel, grid, dim3(256), 0, (hipStream_t)stream, o, r, n, s);
-    else
-        hipLaunchKernelGGL(gk_full_u_kernel, grid, dim3(256), 0, (hipStream_t)stream, o, r, n, s);
+    const int32_t* none = nullptr;
+    if (o.usegeor) hipLaunchKernelGGL(gk_full_geor_kernel<false>, grid, dim3(256), 0, st, o, r, n, s, none);
+    else hipLaunchKernelGGL(gk_full_u_kernel<false>, grid, dim3(256), 0, st, o, r, n, s, none);
+}
+
+// Workspace of the split form: per geodesic two flags, then its record.
+size_t split_workspace_bytes(const np_geokerr_opts& o, int64_t n) {
+    using namespace np::gk;
+    const size_t rec = o.usegeor ? sizeof(FgRecord) : sizeof(FuRecord);
+    return 2 * align_up(sizeof(int32_t) * (size_t)n) + rec * (size_t)n;
+}
+
+// The split form: setup (one lane per geodesic), steps (one lane per
+// geodesic and step slot), for USEGEOR the restart after a sample that left
+// the quartic case analysis, then the serial walk over flagged geodesics.
+int launch_split(const np_geokerr_opts& o, const np_geokerr_rays& r, int64_t n, const np_geokerr_samples& s,
+                 void* ws, hipStream_t st) {
+    using namespace np::gk;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    const int64_t total = (int64_t)o.nup + o.npts - 2 * (int64_t)o.kext;
+    int32_t* flags = (int32_t*)ws;
+    int32_t* first_exit = (int32_t*)((char*)ws + align_up(sizeof(int32_t) * (size_t)n));
+    void* rec = (char*)ws + 2 * align_up(sizeof(int32_t) * (size_t)n);
+    if (o.usegeor) {
+        const int64_t width = total > o.nup ? total : (int64_t)o.nup;
+        const dim3 lanes(blocks_for(n * width, 256, 32));
+        FgRecord* R = (FgRecord*)rec;
+        hipLaunchKernelGGL(gk_fg_setup_kernel, grid, dim3(256), 0, st, o, r, n, s, R, flags, first_exit);
+        hipLaunchKernelGGL(gk_fg_step_kernel, lanes, dim3(256), 0, st, o, n, width, s, (const FgRecord*)R, flags,
+                           first_exit);
+        hipLaunchKernelGGL(gk_fg_exit_kernel, grid, dim3(256), 0, st, o, n, s, R, (const int32_t*)flags,
+                           (const int32_t*)first_exit);
+        hipLaunchKernelGGL(gk_fg_restep_kernel, lanes, dim3(256), 0, st, o, n, width, s, (const FgRecord*)R, flags,
+                           (const int32_t*)first_exit);
+        hipLaunchKernelGGL(gk_full_geor_kernel<true>, grid, dim3(256), 0, st, o, r, n, s, (const int32_t*)flags);
+    } else {
+        const int64_t width = (int64_t)o.nup + o.npts + 2;
+        FuRecord* R = (FuRecord*)rec;
+        hipLaunchKernelGGL(gk_fu_setup_kernel, grid, dim3(256), 0, st, o, r, n, s, R, flags);
+        hipLaunchKernelGGL(gk_fu_step_kernel<false>, dim3(blocks_for(n * width, 256, 32)), dim3(256), 0, st, o, n,
+                           width, s, (const FuRecord*)R, flags);
+        if (o.mufill && o.npts > 0)
+            hipLaunchKernelGGL(gk_fu_step_kernel<true>, dim3(blocks_for(n * o.npts, 256, 32)), dim3(256), 0, st, o,
+                               n, (int64_t)o.npts, s, (const FuRecord*)R, flags);
+        hipLaunchKernelGGL(gk_full_u_kernel<true>, grid, dim3(256), 0, st, o, r, n, s, (const int32_t*)flags);
+    }
+    return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
+}
+
+}  // namespace
+
+int np_geokerr_batch(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
+                     const np_geokerr_samples* out, void* stream) {
+    using namespace np::gk;
+    int err = driver_args(opts, rays, n, out);
+    if (err != NP_OK || n == 0) return err;
+    const np_geokerr_opts o = *opts;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t bytes = split_workspace_bytes(o, n);
+    void* ws = nullptr;
+    bool stream_ordered = true;
+    if (hipMallocAsync(&ws, bytes, st) != hipSuccess) {
+        (void)hipGetLastError();
+        stream_ordered = false;
+        if (hipMalloc(&ws, bytes) != hipSuccess) return NP_EALLOC;
+    }
+    err = launch_split(o, *rays, n, *out, ws, st);
+    if (stream_ordered) {
+        if (hipFreeAsync(ws, st) != hipSuccess && err == NP_OK) err = NP_ELAUNCH;
+    } else {
+        // without stream-ordered allocation the records must outlive the kernels
+        if (hipStreamSynchronize(st) != hipSuccess && err == NP_OK) err = NP_ELAUNCH;
+        (void)hipFree(ws);
+    }
+    return err;
+}
+
+int np_geokerr_batch_serial(const np_geokerr_opts* opts, const np_geokerr_rays* rays, int64_t n,
+                            const np_geokerr_samples* out, void* stream) {
+    int err = driver_args(opts, rays, n, out);
+    if (err != NP_OK || n == 0) return err;
+    launch_serial(*opts, *rays, n, *out, (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;
 }
 

@@ -1587,10 +1587,16 @@ __device__ inline double tfnkerr(double u0, double uf, double u1, double u2, dou
 
 // calcphitmusym / calcphitmuasym with ellphitmu (geokerr_original.f:
 // 2361-2592).  The mu0 and turning-point pieces (cached in v) are computed
-// only with FIRSTPT; the muf pieces on every call.
+// only with FIRSTPT; the muf pieces on every call.  No multiply-add
+// contraction here or in the mu part that sums these pieces: when a root of
+// M(mu) lies near mu = 1, n = mpos / (1 - mpos) reaches ~1e5 and
+// rf + n RJ / 3 cancels to ~1e-6 of its terms, so a fused or separately
+// rounded multiply-add would move phi by up to ~1e-8; without contraction
+// every kernel rounds these terms as the oracle does.
 template <bool FIRSTPT>
 __device__ __forceinline__ void calcphitmu(GkVars& v, bool sym, double a, double mneg, double mpos, double mu0,
                                   double muf, double muplus, double& phimuf, double& tmuf) {
+#pragma clang fp contract(off)
     double fac, m1, n, fac2;
     if (sym) {
         fac = fabs(a) * sqrt(mpos - mneg);
@@ -1864,7 +1870,8 @@ __device__ __forceinline__ PhiTime geophitime(GkVars& v, const double u0, const 
     }
 
     if (ncase > 4) {
-        // mu part from the roots of M(mu)
+        // mu part from the roots of M(mu) (no contraction: see calcphitmu)
+#pragma clang fp contract(off)
         const double amq = a * a - ql2;
         const double yy = -0.5 * (amq + sgn1(amq) * sqrt(amq * amq + 4.0 * q2 * a * a));
         double mneg, mpos;

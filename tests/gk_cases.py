@@ -19,11 +19,16 @@ DRIVER_OPTS = {
 }
 
 
-def _camera(rng, n, a=None):
+def _camera(rng, n, a=None, polar=False):
     g = np.zeros(n, O.GK_ARGS_DTYPE)
     aa = rng.uniform(0.05, 0.998, n) if a is None else np.full(n, a)
     mu0 = rng.uniform(0.05, 0.95, n)
     al, be = rng.uniform(-12, 12, (2, n))
+    if polar:
+        # |alpha| < 0.1 M, |beta| 8-12 M: L ~ 0, large Q2, so a root of M(mu)
+        # lies within ~1e-5 of mu = 1
+        al = rng.uniform(-0.1, 0.1, n)
+        be = rng.choice([-1.0, 1.0], n) * rng.uniform(8, 12, n)
     g["a"] = aa
     g["mu0"] = mu0
     g["l"] = -al * np.sqrt(1 - mu0 ** 2)
@@ -48,17 +53,20 @@ def _q2zero(rng, n):
     return g
 
 
-def geodesics(mode, n=96, seed=7):
+def geodesics(mode, n=96, seed=7, polar=False):
     """n geodesics with the per-mode targets: MUF between mu0 and the mu
     turning point ahead for USEGEOR, a final u in (0, u+) with TPR = 1 for
-    the u-sampled modes."""
+    the u-sampled modes.  polar: Kerr camera rays near the pole only."""
     rng = np.random.default_rng(seed)
-    parts = [_camera(rng, n // 2), _camera(rng, n // 8, a=0.0), _q2zero(rng, n // 8)]
-    out = _camera(rng, n - sum(len(p) for p in parts))
-    out["u0"] = out["uout"] = 0.3
-    out["su"] = -1.0
-    parts.append(out)
-    g = np.concatenate(parts)
+    if polar:
+        g = _camera(rng, n, polar=True)
+    else:
+        parts = [_camera(rng, n // 2), _camera(rng, n // 8, a=0.0), _q2zero(rng, n // 8)]
+        out = _camera(rng, n - sum(len(p) for p in parts))
+        out["u0"] = out["uout"] = 0.3
+        out["su"] = -1.0
+        parts.append(out)
+        g = np.concatenate(parts)
     uplus = 1.0 / (1.0 + np.sqrt(1.0 - g["a"] ** 2))
     if mode == "usegeor":
         # GEOR counts mu turning points against mu_+ (geokerr_original.f:
