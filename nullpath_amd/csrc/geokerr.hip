@@ -518,8 +518,9 @@ enum GEffect : int {
 // (the u0 integrals of rows 1, 2, 7, 8 and the mu0 / turning-point pieces
 // of calcphitmu), which depend only on the geodesic and the case: the
 // caller guarantees sample 1 ran in the case this sample is in.  Returns
-// the sample's GEffect (k >= 2).
-template <bool FIRST, bool FP>
+// the sample's GEffect (k >= 2).  ROW6 = false: the caller guarantees the
+// geodesic is not in Table 1 row 6 (its code is left out).
+template <bool FIRST, bool FP, bool ROW6 = true>
 __device__ __forceinline__ int g_step(const np_geokerr_opts& o, const GConst& c, GState& st, int k, GkSample& out) {
     const double MUN = c.mu0 + ((double)k - o.offset) * (c.MUF - c.mu0) / (double)o.nup;
     if (k >= 2) st.TPM = f_int((sgn1(c.mu0) * c.sm + 1.0) / 2.0);
@@ -530,15 +531,16 @@ __device__ __forceinline__ int g_step(const np_geokerr_opts& o, const GConst& c,
         if (FIRST && k == 1)
             geor_v<true, true>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su, c.sm);
         else
-            geor_v<false, true>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su, c.sm);
+            geor_v<false, true, ROW6>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su,
+                                      c.sm);
         if (k >= 2) {
             const double ee = -c.a * c.a * c.q2;   // GEOR's test for the quartic case
             if (st.U0 != u0_in || equal_root_cubic(st.v)) effect = kChained;
             else if (st.v.ncase != nc_in) effect = (ee != 0.0 && st.v.ncase == 0) ? kExit : kChained;
         }
         if (o.phit) {
-            const PhiTime p = geophitime<FP>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.TPM, st.TPR, c.su,
-                                             c.sm, st.IU);
+            const PhiTime p = geophitime<FP, ROW6>(st.v, st.U0, st.UF, c.mu0, MUN, c.a, c.l, c.q2, st.TPM, st.TPR,
+                                                   c.su, c.sm, st.IU);
             st.dt = p.tmu + p.tu;
             st.dph = p.phimu + p.phiu;
             st.lam = p.lambda;
@@ -602,10 +604,13 @@ gk_fg_setup_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_
 }
 
 // Lane (g, slot): samples k = slot + 1, slot + 1 + width, ... (k >= 2) of
-// geodesic g from the state after sample 1, and the never-written output
-// slots beyond NUP.  A sample that leaves the quartic case analysis records
-// itself in first_exit[g] (the lowest wins); a chained state marks the
-// geodesic for the serial walk.
+// geodesic g from the state after sample 1.  A sample that leaves the
+// quartic case analysis records itself in first_exit[g] (the lowest wins);
+// a chained state marks the geodesic for the serial walk.  ROW6 = false
+// runs the geodesics outside Table 1 row 6 (whose sextic root finders
+// would set this kernel's register budget) and writes the never-written
+// output slots beyond NUP; ROW6 = true runs the row-6 geodesics only.
+template <bool ROW6>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 gk_fg_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samples s,
                   const FgRecord* __restrict__ rec, int32_t* __restrict__ chained, int32_t* __restrict__ first_exit) {
@@ -615,14 +620,15 @@ gk_fg_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_sample
         const int64_t g = i / width;
         const int64_t slot = i - g * width;
         const size_t base = (size_t)g * (size_t)total;
-        for (int64_t j = slot; j < total; j += width)
-            if (j >= o.nup) gap_store(s, base + (size_t)j);
-        if (slot == 0) continue;
+        if (!ROW6)
+            for (int64_t j = slot; j < total; j += width)
+                if (j >= o.nup) gap_store(s, base + (size_t)j);
+        if (slot == 0 || (rec[g].s1.v.ncase == 6) != ROW6) continue;
         const GConst c = rec[g].c;
         for (int64_t k = slot + 1; k <= o.nup; k += width) {
             GState st = rec[g].s1;
             GkSample x;
-            const int effect = g_step<false, false>(o, c, st, (int)k, x);
+            const int effect = g_step<false, false, ROW6>(o, c, st, (int)k, x);
             full_store(s, base, total, x);
             if (effect == kExit) atomicMin(&first_exit[g], (int)k);
             else if (effect == kChained) chained[g] = 1;
@@ -665,9 +671,9 @@ gk_fg_restep_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samp
         const GConst c = rec[g].c;
         for (int64_t k = slot + 1; k <= o.nup; k += width) {
             if (k <= e) continue;
-            GState st = rec[g].se;
+            GState st = rec[g].se;   // NCASE = 0: no case's u integrals run
             GkSample x;
-            const int effect = g_step<false, false>(o, c, st, (int)k, x);
+            const int effect = g_step<false, false, false>(o, c, st, (int)k, x);
             full_store(s, base, total, x);
             if (effect != kNone) chained[g] = 1;
         }
@@ -760,7 +766,7 @@ __device__ __forceinline__ void u_first_solve(const np_geokerr_opts& o, const np
 // 3) or a GEOR step (phase 2), so that only that solver is inlined.
 enum UStepKind : int { kAnyStep = 0, kUStep = 1, kMuStep = 2 };
 
-template <bool ALLOW_FP, int KIND = kAnyStep>
+template <bool ALLOW_FP, int KIND = kAnyStep, bool ROW6 = true>
 __device__ __forceinline__ int u_step(const np_geokerr_opts& o, const UConst& c, UState& st, int64_t it, bool jb_in,
                                       double jb_mu, int jb_tpm, GkSample& out, bool& sticky) {
     int phase;
@@ -777,7 +783,8 @@ __device__ __forceinline__ int u_step(const np_geokerr_opts& o, const UConst& c,
         const double u0_in = st.U0;
         indep_muf(st.muminus, st.muplus, st.mustart, st.MUF, st.tpmstart, st.TPM1, c.sm, (int)k, o.npts, o.offset,
                   st.MUN, st.TPM);
-        geor_v<false, true>(st.v, st.U0, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su, c.sm);
+        geor_v<false, true, ROW6>(st.v, st.U0, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.IU, st.TPM, st.TPR, c.su,
+                                  c.sm);
         if (st.U0 != u0_in || equal_root_cubic(st.v)) sticky = true;
     } else {
         const double kk = (phase == 1) ? (double)((int64_t)c.kmax + o.kext + 1) : (double)k;
@@ -786,7 +793,8 @@ __device__ __forceinline__ int u_step(const np_geokerr_opts& o, const UConst& c,
         const double us_in = us;
         int tpmv = (phase == 1) ? st.TPM1 : st.TPM;
         int tprv = (phase == 0) ? st.TPR1 : st.TPR;
-        geomu<false, true>(st.v, us, st.UN, false, c.mu0, c.a, c.l, c.q2, tpmv, tprv, c.su, c.sm, st.MUN, st.IU);
+        geomu<false, true, false, ROW6>(st.v, us, st.UN, false, c.mu0, c.a, c.l, c.q2, tpmv, tprv, c.su, c.sm, st.MUN,
+                                        st.IU);
         if (us != us_in) sticky = true;
         if (phase == 0) st.U0 = us; else st.UF = us;
         if (phase == 1) st.TPM1 = tpmv; else st.TPM = tpmv;
@@ -808,9 +816,11 @@ __device__ __forceinline__ int u_step(const np_geokerr_opts& o, const UConst& c,
         const int tprv = (phase == 0) ? st.TPR1 : st.TPR;
         PhiTime p;
         if (ALLOW_FP && fp)
-            p = geophitime<true>(st.v, us, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.TPM, tprv, c.su, c.sm, st.IU);
+            p = geophitime<true, ROW6>(st.v, us, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.TPM, tprv, c.su, c.sm,
+                                       st.IU);
         else
-            p = geophitime<false>(st.v, us, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.TPM, tprv, c.su, c.sm, st.IU);
+            p = geophitime<false, ROW6>(st.v, us, st.UN, c.mu0, st.MUN, c.a, c.l, c.q2, st.TPM, tprv, c.su, c.sm,
+                                        st.IU);
         st.dt = p.tmu + p.tu;
         st.dph = p.phimu + p.phiu;
         st.lam = p.lambda;
@@ -961,8 +971,9 @@ gk_fu_setup_kernel(np_geokerr_opts o, np_geokerr_rays in, int64_t n, np_geokerr_
 // each from the state its leg started from, and writes the output slots no
 // step writes (width >= the row length).  MU = true: lane (g, slot) runs
 // MUFILL samples slot + 1, slot + 1 + width, ... (GEOR) from the state
-// after the far-side solve.
-template <bool MU>
+// after the far-side solve.  ROW6 = false runs the geodesics outside Table 1
+// row 6 (and writes the gaps); ROW6 = true the row-6 geodesics only.
+template <bool MU, bool ROW6>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 gk_fu_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samples s,
                   const FuRecord* __restrict__ rec, int32_t* __restrict__ flags) {
@@ -973,23 +984,25 @@ gk_fu_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_sample
         const int64_t slot = i - g * width;
         const size_t base = (size_t)g * (size_t)total;
         const UConst c = rec[g].c;
+        if (!MU && !ROW6)
+            for (int64_t j = slot; j < total; j += width)
+                if (u_gap(o, c, j + 1)) gap_store(s, base + (size_t)j);
+        if (!c.go || (rec[g].s0.v.ncase == 6) != ROW6) continue;
         bool sticky = false;
         if (MU) {
-            for (int64_t k = slot + 1; c.go && c.fill && k <= o.npts; k += width) {
+            for (int64_t k = slot + 1; c.fill && k <= o.npts; k += width) {
                 UState st = rec[g].s1;
                 GkSample x;
-                u_step<false, kMuStep>(o, c, st, c.nfirst + k, false, 0.0, 0, x, sticky);
+                u_step<false, kMuStep, ROW6>(o, c, st, c.nfirst + k, false, 0.0, 0, x, sticky);
                 if (u_last_writer(o, c, 2, x.idx)) full_store(s, base, total, x);
             }
         } else {
-            for (int64_t j = slot; j < total; j += width)
-                if (u_gap(o, c, j + 1)) gap_store(s, base + (size_t)j);
-            for (int64_t it = slot; c.go && it < c.nsteps; it += width) {
+            for (int64_t it = slot; it < c.nsteps; it += width) {
                 const int phase = (it < c.nfirst) ? 0 : (it < c.nfirst + c.nfill) ? ((it == c.nfirst) ? 1 : 2) : 3;
                 if (phase == 1 || phase == 2 || (phase == 0 && it == 0) || (phase == 3 && o.nup == 1)) continue;
                 UState st = (phase == 0) ? rec[g].s0 : rec[g].s2;
                 GkSample x;
-                u_step<false, kUStep>(o, c, st, it, false, 0.0, 0, x, sticky);
+                u_step<false, kUStep, ROW6>(o, c, st, it, false, 0.0, 0, x, sticky);
                 if (u_last_writer(o, c, phase, x.idx)) full_store(s, base, total, x);
             }
         }
@@ -1146,8 +1159,10 @@ int launch_split(const np_geokerr_opts& o, const np_geokerr_rays& r, int64_t n, 
         const dim3 lanes(blocks_for(n * width, 256, 32));
         FgRecord* R = (FgRecord*)rec;
         hipLaunchKernelGGL(gk_fg_setup_kernel, grid, dim3(256), 0, st, o, r, n, s, R, flags, first_exit);
-        hipLaunchKernelGGL(gk_fg_step_kernel, lanes, dim3(256), 0, st, o, n, width, s, (const FgRecord*)R, flags,
-                           first_exit);
+        hipLaunchKernelGGL(gk_fg_step_kernel<false>, lanes, dim3(256), 0, st, o, n, width, s, (const FgRecord*)R,
+                           flags, first_exit);
+        hipLaunchKernelGGL(gk_fg_step_kernel<true>, lanes, dim3(256), 0, st, o, n, width, s, (const FgRecord*)R,
+                           flags, first_exit);
         hipLaunchKernelGGL(gk_fg_exit_kernel, grid, dim3(256), 0, st, o, n, s, R, (const int32_t*)flags,
                            (const int32_t*)first_exit);
         hipLaunchKernelGGL(gk_fg_restep_kernel, lanes, dim3(256), 0, st, o, n, width, s, (const FgRecord*)R, flags,
@@ -1157,11 +1172,18 @@ int launch_split(const np_geokerr_opts& o, const np_geokerr_rays& r, int64_t n, 
         const int64_t width = (int64_t)o.nup + o.npts + 2;
         FuRecord* R = (FuRecord*)rec;
         hipLaunchKernelGGL(gk_fu_setup_kernel, grid, dim3(256), 0, st, o, r, n, s, R, flags);
-        hipLaunchKernelGGL(gk_fu_step_kernel<false>, dim3(blocks_for(n * width, 256, 32)), dim3(256), 0, st, o, n,
-                           width, s, (const FuRecord*)R, flags);
-        if (o.mufill && o.npts > 0)
-            hipLaunchKernelGGL(gk_fu_step_kernel<true>, dim3(blocks_for(n * o.npts, 256, 32)), dim3(256), 0, st, o,
-                               n, (int64_t)o.npts, s, (const FuRecord*)R, flags);
+        const dim3 lanes(blocks_for(n * width, 256, 32));
+        hipLaunchKernelGGL((gk_fu_step_kernel<false, false>), lanes, dim3(256), 0, st, o, n, width, s,
+                           (const FuRecord*)R, flags);
+        hipLaunchKernelGGL((gk_fu_step_kernel<false, true>), lanes, dim3(256), 0, st, o, n, width, s,
+                           (const FuRecord*)R, flags);
+        if (o.mufill && o.npts > 0) {
+            const dim3 fill(blocks_for(n * o.npts, 256, 32));
+            hipLaunchKernelGGL((gk_fu_step_kernel<true, false>), fill, dim3(256), 0, st, o, n, (int64_t)o.npts, s,
+                               (const FuRecord*)R, flags);
+            hipLaunchKernelGGL((gk_fu_step_kernel<true, true>), fill, dim3(256), 0, st, o, n, (int64_t)o.npts, s,
+                               (const FuRecord*)R, flags);
+        }
         hipLaunchKernelGGL(gk_full_u_kernel<true>, grid, dim3(256), 0, st, o, r, n, s, (const int32_t*)flags);
     }
     return hipGetLastError() == hipSuccess ? NP_OK : NP_ELAUNCH;

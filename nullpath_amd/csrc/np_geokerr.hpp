@@ -876,8 +876,10 @@ __device__ inline double calcimuf_rf(bool sym, double mneg, double mpos, double 
 // Fortran's by-reference arguments; `alias` reproduces a caller passing one
 // variable as both u0 and uf (geokerr_estimate_periods).  With FIRSTPT =
 // false the quartic roots, ncase, the u0 piece of rows 1/2/7/8 and the mu0
-// integrals come from v.
-template <bool FIRSTPT, bool PHT, bool MIXED = false>
+// integrals come from v.  ROW6 = false leaves Table 1 row 6 (no real root of
+// U, ncase 6) out of the generated code: the caller guarantees it is not
+// reached (its sextic root finders set the register budget of a kernel).
+template <bool FIRSTPT, bool PHT, bool MIXED = false, bool ROW6 = true>
 __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool alias, const double mu0, const double a,
                              const double l, const double q2, int& tpm, int& tpr, const double su,
                              const double sm, double& muf, double& iu) {
@@ -1031,6 +1033,7 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
                 iu = -su * ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, v.rffu0, uf, u0) /
                      sqrt(fabs(ee));
         } else if (ncase == 6) {
+          if constexpr (ROW6) {
             // the sextic for h1 (Eq. 21) is outside the Fortran's FIRSTPT test
             const double se = sqrt(ee);
             cplx co[7], hroots[6];
@@ -1049,6 +1052,7 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
             const double f2 = f1;
             if (uf > u0) iu = su * elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, v.rffu0, u0, uf) / sqrt(fabs(ee));
             else if (uf < u0) iu = -su * elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, v.rffu0, uf, u0) / sqrt(fabs(ee));
+          }
         } else {
             if (fabs(u3 - u2) > 1e-12) {
                 if (ncase == 7) {
@@ -1161,7 +1165,8 @@ struct GeorResult {
 // FIRSTPT off sets IU0 = 0 whenever u0 != u3 (:2795-2799); rows 3, 5, 6
 // and the q2 = 0 / forbidden-mu cases return uf = -1, ncase = 0.  The
 // forbidden-region retry runs at most 3 times (the Fortran's is unbounded).
-template <bool FIRSTPT, bool PHT>
+// ROW6 as in geomu.
+template <bool FIRSTPT, bool PHT, bool ROW6 = true>
 __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const double mu0, const double muf,
                                        const double a, const double l, const double q2, double& imu,
                                        const int tpm, int& tpr, const double su, const double sm) {
@@ -1381,6 +1386,7 @@ __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const 
             uf = (u4 * c5 + qs * u1 * c4 - (qs * u4 * c5 - u1 * c4) * cn) / ((c4 - qs * c5) * cn + (qs * c4 + c5));
             if (PHT) (void)ellquarticcomplex<0>(-u1, one, qs * u4, -qs * one, 0.0, 0.0, f, g, one, v.rffu0, u0, uf);
         } else if (v.ncase == 6) {
+          if constexpr (ROW6) {
             // Table 2 row 6 (:3028-3095): h1 and mn2 from two sextics
             const double se = sqrt(ee);
             cplx co[7], hr[6];
@@ -1437,6 +1443,7 @@ __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const 
             const double sc = sn / cn;
             uf = c3 + (n * (one + c2 * c2) * sc) / (one - c2 * sc);
             if (PHT) (void)elldoublecomplex<0>(f1, g1, h1, f2, g2, h2, 0.0, 0.0, v.rffu0, u0, uf);
+          }
         } else if (v.ncase == 7) {
             // Table 2 row 7 (:3098-3108)
             if (FIRSTPT && u0 != u2)
@@ -1674,7 +1681,8 @@ struct PhiTime {
 // separate leaf function its body outgrows the 16-bit branch range and the
 // compiler's long-branch sequences then overwrite s[30:31], the
 // return-address register (seen in the generated gfx950 code).
-template <bool FIRSTPT>
+// ROW6 as in geomu.
+template <bool FIRSTPT, bool ROW6 = true>
 __device__ __forceinline__ PhiTime geophitime(GkVars& v, const double u0, const double uf, const double mu0,
                                      const double muf, const double a, const double l, const double q2,
                                      const int tpm, const int tpr, const double su, const double sm,
@@ -1805,7 +1813,7 @@ __device__ __forceinline__ PhiTime geophitime(GkVars& v, const double u0, const 
                 tu2 = sg * ellquarticcomplex<-2>(-u1, one, u4 * qs, -one * qs, 0.0, one, f, g, one, v.rffu0, y, x);
                 tu3 = sg * ellquarticcomplex<-2>(-u1, one, u4 * qs, -one * qs, one, -one / uplus, f, g, one, v.rffu0, y, x);
                 tu4 = sg * ellquarticcomplex<-2>(-u1, one, u4 * qs, -one * qs, one, -umi, f, g, one, v.rffu0, y, x);
-            } else {
+            } else if constexpr (ROW6) {
                 const double h1 = v.h1, h2 = one / h1;
                 const double g1 = dd / ee / (h2 - h1), g2 = -g1;
                 const double f1 = one / sqrt(ee), f2 = f1;
@@ -1813,6 +1821,8 @@ __device__ __forceinline__ PhiTime geophitime(GkVars& v, const double u0, const 
                 tu2 = sg * elldoublecomplex<-2>(f1, g1, h1, f2, g2, h2, 0.0, one, v.rffu0, y, x);
                 tu3 = sg * elldoublecomplex<-2>(f1, g1, h1, f2, g2, h2, one, -one / uplus, v.rffu0, y, x);
                 tu4 = sg * elldoublecomplex<-2>(f1, g1, h1, f2, g2, h2, one, -umi, v.rffu0, y, x);
+            } else {
+                tu1 = tu2 = tu3 = tu4 = 0.0;
             }
             tu = su * ur / sqrt(-qs * ee) * eq47(a, l, uplus, umi, tu1, tu2, tu3, tu4);
             phiu = su * ur / sqrt(-qs * ee) *
