@@ -624,7 +624,7 @@ gk_fg_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_sample
             for (int64_t j = slot; j < total; j += width)
                 if (j >= o.nup) gap_store(s, base + (size_t)j);
         if (slot == 0 || (rec[g].s1.v.ncase == 6) != ROW6) continue;
-        const GConst c = rec[g].c;
+        const GConst& c = rec[g].c;
         for (int64_t k = slot + 1; k <= o.nup; k += width) {
             GState st = rec[g].s1;
             GkSample x;
@@ -668,7 +668,7 @@ gk_fg_restep_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_samp
         const int e = first_exit[g];
         if (e > o.nup || chained[g]) continue;
         const size_t base = (size_t)g * (size_t)total;
-        const GConst c = rec[g].c;
+        const GConst& c = rec[g].c;
         for (int64_t k = slot + 1; k <= o.nup; k += width) {
             if (k <= e) continue;
             GState st = rec[g].se;   // NCASE = 0: no case's u integrals run
@@ -707,6 +707,7 @@ struct FuRecord {
 // the driver's sample counts (:468-537).
 __device__ __forceinline__ void u_first_solve(const np_geokerr_opts& o, const np_geokerr_rays& in, int64_t g,
                                               UConst& c, UState& st) {
+#pragma clang fp contract(off)   // DU as the Fortran rounds it (see u_step)
     c = UConst{};
     st = UState{};
     c.uout = in.uout[g]; c.mu0 = in.mu0[g]; c.a = in.a[g]; c.l = in.l[g]; c.q2 = in.q2[g];
@@ -769,6 +770,12 @@ enum UStepKind : int { kAnyStep = 0, kUStep = 1, kMuStep = 2 };
 template <bool ALLOW_FP, int KIND = kAnyStep, bool ROW6 = true>
 __device__ __forceinline__ int u_step(const np_geokerr_opts& o, const UConst& c, UState& st, int64_t it, bool jb_in,
                                       double jb_mu, int jb_tpm, GkSample& out, bool& sticky) {
+    // The sample positions UN = UOUT + (K - OFFSET) DU are rounded as the
+    // Fortran rounds them (no fused multiply-add): a sample placed on the u
+    // turning point enters the integrals through sqrt(u2 - u), where one ulp
+    // of UN moves phi by ~1e-8, so every kernel running this step has to
+    // place it identically.
+#pragma clang fp contract(off)
     int phase;
     int64_t k;
     if (it < c.nfirst) { phase = 0; k = it + 1; }
@@ -983,7 +990,7 @@ gk_fu_step_kernel(np_geokerr_opts o, int64_t n, int64_t width, np_geokerr_sample
         const int64_t g = i / width;
         const int64_t slot = i - g * width;
         const size_t base = (size_t)g * (size_t)total;
-        const UConst c = rec[g].c;
+        const UConst& c = rec[g].c;
         if (!MU && !ROW6)
             for (int64_t j = slot; j < total; j += width)
                 if (u_gap(o, c, j + 1)) gap_store(s, base + (size_t)j);

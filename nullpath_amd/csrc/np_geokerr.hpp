@@ -23,6 +23,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Every FP64 product and sum below is rounded on its own (no fused
+// multiply-add), as in the Fortran and the oracle.  GEOKERR's results hinge
+// on comparisons of sample positions against turning points and on
+// cancelling sums (eq. 48 near a = 1, ellphitmu near the pole); with fusion
+// left to the compiler, two kernels inlining this same source rounded them
+// differently and a sample on a turning point moved by 1e-8 or became NaN.
+// Fast contraction is restored at the end of this header.
+#pragma clang fp contract(off)
+
 namespace np {
 namespace gk {
 
@@ -872,6 +881,22 @@ __device__ inline double calcimuf_rf(bool sym, double mneg, double mpos, double 
     return rf_d(1.0 - sf * sf, qf, 1.0);
 }
 
+// The coefficients of U(u) = 1 + cc u^2 + dd u^3 + ee u^4 and the pieces
+// beside them, shared by GEOMU and GEOR (the turning points derive from them).
+struct UCoefs {
+    double uplus, l2, ql2, cc, dd, ee;
+};
+__device__ __forceinline__ UCoefs u_coefs(double a, double l, double q2) {
+    UCoefs k;
+    k.uplus = 1.0 / (1.0 + sqrt(1.0 - a * a));
+    k.l2 = l * l;
+    k.ql2 = q2 + k.l2;
+    k.cc = a * a - q2 - k.l2;
+    k.dd = 2.0 * ((a - l) * (a - l) + q2);
+    k.ee = -a * a * q2;
+    return k;
+}
+
 // GEOMU (geokerr_original.f:771-1186).  u0, uf, tpm and tpr are the
 // Fortran's by-reference arguments; `alias` reproduces a caller passing one
 // variable as both u0 and uf (geokerr_estimate_periods).  With FIRSTPT =
@@ -884,12 +909,8 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
                              const double l, const double q2, int& tpm, int& tpr, const double su,
                              const double sm, double& muf, double& iu) {
     const double one = 1.0, two = 2.0, third = 0.3333333333333333;
-    const double uplus = one / (one + sqrt(one - a * a));
-    const double l2 = l * l;
-    const double ql2 = q2 + l2;
-    const double cc = a * a - q2 - l2;
-    const double dd = two * ((a - l) * (a - l) + q2);
-    const double ee = -a * a * q2;
+    const UCoefs kc = u_coefs(a, l, q2);
+    const double uplus = kc.uplus, l2 = kc.l2, ql2 = kc.ql2, cc = kc.cc, dd = kc.dd, ee = kc.ee;
     double iu1 = 0.0;
     int ncase = v.ncase;
     double u1 = v.u1, u2 = v.u2, u3 = v.u3, u4 = v.u4;
@@ -899,6 +920,9 @@ __device__ __forceinline__ void geomu(GkVars& v, double& u0, double& uf, bool al
     muf = 0.0;
 
     if (ee == 0.0 && dd != 0.0) {
+        // The cubic's roots are formed again on every call (not only with
+        // FIRSTPT); a sample the driver places on the turning point u2 moves
+        // mu by ~1e-8 per ulp of u2.
         const double qq = cc * cc / dd / dd / 9.0;
         const double rr = (two * (cc * cc * cc) / (dd * dd * dd) + 27.0 / dd) / 54.0;
         const double dis = rr * rr - qq * qq * qq;
@@ -1171,12 +1195,8 @@ __device__ __forceinline__ void geor_v(GkVars& v, double& u0, double& uf, const 
                                        const double a, const double l, const double q2, double& imu,
                                        const int tpm, int& tpr, const double su, const double sm) {
     const double one = 1.0, two = 2.0, half = 0.5, third = 0.3333333333333333, three = 3.0;
-    const double uplus = one / (one + sqrt(one - a * a));
-    const double l2 = l * l;
-    const double ql2 = q2 + l2;
-    const double cc = a * a - q2 - l2;
-    const double dd = two * ((a - l) * (a - l) + q2);
-    const double ee = -a * a * q2;
+    const UCoefs kc = u_coefs(a, l, q2);
+    const double uplus = kc.uplus, l2 = kc.l2, ql2 = kc.ql2, cc = kc.cc, dd = kc.dd, ee = kc.ee;
     const unsigned utpm = (unsigned)tpm;
     double dummy = 0.0, sn, cn, dn;
 
@@ -1594,16 +1614,13 @@ __device__ inline double tfnkerr(double u0, double uf, double u1, double u2, dou
 
 // calcphitmusym / calcphitmuasym with ellphitmu (geokerr_original.f:
 // 2361-2592).  The mu0 and turning-point pieces (cached in v) are computed
-// only with FIRSTPT; the muf pieces on every call.  No multiply-add
-// contraction here or in the mu part that sums these pieces: when a root of
-// M(mu) lies near mu = 1, n = mpos / (1 - mpos) reaches ~1e5 and
-// rf + n RJ / 3 cancels to ~1e-6 of its terms, so a fused or separately
-// rounded multiply-add would move phi by up to ~1e-8; without contraction
-// every kernel rounds these terms as the oracle does.
+// only with FIRSTPT; the muf pieces on every call.  When a root of M(mu)
+// lies near mu = 1, n = mpos / (1 - mpos) reaches ~1e5 and rf + n RJ / 3
+// cancels to ~1e-6 of its terms: one of the sums whose rounding the
+// header-wide rule (no fused multiply-add) keeps the same in every kernel.
 template <bool FIRSTPT>
 __device__ __forceinline__ void calcphitmu(GkVars& v, bool sym, double a, double mneg, double mpos, double mu0,
                                   double muf, double muplus, double& phimuf, double& tmuf) {
-#pragma clang fp contract(off)
     double fac, m1, n, fac2;
     if (sym) {
         fac = fabs(a) * sqrt(mpos - mneg);
@@ -1677,7 +1694,9 @@ struct PhiTime {
     double phimu, tmu, phiu, tu, lambda;
 };
 
-// GEOPHITIME (geokerr_original.f:1648-2360).  Always inlined: as a
+// GEOPHITIME (geokerr_original.f:1648-2360).  Near a = 1 the two
+// horizon-pole integrals tu3 and tu4 nearly cancel in eq. 48 and are scaled
+// by 1 / (2 sqrt(1 - a^2)).  Always inlined: as a
 // separate leaf function its body outgrows the 16-bit branch range and the
 // compiler's long-branch sequences then overwrite s[30:31], the
 // return-address register (seen in the generated gfx950 code).
@@ -1880,8 +1899,7 @@ __device__ __forceinline__ PhiTime geophitime(GkVars& v, const double u0, const 
     }
 
     if (ncase > 4) {
-        // mu part from the roots of M(mu) (no contraction: see calcphitmu)
-#pragma clang fp contract(off)
+        // mu part from the roots of M(mu)
         const double amq = a * a - ql2;
         const double yy = -0.5 * (amq + sgn1(amq) * sqrt(amq * amq + 4.0 * q2 * a * a));
         double mneg, mpos;
@@ -1918,3 +1936,5 @@ __device__ __forceinline__ PhiTime geophitime(GkVars& v, const double u0, const 
 
 }  // namespace gk
 }  // namespace np
+
+#pragma clang fp contract(fast)
